@@ -1,0 +1,308 @@
+"""fyfft_amd -- MI355X-native batched 1-D complex FFT.
+
+Python binding over the C ABI of ``fyfft_amd/lib/libfyfft.so``
+(declared in ``include/fyfft.h``).  PyTorch is used only for device memory
+and streams; every transform runs in the library's HIP kernels.  There is no
+CPU fallback: if the library is missing, loading raises.
+
+Mirror of the reference interface (kjear/fyFFT, ``foye_FFT.hpp``):
+
+* :class:`FFT1D_C2C_invoker` -- ``FFT1D_C2C_invoker(N)`` +
+  ``dispatch(input, output, invert)`` on host (numpy) buffers with the
+  reference's synchronous semantics (``foye_FFT.hpp:101-151``), and
+  ``acquired_execution_strategy(N)`` (``foye_FFT.hpp:162-184``).
+* :class:`Plan` -- the device-pointer fast path: batched, stream-ordered.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfyfft.so")
+
+min_input_length = 2          # foye_FFT.hpp:86
+max_input_length = 80_000     # foye_FFT.hpp:87 (the reference's limit)
+MAX_LENGTH = 1 << 26          # this library's limit (FYFFT_MAX_LENGTH)
+
+FORWARD, INVERSE, ADJOINT = 0, 1, 2
+F32, F64, BF16, F16 = 0, 1, 2, 3
+
+
+class FFT_execution_strategy(enum.IntEnum):
+    """Same order as the reference enum (foye_FFT.hpp:153-160)."""
+    kernel_direct = 0
+    mix_radix = 1
+    rader = 2
+    bluestein = 3
+    DFT = 4
+
+
+_STATUS = {0: "success", 1: "invalid argument", 2: "invalid length", 3: "unsupported",
+           4: "allocation failed", 5: "device error", 6: "invalid plan", 7: "no device"}
+
+
+class FyfftError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        detail = ""
+        try:
+            detail = _load().fyfft_last_error().decode()
+        except Exception:  # pragma: no cover - library itself failed to load
+            pass
+        super().__init__(f"{what}: {_STATUS.get(status, status)}" + (f" ({detail})" if detail else ""))
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def _load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"fyfft_amd: native library not built ({LIB_PATH}); run "
+                "`python __graft_entry__.py` or `python fyfft_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i, ip = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_int)
+        L.fyfft_plan_create.argtypes = [ctypes.POINTER(vp), sz, sz, i]
+        L.fyfft_plan_destroy.argtypes = [vp]
+        L.fyfft_execute.argtypes = [vp, vp, vp, i, vp]
+        L.fyfft_execute_host.argtypes = [vp, vp, vp, i]
+        L.fyfft_plan_strategy.argtypes = [vp, ip]
+        L.fyfft_length_strategy.argtypes = [sz, ip]
+        L.fyfft_length_radices.argtypes = [sz, ip, i, ip]
+        L.fyfft_plan_workspace_bytes.argtypes = [vp, ctypes.POINTER(sz)]
+        L.fyfft_plan_describe.argtypes = [vp, ctypes.c_char_p, sz]
+        L.fyfft_status_string.argtypes = [i]
+        L.fyfft_status_string.restype = ctypes.c_char_p
+        L.fyfft_last_error.argtypes = []
+        L.fyfft_last_error.restype = ctypes.c_char_p
+        for name in ("fyfft_plan_create", "fyfft_plan_destroy", "fyfft_execute",
+                     "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
+                     "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe"):
+            getattr(L, name).restype = i
+        _lib = L
+        return L
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libfyfft.so (raises ImportError if it was never built)."""
+    return _load()
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise FyfftError(status, what)
+
+
+# ------------------------------------------------------------------ queries
+def length_strategy(n: int) -> FFT_execution_strategy:
+    out = ctypes.c_int()
+    _check(_load().fyfft_length_strategy(n, ctypes.byref(out)), f"strategy(N={n})")
+    return FFT_execution_strategy(out.value)
+
+
+def length_radices(n: int) -> list[int]:
+    buf = (ctypes.c_int * 32)()
+    cnt = ctypes.c_int()
+    _check(_load().fyfft_length_radices(n, buf, 32, ctypes.byref(cnt)), f"radices(N={n})")
+    return [buf[k] for k in range(cnt.value)]
+
+
+def _precision_of(dtype) -> int:
+    import torch
+    if dtype == torch.complex64:
+        return F32
+    if dtype == torch.complex128:
+        return F64
+    raise TypeError(f"fyfft_amd: unsupported dtype {dtype} (complex64 / complex128)")
+
+
+# ------------------------------------------------------------------ plan
+class Plan:
+    """A batched device plan: ``batch`` contiguous transforms of length ``n``.
+
+    ``execute(x, out, direction)`` takes torch tensors on the plan's device
+    whose last dimension holds the transforms; it enqueues on the current
+    torch stream (or ``stream``) and does not synchronise.
+    """
+
+    def __init__(self, n: int, batch: int = 1, precision: int = F32):
+        self.n, self.batch, self.precision = int(n), int(batch), int(precision)
+        self._h = ctypes.c_void_p()
+        _check(_load().fyfft_plan_create(ctypes.byref(self._h), self.n, self.batch, self.precision),
+               f"plan_create(N={n}, batch={batch}, precision={precision})")
+
+    def close(self) -> None:
+        if self._h:
+            _load().fyfft_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def strategy(self) -> FFT_execution_strategy:
+        out = ctypes.c_int()
+        _check(_load().fyfft_plan_strategy(self._h, ctypes.byref(out)), "plan_strategy")
+        return FFT_execution_strategy(out.value)
+
+    @property
+    def workspace_bytes(self) -> int:
+        out = ctypes.c_size_t()
+        _check(_load().fyfft_plan_workspace_bytes(self._h, ctypes.byref(out)), "workspace_bytes")
+        return out.value
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        _check(_load().fyfft_plan_describe(self._h, buf, 512), "describe")
+        return buf.value.decode()
+
+    def _check_tensor(self, t, name):
+        import torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError(f"{name} must be a device tensor")
+        if _precision_of(t.dtype) != self.precision:
+            raise TypeError(f"{name} dtype {t.dtype} does not match the plan precision")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        if t.numel() != self.n * self.batch:
+            raise ValueError(f"{name} has {t.numel()} elements, plan needs {self.n * self.batch}")
+
+    def execute_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0) -> None:
+        _check(_load().fyfft_execute(self._h, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
+                                     int(direction), ctypes.c_void_p(stream)), "execute")
+
+    def execute(self, x, out=None, direction: int = FORWARD, stream=None):
+        import torch
+        self._check_tensor(x, "input")
+        if out is None:
+            out = torch.empty_like(x)
+        self._check_tensor(out, "output")
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+        self.execute_ptr(x.data_ptr(), out.data_ptr(), direction, handle)
+        return out
+
+    def execute_host(self, x: np.ndarray, out: np.ndarray | None = None, direction: int = FORWARD):
+        dt = np.complex64 if self.precision == F32 else np.complex128
+        x = np.ascontiguousarray(x, dtype=dt)
+        if x.size != self.n * self.batch:
+            raise ValueError("host buffer size does not match the plan")
+        if out is None:
+            out = np.empty_like(x)
+        _check(_load().fyfft_execute_host(self._h, ctypes.c_void_p(x.ctypes.data),
+                                          ctypes.c_void_p(out.ctypes.data), int(direction)),
+               "execute_host")
+        return out
+
+
+_plan_cache: dict = {}
+_plan_cache_lock = threading.Lock()
+
+
+def _cached_plan(n: int, batch: int, precision: int, device: int) -> Plan:
+    import torch
+    key = (n, batch, precision, device)
+    with _plan_cache_lock:
+        p = _plan_cache.get(key)
+        if p is None:
+            with torch.cuda.device(device):
+                p = Plan(n, batch, precision)
+            if len(_plan_cache) > 64:
+                _plan_cache.clear()
+            _plan_cache[key] = p
+        return p
+
+
+def _run(x, direction: int):
+    import torch
+    if not x.is_cuda:
+        raise TypeError("fyfft_amd.fft: input must be a device tensor")
+    if x.dtype not in (torch.complex64, torch.complex128):
+        raise TypeError("fyfft_amd.fft: complex64 / complex128 input expected")
+    x = x.contiguous()
+    n = x.shape[-1]
+    batch = x.numel() // n if n else 0
+    p = _cached_plan(n, batch, _precision_of(x.dtype), x.device.index or 0)
+    return p.execute(x, torch.empty_like(x), direction)
+
+
+def fft(x):
+    """Forward C2C FFT over the last dimension (unnormalised)."""
+    return _run(x, FORWARD)
+
+
+def ifft(x):
+    """Inverse C2C FFT over the last dimension (scaled by 1/N)."""
+    return _run(x, INVERSE)
+
+
+def adjoint(x):
+    """Adjoint of the forward transform (= N * ifft)."""
+    return _run(x, ADJOINT)
+
+
+# ------------------------------------------------------------------ drop-in mirror
+class FFT1D_C2C_invoker:
+    """Host-pointer drop-in for ``fy::fft::FFT1D_C2C_invoker``
+    (foye_FFT.hpp:89-185).
+
+    ``dispatch(input, output, invert)`` transforms one length-N sequence of
+    interleaved (re, im) float32 values, synchronously, like
+    ``dispatch<invert>(const float*, float*)``.  ``input``/``output`` may be
+    float32 arrays of 2N values or complex64 arrays of N values; in place is
+    allowed.  Out-of-range N raises ``ValueError`` where the reference has
+    undefined behaviour (foye_FFT.hpp:103-104).
+    """
+
+    FFT_execution_strategy = FFT_execution_strategy
+
+    def __init__(self, length: int):
+        n = int(length)
+        if not (min_input_length <= n <= MAX_LENGTH):
+            raise ValueError(f"FFT1D_C2C_invoker: N={n} outside [{min_input_length}, {MAX_LENGTH}]")
+        self.N = n
+        self._plan = Plan(n, 1, F32)
+
+    def dispatch(self, input, output, invert: bool = False) -> None:
+        src = np.asarray(input)
+        dst = output
+        if src.dtype == np.float32:
+            src = src.view(np.complex64)
+        if dst.dtype == np.float32:
+            dst_c = dst.view(np.complex64)
+        else:
+            dst_c = dst
+        if src.size != self.N or dst_c.size != self.N:
+            raise ValueError("dispatch: buffers must hold N complex values")
+        if dst_c.dtype != np.complex64 or not dst_c.flags.c_contiguous:
+            raise TypeError("dispatch: output must be contiguous float32/complex64")
+        src = np.ascontiguousarray(src, dtype=np.complex64)
+        _check(_load().fyfft_execute_host(self._plan._h, ctypes.c_void_p(src.ctypes.data),
+                                          ctypes.c_void_p(dst_c.ctypes.data),
+                                          INVERSE if invert else FORWARD), "dispatch")
+
+    @staticmethod
+    def acquired_execution_strategy(length: int) -> FFT_execution_strategy:
+        return length_strategy(int(length))

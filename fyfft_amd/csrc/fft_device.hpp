@@ -1,0 +1,327 @@
+// Device-side building blocks of the MI355X FFT: complex helpers, radix
+// 2/3/4/5/7/8/16 DFT codelets held in registers, and the Stockham
+// (self-sorting) pass that moves one transform between LDS / HBM and
+// registers.
+//
+// Pass formulation.  With N = R_0 * R_1 * ... * R_{P-1}, pass p has radix
+// R = R_p and span Ls = R_0 * ... * R_{p-1} (Ls = 1 for pass 0).  Butterfly
+// j in [0, N/R) with k = j mod Ls reads x[j + r*N/R] (r < R), multiplies
+// input r by w_{Ls*R}^{k*r}, runs an R-point DFT and writes
+// y[(j-k)*R + k + r*Ls].  After the last pass the data is in natural order,
+// so there is no digit-reversal step (the reference's direct invokers
+// permute first: radix2.hpp:308-425, radix4.hpp:505-553).  Pass 0 reads its
+// inputs with stride N/R and the last pass writes with stride N/R, so both
+// touch HBM fully coalesced.
+//
+// Twiddle table layout (one per plan, FP64-generated): the factors of pass
+// p start at element Ls_p - 1 and hold w_{Ls*R}^{k*r} at
+// [Ls - 1 + k*(R-1) + (r-1)], k < Ls, 1 <= r < R.  The slices telescope to
+// N - 1 entries in total.  Forward uses the table, inverse/adjoint its
+// conjugate.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fyfft {
+namespace dev {
+
+template <typename V> struct scalar_of;
+template <> struct scalar_of<float2> { using type = float; };
+template <> struct scalar_of<double2> { using type = double; };
+template <typename V> using S = typename scalar_of<V>::type;
+
+template <typename V> __device__ __forceinline__ V cadd(V a, V b) { return V{a.x + b.x, a.y + b.y}; }
+template <typename V> __device__ __forceinline__ V csub(V a, V b) { return V{a.x - b.x, a.y - b.y}; }
+template <typename V> __device__ __forceinline__ V cscale(V a, S<V> s) { return V{a.x * s, a.y * s}; }
+template <typename V> __device__ __forceinline__ V cmul(V a, V b) {
+  return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename V> __device__ __forceinline__ V cmulc(V a, V b) {
+  return V{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+// a * w when DIR < 0 (forward), a * conj(w) when DIR > 0
+template <int DIR, typename V> __device__ __forceinline__ V twmul(V a, V w) {
+  if constexpr (DIR < 0) return cmul(a, w); else return cmulc(a, w);
+}
+// a * (DIR * i): multiplication by exp(DIR * i*pi/2)
+template <int DIR, typename V> __device__ __forceinline__ V rot90(V a) {
+  if constexpr (DIR < 0) return V{a.y, -a.x}; else return V{-a.y, a.x};
+}
+// a * (c + DIR*i*s)
+template <int DIR, typename V> __device__ __forceinline__ V rotc(V a, double c, double s) {
+  const S<V> cc = (S<V>)c, ss = (S<V>)(DIR < 0 ? -s : s);
+  return V{a.x * cc - a.y * ss, a.x * ss + a.y * cc};
+}
+
+// cos/sin(2*pi*j/R) for the odd prime radices, exact decimal expansions.
+template <int R> __device__ __forceinline__ constexpr double cos_r(int j) {
+  j %= R;
+  if constexpr (R == 3) {
+    return j == 0 ? 1.0 : -0.5;
+  } else if constexpr (R == 5) {
+    return j == 0 ? 1.0 : (j == 1 || j == 4) ? 0.30901699437494742410229341718281906
+                                             : -0.80901699437494742410229341718281906;
+  } else {
+    return j == 0 ? 1.0
+         : (j == 1 || j == 6) ? 0.62348980185873353052500488400423981
+         : (j == 2 || j == 5) ? -0.22252093395631440428890256449679476
+                              : -0.90096886790241912623610231950744505;
+  }
+}
+template <int R> __device__ __forceinline__ constexpr double sin_r(int j) {
+  j %= R;
+  if constexpr (R == 3) {
+    return j == 0 ? 0.0 : j == 1 ? 0.86602540378443864676372317075293618
+                                 : -0.86602540378443864676372317075293618;
+  } else if constexpr (R == 5) {
+    return j == 0 ? 0.0
+         : j == 1 ? 0.95105651629515357211643933337938214
+         : j == 2 ? 0.58778525229247312916870595463907277
+         : j == 3 ? -0.58778525229247312916870595463907277
+                  : -0.95105651629515357211643933337938214;
+  } else {
+    return j == 0 ? 0.0
+         : j == 1 ? 0.78183148246802980870844452667405775
+         : j == 2 ? 0.97492791218182360701813168299393122
+         : j == 3 ? 0.43388373911755812047576833284835875
+         : j == 4 ? -0.43388373911755812047576833284835875
+         : j == 5 ? -0.97492791218182360701813168299393122
+                  : -0.78183148246802980870844452667405775;
+  }
+}
+
+constexpr double kSqrtHalf = 0.70710678118654752440084436210484904;
+constexpr double kCos8 = 0.92387953251128675612818318939678829;   // cos(pi/8)
+constexpr double kSin8 = 0.38268343236508977172845998403039887;   // sin(pi/8)
+
+// ---------------------------------------------------------------- codelets
+// In-place R-point DFT of v[0..R) with sign DIR (-1 forward, +1 inverse),
+// natural order in and out, no scaling.
+template <int R, int DIR, typename V> struct Dft;
+
+template <int DIR, typename V> struct Dft<2, DIR, V> {
+  static __device__ __forceinline__ void run(V* v) {
+    V a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <int DIR, typename V> struct Dft<4, DIR, V> {
+  static __device__ __forceinline__ void run(V* v) {
+    V s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+    V s1 = cadd(v[1], v[3]), d1 = rot90<DIR>(csub(v[1], v[3]));
+    v[0] = cadd(s0, s1);
+    v[2] = csub(s0, s1);
+    v[1] = cadd(d0, d1);
+    v[3] = csub(d0, d1);
+  }
+};
+
+template <int DIR, typename V> struct Dft<8, DIR, V> {
+  static __device__ __forceinline__ void run(V* v) {
+    V e[4] = {v[0], v[2], v[4], v[6]};
+    V o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, DIR, V>::run(e);
+    Dft<4, DIR, V>::run(o);
+    o[1] = rotc<DIR>(o[1], kSqrtHalf, kSqrtHalf);
+    o[2] = rot90<DIR>(o[2]);
+    o[3] = rotc<DIR>(o[3], -kSqrtHalf, kSqrtHalf);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// w16^m multiply for m = n2*k1 with n2, k1 in 1..3
+template <int DIR, typename V> __device__ __forceinline__ V w16(V a, int m) {
+  switch (m) {
+    case 1: return rotc<DIR>(a, kCos8, kSin8);
+    case 2: return rotc<DIR>(a, kSqrtHalf, kSqrtHalf);
+    case 3: return rotc<DIR>(a, kSin8, kCos8);
+    case 4: return rot90<DIR>(a);
+    case 6: return rotc<DIR>(a, -kSqrtHalf, kSqrtHalf);
+    case 9: return rotc<DIR>(a, -kCos8, -kSin8);
+    default: return a;
+  }
+}
+
+// 16 = 4 x 4: x[4*n1 + n2]; DFT4 over n1, twiddle w16^(n2*k1), DFT4 over n2,
+// output X[k1 + 4*k2].
+template <int DIR, typename V> struct Dft<16, DIR, V> {
+  static __device__ __forceinline__ void run(V* v) {
+    V y[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      V t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Dft<4, DIR, V>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2 * 4 + k1] = (n2 && k1) ? w16<DIR>(t[k1], n2 * k1) : t[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      V t[4] = {y[k1], y[4 + k1], y[8 + k1], y[12 + k1]};
+      Dft<4, DIR, V>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
+// Odd prime radix by symmetric pairs:
+//   X_k = x0 + sum_m (x_m + x_{R-m}) cos(2pi mk/R) + DIR*i * sum_m (x_m - x_{R-m}) sin(2pi mk/R)
+template <int R, int DIR, typename V> struct DftOdd {
+  static __device__ __forceinline__ void run(V* v) {
+    constexpr int H = (R - 1) / 2;
+    V s[H], d[H];
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      s[m - 1] = cadd(v[m], v[R - m]);
+      d[m - 1] = csub(v[m], v[R - m]);
+    }
+    const V x0 = v[0];
+    V sum = x0;
+#pragma unroll
+    for (int m = 0; m < H; ++m) sum = cadd(sum, s[m]);
+    v[0] = sum;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      V a = x0, b = V{0, 0};
+#pragma unroll
+      for (int m = 1; m <= H; ++m) {
+        const S<V> c = (S<V>)cos_r<R>(m * k), sn = (S<V>)sin_r<R>(m * k);
+        a = V{a.x + s[m - 1].x * c, a.y + s[m - 1].y * c};
+        b = V{b.x + d[m - 1].x * sn, b.y + d[m - 1].y * sn};
+      }
+      const V ib = rot90<DIR>(b);
+      v[k] = cadd(a, ib);
+      v[R - k] = csub(a, ib);
+    }
+  }
+};
+template <int DIR, typename V> struct Dft<3, DIR, V> : DftOdd<3, DIR, V> {};
+template <int DIR, typename V> struct Dft<5, DIR, V> : DftOdd<5, DIR, V> {};
+template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
+
+// ---------------------------------------------------------------- LDS layout
+// One pad element every 16 complex values breaks the power-of-two strides
+// of the pass read/write patterns across LDS banks.
+constexpr int kPadShift = 4;
+__host__ __device__ __forceinline__ constexpr int lds_len(int n, int ps) { return (n - 1) + ((n - 1) >> ps) + 1; }
+
+constexpr int kMaxPasses = 16;
+struct PassDesc {
+  int npass;
+  int radix[kMaxPasses];
+};
+
+__host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------- pass
+// One Stockham pass of radix R over the transform of length N held by T
+// threads (this thread is `tid`).  SRC_G / DST_G select global (HBM) or LDS
+// addressing (LDS index i is stored at i + (i >> ps)).  Each thread owns at most MAXB
+// butterflies.  When source and destination are both LDS the pass is in
+// place: every read completes (barrier) before any write.  A barrier
+// follows LDS writes so the next pass sees them.  `scale` is applied only
+// on global stores.
+template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G>
+__device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int Ls, int T, int tid,
+                                              const V* __restrict__ tw, bool active, S<V> scale,
+                                              int ps) {
+  const int nb = N / R;
+  V v[MAXB][R];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int j = tid + b * T;
+    if (active && j < nb) {
+      const int k = (Ls == 1) ? 0 : j % Ls;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int idx = j + r * nb;
+        v[b][r] = SRC_G ? src[idx] : src[idx + (idx >> ps)];
+      }
+      if (Ls > 1) {
+        const V* t = tw + (Ls - 1) + k * (R - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = twmul<DIR>(v[b][r], t[r - 1]);
+      }
+      Dft<R, DIR, V>::run(v[b]);
+    }
+  }
+  if (!SRC_G && !DST_G) __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int j = tid + b * T;
+    if (active && j < nb) {
+      const int k = (Ls == 1) ? 0 : j % Ls;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int idx = base + r * Ls;
+        if (DST_G) dst[idx] = cscale(v[b][r], scale);
+        else dst[idx + (idx >> ps)] = v[b][r];
+      }
+    }
+  }
+  if (!DST_G) __syncthreads();
+}
+
+// One LDS -> LDS pass kept out of line: inlining every radix case into the
+// pass loop of lds_fft makes the register allocator keep all cases' arrays
+// live at once and spill; as separate functions each gets its own budget.
+template <typename V, int R, int MAXB, int DIR>
+__device__ __noinline__ void lds_pass_ool(V* lds, int N, int Ls, int T, int tid,
+                                          const V* __restrict__ tw, bool active, int ps) {
+  stockham_pass<V, R, MAXB, DIR, false, false>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
+}
+
+// Run every pass of `pd` on an LDS-resident transform (LDS -> LDS).
+// PPT bounds the points per thread (N <= PPT * T).
+template <typename V, int PPT, int DIR>
+__device__ __forceinline__ void lds_fft(V* lds, int N, int T, int tid, const PassDesc& pd,
+                                        const V* __restrict__ tw, bool active, int ps) {
+  int Ls = 1;
+  for (int p = 0; p < pd.npass; ++p) {
+    const int R = pd.radix[p];
+    switch (R) {
+      case 2: lds_pass_ool<V, 2, cdiv(PPT, 2), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 3: lds_pass_ool<V, 3, cdiv(PPT, 3), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 4: lds_pass_ool<V, 4, cdiv(PPT, 4), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 5: lds_pass_ool<V, 5, cdiv(PPT, 5), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 7: lds_pass_ool<V, 7, cdiv(PPT, 7), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 8: lds_pass_ool<V, 8, cdiv(PPT, 8), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      default: lds_pass_ool<V, 16, cdiv(PPT, 16), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+    }
+    Ls *= R;
+  }
+}
+
+// ---------------------------------------------------------------- roots
+// exp(sign * 2*pi*i * m / D) in FP64 with an exact integer reduction of m to
+// a quadrant, so the only rounding is in r/D and in sincospi (|angle| <=
+// pi/2 after reduction).
+__device__ __forceinline__ double2 root_fp64(long long m, long long D, int sign) {
+  m %= D;
+  if (m < 0) m += D;
+  const long long m4 = 4 * m;
+  const long long q = m4 / D;          // quadrant 0..3
+  const long long r = m4 - q * D;      // 0 <= r < D
+  double s, c;
+  sincospi(0.5 * ((double)r / (double)D), &s, &c);
+  double re, im;  // exp(+i*(q*pi/2 + theta))
+  switch (q) {
+    case 0: re = c; im = s; break;
+    case 1: re = -s; im = c; break;
+    case 2: re = -c; im = -s; break;
+    default: re = s; im = -c; break;
+  }
+  return double2{re, sign < 0 ? -im : im};
+}
+
+}  // namespace dev
+}  // namespace fyfft

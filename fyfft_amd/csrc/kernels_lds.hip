@@ -1,0 +1,162 @@
+// LDS-resident batched transforms.
+//
+// k_lds_generic: any 2/3/5/7-smooth N that fits one workgroup's LDS.  The
+//   pass list is a kernel argument; each pass switches on its radix.  HBM is
+//   touched once per element in each direction (coalesced copy in, passes in
+//   LDS, coalesced copy out with the 1/N scale folded in).
+// k_lds_fixed: the same passes with N, radices and geometry fixed at compile
+//   time, for the lengths that carry the headline workloads.  Pass 0 reads
+//   HBM straight into registers and the last pass writes HBM straight from
+//   registers, so a P-pass transform costs P-1 LDS round trips.
+#include <cstdio>
+
+#include "internal.hpp"
+
+namespace fyfft {
+using namespace dev;
+
+template <typename V, int PPT, int DIR>
+__global__ __launch_bounds__(1024) void k_lds_generic(const V* in, V* out, long long batch,
+                                                      long long dist, int N, int T, int TPB,
+                                                      int ps, int lds_stride, PassDesc pd,
+                                                      const V* __restrict__ tw, S<V> scale) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  V* lds_all = reinterpret_cast<V*>(smem_raw);
+  const int slot = threadIdx.x / T;
+  const int tid = threadIdx.x - slot * T;
+  const long long t = (long long)blockIdx.x * TPB + slot;
+  const bool active = t < batch;
+  V* lds = lds_all + slot * lds_stride;
+  const V* gin = in + (active ? t : 0) * dist;
+  V* gout = out + (active ? t : 0) * dist;
+  if (active)
+    for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gin[i];
+  __syncthreads();
+  lds_fft<V, PPT, DIR>(lds, N, T, tid, pd, tw, active, ps);
+  if (active)
+    for (int i = tid; i < N; i += T) gout[i] = cscale(lds[i + (i >> ps)], scale);
+}
+
+// ------------------------------------------------------------ fixed kernels
+template <typename V, int DIR, int N, int T, int PS, int Ls, int R, int... Rest>
+__device__ __forceinline__ void run_fixed(const V* gin, V* lds, V* gout, int tid,
+                                          const V* __restrict__ tw, bool active, S<V> scale) {
+  constexpr bool first = (Ls == 1);
+  constexpr bool last = (sizeof...(Rest) == 0);
+  constexpr int MAXB = cdiv(N / R, T);
+  const V* src = first ? gin : lds;
+  V* dst = last ? gout : lds;
+  stockham_pass<V, R, MAXB, DIR, first, last>(src, dst, N, Ls, T, tid, tw, active, scale, PS);
+  if constexpr (!last) run_fixed<V, DIR, N, T, PS, Ls * R, Rest...>(gin, lds, gout, tid, tw, active, scale);
+}
+
+template <typename V, int DIR, int N, int T, int TPB, int PS, int... Rs>
+__global__ __launch_bounds__(T * TPB) void k_lds_fixed(const V* in, V* out, long long batch,
+                                                       long long dist, const V* __restrict__ tw,
+                                                       S<V> scale) {
+  constexpr int stride = lds_len(N, PS);
+  __shared__ V lds_all[TPB * stride];
+  const int slot = threadIdx.x / T;
+  const int tid = threadIdx.x - slot * T;
+  const long long t = (long long)blockIdx.x * TPB + slot;
+  const bool active = t < batch;
+  const V* gin = in + (active ? t : 0) * dist;
+  V* gout = out + (active ? t : 0) * dist;
+  run_fixed<V, DIR, N, T, PS, 1, Rs...>(gin, lds_all + slot * stride, gout, tid, tw, active, scale);
+}
+
+using FixedLauncher = hipError_t (*)(int dir, const void* in, void* out, size_t batch,
+                                     size_t dist, const void* tw, double scale, hipStream_t s);
+
+template <typename V, int N, int T, int TPB, int... Rs>
+static hipError_t launch_fixed(int dir, const void* in, void* out, size_t batch, size_t dist,
+                               const void* tw, double scale, hipStream_t s) {
+  const dim3 grid((unsigned)((batch + TPB - 1) / TPB)), block(T * TPB);
+  if (dir == kForward)
+    hipLaunchKernelGGL((k_lds_fixed<V, -1, N, T, TPB, kPadShift, Rs...>), grid, block, 0, s,
+                       (const V*)in, (V*)out, (long long)batch, (long long)dist, (const V*)tw,
+                       (S<V>)scale);
+  else
+    hipLaunchKernelGGL((k_lds_fixed<V, +1, N, T, TPB, kPadShift, Rs...>), grid, block, 0, s,
+                       (const V*)in, (V*)out, (long long)batch, (long long)dist, (const V*)tw,
+                       (S<V>)scale);
+  return hipGetLastError();
+}
+
+struct FixedEntry {
+  int prec, n, threads, per_block, npass;
+  int radix[6];
+  FixedLauncher launch;
+};
+
+#define FIXED(P, V, N, T, TPB, NP, ...) \
+  {P, N, T, TPB, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, __VA_ARGS__>}
+
+// Radix lists here must equal radices_for(N) (planner.cpp) so that the plan's
+// twiddle table matches; find_fixed_kernel checks this at plan time.
+static const FixedEntry kFixed[] = {
+    FIXED(kF32, float2, 256, 16, 16, 2, 16, 16),
+    FIXED(kF32, float2, 512, 32, 8, 3, 8, 8, 8),
+    FIXED(kF32, float2, 1024, 64, 4, 3, 16, 8, 8),
+    FIXED(kF32, float2, 2048, 128, 2, 3, 16, 16, 8),
+    FIXED(kF32, float2, 4096, 256, 1, 3, 16, 16, 16),
+    FIXED(kF32, float2, 8192, 512, 1, 4, 16, 8, 8, 8),
+    FIXED(kF32, float2, 16384, 1024, 1, 4, 16, 16, 8, 8),
+    FIXED(kF64, double2, 1024, 64, 4, 3, 16, 8, 8),
+    FIXED(kF64, double2, 4096, 256, 1, 3, 16, 16, 16),
+};
+#undef FIXED
+
+int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
+  for (int i = 0; i < (int)(sizeof(kFixed) / sizeof(kFixed[0])); ++i) {
+    const FixedEntry& e = kFixed[i];
+    if (e.prec != prec || e.n != g->n || e.npass != pd.npass) continue;
+    bool same = true;
+    for (int p = 0; p < pd.npass; ++p) same = same && (e.radix[p] == pd.radix[p]);
+    if (!same) continue;
+    g->threads = e.threads;
+    g->per_block = e.per_block;
+    g->pad_shift = kPadShift;
+    g->lds_bytes = (size_t)lds_len(e.n, kPadShift) * e.per_block * elem_bytes(prec);
+    return i;
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------ launch
+template <typename V, int PPT>
+static hipError_t launch_generic_t(int dir, const LdsLaunch& L, const void* in, void* out,
+                                   size_t batch, size_t dist, double scale, hipStream_t s) {
+  const LdsGeom& g = L.geom;
+  const int stride = lds_len(g.n, g.pad_shift);
+  const dim3 grid((unsigned)((batch + g.per_block - 1) / g.per_block));
+  const dim3 block(g.threads * g.per_block);
+  const size_t lds = (size_t)stride * g.per_block * sizeof(V);
+  if (dir == kForward) {
+    auto k = k_lds_generic<V, PPT, -1>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, grid, block, lds, s, (const V*)in, (V*)out, (long long)batch,
+                       (long long)dist, g.n, g.threads, g.per_block, g.pad_shift, stride, L.pd,
+                       (const V*)L.tw, (S<V>)scale);
+  } else {
+    auto k = k_lds_generic<V, PPT, +1>;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, grid, block, lds, s, (const V*)in, (V*)out, (long long)batch,
+                       (long long)dist, g.n, g.threads, g.per_block, g.pad_shift, stride, L.pd,
+                       (const V*)L.tw, (S<V>)scale);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_lds(int prec, int dir, const LdsLaunch& L, const void* in, void* out,
+                      size_t batch, size_t dist, double scale, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  if (L.fixed >= 0) return kFixed[L.fixed].launch(dir, in, out, batch, dist, L.tw, scale, s);
+  if (prec == kF64)
+    return L.geom.ppt <= 8 ? launch_generic_t<double2, 8>(dir, L, in, out, batch, dist, scale, s)
+                           : launch_generic_t<double2, 16>(dir, L, in, out, batch, dist, scale, s);
+  return L.geom.ppt <= 16 ? launch_generic_t<float2, 16>(dir, L, in, out, batch, dist, scale, s)
+                          : launch_generic_t<float2, 32>(dir, L, in, out, batch, dist, scale, s);
+}
+
+}  // namespace fyfft

@@ -1,0 +1,144 @@
+// Host planner (see planner.hpp).
+#include "planner.hpp"
+
+#include <algorithm>
+
+#include "fft_device.hpp"
+
+namespace fyfft {
+
+// The reference's directly supported lengths (src/foye_FFT_radix_wrap.hpp:93-98).
+static const size_t kRegistry[38] = {
+    2, 3, 4, 5, 7, 8, 9, 16, 25, 27, 32, 49, 64, 81,
+    125, 128, 243, 256, 343, 512, 625, 729, 1024,
+    2048, 2187, 2401, 3125, 4096, 6561, 8192, 15625,
+    16384, 16807, 19683, 32768, 59049, 65536, 78125};
+
+bool in_reference_registry(size_t n) {
+  return std::binary_search(kRegistry, kRegistry + 38, n);
+}
+
+bool is_smooth7(size_t n) {
+  if (n == 0) return false;
+  for (size_t p : {2u, 3u, 5u, 7u})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+bool is_prime(size_t n) {
+  if (n < 2) return false;
+  if (n < 4) return true;
+  if (n % 2 == 0 || n % 3 == 0) return false;
+  for (size_t i = 5; i * i <= n; i += 6)
+    if (n % i == 0 || n % (i + 2) == 0) return false;
+  return true;
+}
+
+uint64_t power_mod(uint64_t base, uint64_t e, uint64_t mod) {
+  unsigned __int128 r = 1, b = base % mod;
+  while (e) {
+    if (e & 1) r = (r * b) % mod;
+    b = (b * b) % mod;
+    e >>= 1;
+  }
+  return (uint64_t)r;
+}
+
+uint64_t primitive_root(uint64_t p) {
+  if (p == 2) return 1;
+  const uint64_t phi = p - 1;
+  uint64_t m = phi;
+  std::vector<uint64_t> f;
+  for (uint64_t i = 2; i * i <= m; ++i)
+    if (m % i == 0) {
+      f.push_back(i);
+      while (m % i == 0) m /= i;
+    }
+  if (m > 1) f.push_back(m);
+  for (uint64_t g = 2; g < p; ++g) {
+    bool ok = true;
+    for (uint64_t q : f)
+      if (power_mod(g, phi / q, p) == 1) { ok = false; break; }
+    if (ok) return g;
+  }
+  return 0;
+}
+
+int strategy_for(size_t n) {
+  if (in_reference_registry(n)) return kDirect;
+  if (is_smooth7(n)) return kMix;
+  if (is_prime(n) && is_smooth7(n - 1)) return kRader;
+  return kBluestein;
+}
+
+std::vector<int> radices_for(size_t n) {
+  std::vector<int> out;
+  if (n < 2 || !is_smooth7(n)) return out;
+  int e[8] = {0};
+  for (int p : {2, 3, 5, 7})
+    while (n % p == 0) { n /= p; ++e[p]; }
+  const int a = e[2];
+  if (a > 0) {
+    const int np = (a + 3) / 4;
+    const int base = a / np, extra = a % np;
+    for (int i = 0; i < np; ++i) out.push_back(1 << (base + (i < extra ? 1 : 0)));
+  }
+  for (int i = 0; i < e[7]; ++i) out.push_back(7);
+  for (int i = 0; i < e[5]; ++i) out.push_back(5);
+  for (int i = 0; i < e[3]; ++i) out.push_back(3);
+  return out;
+}
+
+size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap) { return lds_cap / elem_bytes; }
+
+bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
+  if (n < 1 || n > (size_t)1 << 20) return false;
+  LdsGeom r;
+  r.n = (int)n;
+  // points per thread: the generic kernels exist for F32 PPT 16/32 and
+  // F64 PPT 8/16 (F64 keeps PPT low to stay within the VGPR budget)
+  if (elem_bytes >= 16) r.ppt = n <= 8192 ? 8 : 16;
+  else r.ppt = n <= 16384 ? 16 : 32;
+  r.threads = (int)((n + r.ppt - 1) / r.ppt);
+  if (r.threads > 1024) return false;
+  r.pad_shift = dev::kPadShift;
+  size_t per = (size_t)((n - 1) + ((n - 1) >> r.pad_shift) + 1) * elem_bytes;
+  if (per > lds_cap) {
+    r.pad_shift = 30;  // no padding
+    per = n * elem_bytes;
+    if (per > lds_cap) return false;
+  }
+  int tpb = std::max(1, 256 / r.threads);
+  tpb = std::min<size_t>(tpb, lds_cap / per);
+  tpb = std::max(1, std::min(tpb, 1024 / r.threads));
+  r.per_block = tpb;
+  r.lds_bytes = per * (size_t)tpb;
+  *g = r;
+  return true;
+}
+
+bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2) {
+  if (!is_smooth7(n)) return false;
+  size_t best1 = 0;
+  for (size_t d = 2; d * d <= n; ++d) {
+    if (n % d) continue;
+    const size_t e = n / d;
+    if (d <= cap_elems && e <= cap_elems) best1 = d;  // largest d <= sqrt(n)
+  }
+  if (!best1) return false;
+  *n1 = best1;
+  *n2 = n / best1;
+  return true;
+}
+
+size_t bluestein_length(size_t n, size_t cap_elems) {
+  const size_t lo = 2 * n - 1;
+  for (size_t m = lo; m < 4 * lo + 64; ++m) {
+    if (!is_smooth7(m)) continue;
+    size_t a, b;
+    if (m <= cap_elems || four_step_split(m, cap_elems, &a, &b)) return m;
+  }
+  return 0;
+}
+
+}  // namespace fyfft

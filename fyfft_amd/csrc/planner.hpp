@@ -1,0 +1,49 @@
+// Host planner: strategy choice, radix factorisation, launch geometry and
+// number theory.  Pure C++ (no HIP), so the C ABI's planning queries work
+// without a GPU.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace fyfft {
+
+// strategy codes, same values as fyfft_strategy in include/fyfft.h
+enum Strategy { kDirect = 0, kMix = 1, kRader = 2, kBluestein = 3, kDft = 4 };
+
+bool in_reference_registry(size_t n);  // the reference's 38 direct lengths
+bool is_smooth7(size_t n);
+bool is_prime(size_t n);
+uint64_t power_mod(uint64_t base, uint64_t e, uint64_t mod);
+uint64_t primitive_root(uint64_t p);  // smallest generator of (Z/pZ)*, p prime
+int strategy_for(size_t n);
+
+// Radix passes (each in {2,3,4,5,7,8,16}) for a smooth length: powers of two
+// spread over ceil(log2/4) passes of at most 16, then 7s, 5s, 3s.  Empty if
+// n is not 7-smooth or n < 2.
+std::vector<int> radices_for(size_t n);
+
+// Geometry of one LDS-resident transform group.
+struct LdsGeom {
+  int n = 0;         // transform length
+  int ppt = 16;      // max points per thread (kernel template)
+  int threads = 0;   // threads per transform (T)
+  int per_block = 1; // transforms per workgroup (TPB)
+  int pad_shift = 4; // LDS index i -> i + (i >> pad_shift)
+  size_t lds_bytes = 0;
+};
+// Fits n (elements of `elem_bytes`) in one workgroup's LDS of `lds_cap`
+// bytes; returns false if it does not fit even unpadded.
+bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g);
+size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded
+
+// Four-step split n = n1 * n2 with both factors smooth and LDS-resident
+// (n1 <= n2, as balanced as possible).  False if none exists.
+bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2);
+
+// Bluestein convolution length: the smallest 7-smooth M >= 2n-1 that is
+// either LDS-resident or has a four-step split.
+size_t bluestein_length(size_t n, size_t cap_elems);
+
+}  // namespace fyfft

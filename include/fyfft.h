@@ -1,0 +1,140 @@
+/*
+ * fyfft.h -- C ABI of the MI355X-native 1-D complex FFT (libfyfft.so).
+ *
+ * Plain pointers and sizes only; no HIP or torch types in the signatures
+ * (a stream is passed as `void*` holding a hipStream_t, NULL = the null
+ * stream).
+ *
+ * The reference (kjear/fyFFT) exports a header-only C++ class and no C ABI:
+ *   fy::fft::FFT1D_C2C_invoker(std::size_t N)          foye_FFT.hpp:101-126
+ *   template<bool invert> dispatch(const float*, float*) foye_FFT.hpp:128-151
+ *   acquired_execution_strategy(std::size_t)           foye_FFT.hpp:162-184
+ *   min_input_length / max_input_length                foye_FFT.hpp:86-87
+ * Each entry point below names the piece of that interface it replaces.
+ * include/foye_FFT.hpp re-creates the class itself on top of this ABI.
+ *
+ * Conventions (same as the reference):
+ *   - data is interleaved (re, im), std::complex<T>-compatible;
+ *   - forward:  X[k] = sum_n x[n] exp(-2*pi*i*n*k/N)   (unnormalised);
+ *   - inverse:  x[n] = (1/N) sum_k X[k] exp(+2*pi*i*n*k/N);
+ *   - adjoint (this ABI only): the inverse without 1/N, i.e. the exact
+ *     adjoint of the forward transform (= N * inverse);
+ *   - in == out (in place) is allowed.
+ * Errors are reported as status codes where the reference has undefined
+ * behaviour (foye_FFT.hpp:39-43,103-104).
+ */
+#ifndef FYFFT_H
+#define FYFFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FYFFT_VERSION_MAJOR 0
+#define FYFFT_VERSION_MINOR 1
+
+/* Longest transform a plan accepts.  The reference stops at 80000
+ * (foye_FFT.hpp:87); this ABI goes beyond it (Bluestein N = 1000003 and
+ * smooth N = 2^20 are required workloads). */
+#define FYFFT_MAX_LENGTH ((size_t)1 << 26)
+
+typedef enum fyfft_status {
+  FYFFT_SUCCESS = 0,
+  FYFFT_INVALID_ARGUMENT = 1, /* null pointer, unknown enum value, batch 0 */
+  FYFFT_INVALID_LENGTH = 2,   /* N < 1 or N > FYFFT_MAX_LENGTH            */
+  FYFFT_UNSUPPORTED = 3,      /* valid request this build does not serve  */
+  FYFFT_ALLOC_FAILED = 4,     /* device or host allocation failed         */
+  FYFFT_DEVICE_ERROR = 5,     /* a HIP runtime call failed                */
+  FYFFT_INVALID_PLAN = 6,     /* null or destroyed plan                   */
+  FYFFT_NO_DEVICE = 7         /* no GPU visible to the process            */
+} fyfft_status;
+
+typedef enum fyfft_precision {
+  FYFFT_F32 = 0,  /* complex<float>  (the reference's only precision) */
+  FYFFT_F64 = 1,  /* complex<double>                                   */
+  FYFFT_BF16 = 2, /* reserved: bf16 storage                            */
+  FYFFT_F16 = 3   /* reserved: fp16 storage                            */
+} fyfft_precision;
+
+typedef enum fyfft_direction {
+  FYFFT_FORWARD = 0, /* dispatch<false>                     */
+  FYFFT_INVERSE = 1, /* dispatch<true> (normalised by 1/N)  */
+  FYFFT_ADJOINT = 2  /* conjugate twiddles, no 1/N          */
+} fyfft_direction;
+
+/* Same order and meaning as FFT1D_C2C_invoker::FFT_execution_strategy
+ * (foye_FFT.hpp:153-160).  The values report THIS library's choice:
+ *   KERNEL_DIRECT - N is one of the reference's 38 directly supported
+ *                   lengths (radix_wrap.hpp:93-98);
+ *   MIX_RADIX     - any other 2/3/5/7-smooth N;
+ *   RADER         - prime N whose N-1 is 2/3/5/7-smooth;
+ *   BLUESTEIN     - every other N;
+ *   DFT           - never chosen by this library (kept for ABI parity). */
+typedef enum fyfft_strategy {
+  FYFFT_KERNEL_DIRECT = 0,
+  FYFFT_MIX_RADIX = 1,
+  FYFFT_RADER = 2,
+  FYFFT_BLUESTEIN = 3,
+  FYFFT_DFT = 4
+} fyfft_strategy;
+
+typedef struct fyfft_plan fyfft_plan;
+
+/* Replaces FFT1D_C2C_invoker::FFT1D_C2C_invoker(std::size_t)
+ * (foye_FFT.hpp:101-126), extended with a batch of `batch` contiguous
+ * transforms (transform b starts at element b*N) and a precision.
+ * Plans the factorisation on the host, allocates device tables (generated
+ * in FP64 on the GPU) and any workspace, on the current HIP device.
+ * Synchronous. */
+fyfft_status fyfft_plan_create(fyfft_plan** plan, size_t n, size_t batch,
+                               fyfft_precision precision);
+
+/* Releases every device allocation of the plan; NULL is accepted. */
+fyfft_status fyfft_plan_destroy(fyfft_plan* plan);
+
+/* Replaces dispatch<invert>(const float* in, float* out)
+ * (foye_FFT.hpp:128-151) on DEVICE pointers: enqueues the batch on
+ * `stream` and returns without waiting.  `direction` is a fyfft_direction.
+ * A plan may be executed repeatedly; executions that overlap in time on
+ * different streams must use different plans (the reference invoker is
+ * not reentrant either: mix_radix.hpp:538, rader.hpp:363,
+ * bluestein.hpp:240). */
+fyfft_status fyfft_execute(const fyfft_plan* plan, const void* in, void* out,
+                           int direction, void* stream);
+
+/* dispatch<invert> with the reference's HOST-pointer semantics: copies the
+ * batch to the device, runs it, copies back and synchronises before
+ * returning.  in == out is allowed. */
+fyfft_status fyfft_execute_host(const fyfft_plan* plan, const void* in, void* out,
+                                int direction);
+
+/* Replaces acquired_execution_strategy(std::size_t) (foye_FFT.hpp:162-184)
+ * for a planned transform, and for a bare length (no GPU needed). */
+fyfft_status fyfft_plan_strategy(const fyfft_plan* plan, int* strategy);
+fyfft_status fyfft_length_strategy(size_t n, int* strategy);
+
+/* Radix passes the planner chooses for a 2/3/5/7-smooth length held in one
+ * workgroup's LDS, or for each factor of a four-step split (no GPU needed).
+ * Writes up to `max` radices, sets *count; FYFFT_UNSUPPORTED if N is not
+ * smooth.  The counterpart of the reference's split planner
+ * (mix_radix.hpp:253-352, 577-598). */
+fyfft_status fyfft_length_radices(size_t n, int* radices, int max, int* count);
+
+/* Device bytes the plan holds (tables + workspace). */
+fyfft_status fyfft_plan_workspace_bytes(const fyfft_plan* plan, size_t* bytes);
+
+/* Human-readable one-line description of the plan's kernels (for logs). */
+fyfft_status fyfft_plan_describe(const fyfft_plan* plan, char* buf, size_t len);
+
+/* Static text for a status code; never NULL. */
+const char* fyfft_status_string(fyfft_status status);
+
+/* Text of the last HIP error seen on this thread ("" if none). */
+const char* fyfft_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FYFFT_H */

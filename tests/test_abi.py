@@ -1,0 +1,85 @@
+"""libfyfft.so loads and exports every symbol include/fyfft.h declares;
+the planner queries and argument checks work without a GPU (no compute
+calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "fyfft.h")).read()
+    return sorted(set(re.findall(r"\b(fyfft_[a-z_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol(fy):
+    syms = header_symbols()
+    assert len(syms) >= 11
+    lib = fy.lib()
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_status_strings(fy):
+    lib = fy.lib()
+    for code in range(8):
+        assert lib.fyfft_status_string(code)
+    assert lib.fyfft_status_string(99) == b"unknown status"
+
+
+def test_argument_checks_without_gpu(fy):
+    lib = fy.lib()
+    h = ctypes.c_void_p()
+    assert lib.fyfft_plan_create(ctypes.byref(h), 0, 1, 0) == 2          # invalid length
+    assert lib.fyfft_plan_create(ctypes.byref(h), (1 << 26) + 1, 1, 0) == 2
+    assert lib.fyfft_plan_create(ctypes.byref(h), 64, 0, 0) == 1         # batch 0
+    assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 2) == 3         # bf16 reserved
+    assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 9) == 1         # unknown precision
+    assert lib.fyfft_plan_create(None, 64, 1, 0) == 1
+    assert lib.fyfft_execute(None, None, None, 0, None) == 6             # invalid plan
+    assert lib.fyfft_plan_destroy(None) == 0
+    s = ctypes.c_int()
+    assert lib.fyfft_length_strategy(0, ctypes.byref(s)) == 2
+    assert lib.fyfft_length_strategy(16, None) == 1
+
+
+@pytest.mark.parametrize("n,expect", [
+    (1024, "kernel_direct"), (4096, "kernel_direct"), (78125, "kernel_direct"),
+    (6720, "mix_radix"), (12, "mix_radix"), (1 << 20, "mix_radix"), (80000, "mix_radix"),
+    (65537, "rader"), (11, "rader"), (1009, "rader"),
+    (10007, "bluestein"), (1000003, "bluestein"), (10006, "bluestein"), (39062, "bluestein"),
+])
+def test_length_strategy(fy, n, expect):
+    assert fy.length_strategy(n).name == expect
+
+
+def test_radices_factor_every_smooth_length(fy):
+    for n in list(range(2, 2000)) + [4096, 6720, 16384, 19683, 78125, 1 << 20]:
+        smooth = all(True for _ in [0])
+        m = n
+        for p in (2, 3, 5, 7):
+            while m % p == 0:
+                m //= p
+        if m != 1:
+            with pytest.raises(fy.FyfftError):
+                fy.length_radices(n)
+            continue
+        r = fy.length_radices(n)
+        assert all(x in (2, 3, 4, 5, 7, 8, 16) for x in r)
+        prod = 1
+        for x in r:
+            prod *= x
+        assert prod == n
+    assert fy.length_radices(4096) == [16, 16, 16]
+    assert fy.length_radices(6720) == [8, 8, 7, 5, 3]
+
+
+def test_invoker_mirror_strategy(fy):
+    S = fy.FFT1D_C2C_invoker.acquired_execution_strategy
+    assert S(1024) == fy.FFT_execution_strategy.kernel_direct
+    assert S(65537) == fy.FFT_execution_strategy.rader
+    with pytest.raises(ValueError):
+        fy.FFT1D_C2C_invoker(1)

@@ -1,0 +1,123 @@
+"""GPU parity of the LDS-resident smooth-length path against the FP64
+oracle, through the C ABI (fyfft_execute on device pointers).
+
+Tolerances (relative L2 against the oracle):
+  FP32: 1e-7 * log2(N) + 1e-7      FP64: 2e-16 * log2(N) + 5e-16
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def tol(n, prec):
+    l2 = max(1.0, math.log2(n))
+    return 1e-7 * l2 + 1e-7 if prec == "f32" else 2e-16 * l2 + 5e-16
+
+
+def data(rng, batch, n, prec):
+    x = rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))
+    return x.astype(np.complex64 if prec == "f32" else np.complex128)
+
+
+def run(fy, torch, x, direction, inplace=False):
+    batch, n = x.shape
+    prec = fy.F32 if x.dtype == np.complex64 else fy.F64
+    xd = torch.from_numpy(x).cuda()
+    with fy.Plan(n, batch, prec) as p:
+        if inplace:
+            p.execute(xd, xd, direction)
+            out = xd
+        else:
+            out = p.execute(xd, None, direction)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+SMOOTH = [2, 3, 4, 5, 7, 8, 9, 12, 16, 25, 27, 32, 49, 60, 64, 81, 105, 125, 128, 243, 256,
+          343, 360, 512, 625, 720, 729, 1000, 1024, 2048, 2187, 2401, 3125, 4096, 4608,
+          6561, 6720, 8192, 15625, 16384, 16807]
+
+
+@pytest.mark.parametrize("n", SMOOTH)
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_forward_inverse_vs_oracle(fy, orc, cuda, n, prec):
+    import torch
+    if prec == "f64" and n > 8192:
+        pytest.skip("FP64 LDS path covers N <= 8192 in one workgroup")
+    rng = np.random.default_rng(n)
+    batch = 3
+    x = data(rng, batch, n, prec)
+    ref = orc.fft_rows(x.astype(np.complex128))
+    y = run(fy, torch, x, fy.FORWARD)
+    err = orc.rel_l2(y, ref)
+    print(f"N={n} {prec} fwd rel-L2 {err:.3e}")
+    assert err < tol(n, prec), err
+    z = run(fy, torch, x, fy.INVERSE)
+    refi = orc.fft_rows(x.astype(np.complex128), inverse=True)
+    assert orc.rel_l2(z, refi) < tol(n, prec)
+    a = run(fy, torch, x, fy.ADJOINT)
+    assert orc.rel_l2(a, refi * n) < tol(n, prec)
+
+
+@pytest.mark.parametrize("n,batch", [(4096, 257), (1024, 1029), (64, 1001), (6720, 5), (2, 7)])
+def test_batch_and_inplace(fy, orc, cuda, n, batch):
+    import torch
+    rng = np.random.default_rng(batch)
+    x = data(rng, batch, n, "f32")
+    ref = orc.fft_rows(x.astype(np.complex128)) if batch * n <= 1 << 20 else None
+    y = run(fy, torch, x, fy.FORWARD)
+    y2 = run(fy, torch, x, fy.FORWARD, inplace=True)
+    assert np.array_equal(y, y2)  # same kernels, same order of operations
+    if ref is not None:
+        assert orc.rel_l2(y, ref) < tol(n, "f32")
+    # round trip on the whole batch
+    z = run(fy, torch, y, fy.INVERSE)
+    assert orc.rel_l2(z, x) < 2 * tol(n, "f32")
+
+
+def test_config1_golden(fy, orc, cuda):
+    g = np.load(os.path.join(HERE, "golden", "config1_n1024.npz"))
+    x = g["x"]
+    inv = fy.FFT1D_C2C_invoker(1024)
+    assert inv.acquired_execution_strategy(1024) == fy.FFT_execution_strategy.kernel_direct
+    out = np.empty(2048, np.float32)
+    inv.dispatch(x.view(np.float32), out, invert=False)
+    err = orc.rel_l2(out.view(np.complex64), g["fwd"])
+    print(f"config1 N=1024 fwd rel-L2 {err:.3e}")
+    assert err < tol(1024, "f32")
+    back = np.empty(2048, np.float32)
+    inv.dispatch(x.view(np.float32), back, invert=True)
+    assert orc.rel_l2(back.view(np.complex64), g["inv"]) < tol(1024, "f32")
+    # in place, like dispatch(p, p)
+    buf = x.view(np.float32).copy()
+    inv.dispatch(buf, buf, invert=False)
+    assert orc.rel_l2(buf.view(np.complex64), g["fwd"]) < tol(1024, "f32")
+
+
+def test_closed_forms_on_gpu(fy, cuda):
+    import torch
+    n = 4096
+    x = np.zeros((2, n), np.complex64)
+    x[0, 0] = 1
+    k0 = 5
+    x[1] = np.exp(2j * np.pi * k0 * np.arange(n) / n).astype(np.complex64)
+    y = run(fy, torch, x, fy.FORWARD)
+    assert np.abs(y[0] - 1).max() < 1e-6
+    expect = np.zeros(n)
+    expect[k0] = n
+    assert np.abs(y[1] - expect).max() < 1e-3
+
+
+def test_plan_queries(fy, cuda):
+    with fy.Plan(4096, 4, fy.F32) as p:
+        assert p.strategy == fy.FFT_execution_strategy.kernel_direct
+        assert p.workspace_bytes >= 4095 * 8
+        assert "N=4096" in p.describe()
+    with fy.Plan(6720, 1, fy.F32) as p:
+        assert p.strategy == fy.FFT_execution_strategy.mix_radix

@@ -1,0 +1,201 @@
+"""The CPU oracle, pinned against what fixes it independently of its own
+formulas: the survey's measurements of the compiled reference
+(golden/reference_facts.json), closed forms of the DFT, numpy's pocketfft as
+a second FP64 opinion, brute force on tiny inputs, and checks that a planted
+mistake (wrong sign, dropped twiddle, transposed output) is caught."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FACTS = json.load(open(os.path.join(HERE, "golden", "reference_facts.json")))
+
+G2_SIZES = [2, 3, 4, 5, 7, 8, 16, 32, 128, 243, 625, 2401, 4096,
+            6, 12, 60, 105, 720, 6720, 11, 101, 1009, 10007, 22, 121, 10006]
+
+
+def cx(rng, n):
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+# ---------------------------------------------------------------- planner
+def test_registry_matches_reference(orc):
+    assert orc.smooth_lengths() == FACTS["reference_registry"]
+
+
+@pytest.mark.parametrize("name,code", [("kernel_direct", 0), ("mix_radix", 1), ("rader", 2),
+                                       ("bluestein", 3), ("DFT", 4)])
+def test_strategy_facts(orc, name, code):
+    for n in FACTS["strategy"][name]:
+        assert orc.strategy(n) == code, (n, name)
+
+
+def test_decomposition_facts(orc):
+    for n, dec in FACTS["decomposition"].items():
+        assert orc.decomposition(int(n)) == dec
+    for n, n1 in FACTS["compute_n1"].items():
+        assert orc.compute_n1(int(n)) == n1
+
+
+def test_pad_facts(orc):
+    for n, m in FACTS["rader_M"].items():
+        assert orc.rader_M(int(n)) == m
+    for n, m in FACTS["bluestein_M"].items():
+        assert orc.bluestein_M(int(n)) == m
+
+
+def test_rader_domain_edges(orc):
+    # prime, not in the registry, 2N-3 <= 78125  =>  N <= 39064
+    assert orc.rader_is_processable(39043)
+    assert not orc.rader_is_processable(65537)
+    assert not orc.rader_is_processable(7)      # in the registry
+    assert not orc.rader_is_processable(10006)  # not prime
+
+
+def test_primitive_root_brute_force(orc):
+    for p in [3, 5, 7, 11, 13, 101, 1009, 10007, 65537]:
+        g = orc.primitive_root(p)
+        # brute force: the order of g is p-1 and no smaller generator exists
+        seen, v = set(), 1
+        for _ in range(p - 1):
+            v = v * g % p
+            seen.add(v)
+        assert len(seen) == p - 1, p
+        for h in range(2, g):
+            order_full = len({pow(h, k, p) for k in range(1, p)}) == p - 1
+            assert not order_full, (p, h)
+    assert orc.power_mod(3, 10**12, 10**9 + 7) == pow(3, 10**12, 10**9 + 7)
+
+
+def test_is_prime_matches_sieve(orc):
+    lim = 5000
+    sieve = np.ones(lim, bool)
+    sieve[:2] = False
+    for i in range(2, int(lim ** 0.5) + 1):
+        sieve[i * i::i] = False
+    assert [orc.is_prime(i) for i in range(lim)] == list(sieve)
+
+
+def test_mix_processable_is_greedy_registry_division(orc):
+    reg = FACTS["reference_registry"]
+    for n in range(2, 3000):
+        m, ok = n, True
+        while m > 1:
+            f = next((f for f in reversed(reg) if f <= m and m % f == 0), None)
+            if f is None:
+                ok = False
+                break
+            m //= f
+        assert orc.mix_is_processable(n) == ok, n
+
+
+# ---------------------------------------------------------------- naive DFT
+def test_dft_brute_force_tiny(orc):
+    rng = np.random.default_rng(3)
+    for n in range(1, 9):
+        x = cx(rng, n)
+        ref = np.array([sum(x[j] * np.exp(-2j * np.pi * j * k / n) for j in range(n)) for k in range(n)])
+        assert np.allclose(orc.dft_naive(x), ref, atol=1e-13)
+
+
+@pytest.mark.parametrize("n", [7, 64, 1000, 1024])
+def test_dft_closed_forms(orc, n):
+    imp = np.zeros(n, complex)
+    imp[0] = 1
+    assert np.array_equal(orc.dft_naive(imp), np.ones(n, complex))
+    dc = np.ones(n, complex)
+    y = orc.dft_naive(dc)
+    assert abs(y[0] - n) < 1e-12 and np.abs(y[1:]).max() < 1e-12 * n
+    k0 = 3 % n
+    tone = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+    y = orc.dft_naive(tone)
+    expect = np.zeros(n, complex)
+    expect[k0] = n
+    assert np.abs(y - expect).max() < 1e-12 * n
+    # shifted impulse gives a pure phase ramp
+    sh = np.zeros(n, complex)
+    sh[1 % n] = 1
+    assert np.abs(orc.dft_naive(sh) - np.exp(-2j * np.pi * np.arange(n) * (1 % n) / n)).max() < 1e-15 * n
+
+
+def test_dft_parseval_linearity_shift(orc):
+    rng = np.random.default_rng(5)
+    n = 360
+    x, z = cx(rng, n), cx(rng, n)
+    X, Z = orc.dft_naive(x), orc.dft_naive(z)
+    assert abs(np.vdot(X, X).real - n * np.vdot(x, x).real) < 1e-10 * n * np.vdot(x, x).real
+    assert np.abs(orc.dft_naive(2 * x - 3j * z) - (2 * X - 3j * Z)).max() < 1e-10
+    s = 7
+    Xs = orc.dft_naive(np.roll(x, s))
+    assert np.abs(Xs - X * np.exp(-2j * np.pi * np.arange(n) * s / n)).max() < 1e-10
+    assert np.abs(orc.dft_naive(X, inverse=True) - x).max() < 1e-13
+
+
+def test_dft_matches_numpy(orc):
+    rng = np.random.default_rng(7)
+    for n in [5, 97, 512, 1000, 4096]:
+        x = cx(rng, n)
+        assert orc.rel_l2(orc.dft_naive(x), np.fft.fft(x)) < 5e-16
+
+
+# ---------------------------------------------------------------- restated ladder
+@pytest.mark.parametrize("n", G2_SIZES)
+def test_ladder_f64_vs_naive(orc, n):
+    rng = np.random.default_rng(n)
+    x = cx(rng, n)
+    ref = orc.dft_naive(x)
+    assert orc.rel_l2(orc.fft64(x), ref) < 2e-16 * max(1.0, math.log2(n)) + 5e-16
+    assert orc.rel_l2(orc.fft64(x, inverse=True), orc.dft_naive(x, inverse=True)) < 1e-15
+
+
+@pytest.mark.parametrize("n", G2_SIZES)
+def test_ladder_f32_error_level(orc, n):
+    """FP32 port error sits in the band the compiled reference shows
+    (3.6e-8 .. 3.1e-7) and under 1e-7*log2(N) + 2e-7."""
+    rng = np.random.default_rng(n + 1)
+    x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
+    ref = orc.dft_naive(x.astype(np.complex128))
+    err = orc.rel_l2(orc.fft32(x), ref)
+    assert err < 1e-7 * math.log2(n) + 2e-7, err
+    rt = orc.rel_l2(orc.fft32(orc.fft32(x), inverse=True), x)
+    assert rt < 1e-6
+
+
+@pytest.mark.parametrize("n", [65537, 1 << 20, 1000003])
+def test_ext_large_vs_numpy(orc, n):
+    rng = np.random.default_rng(11)
+    x = cx(rng, n)
+    assert orc.rel_l2(orc.fft_ext(x), np.fft.fft(x)) < 3e-15
+
+
+def test_config1_golden(orc):
+    g = np.load(os.path.join(HERE, "golden", "config1_n1024.npz"))
+    x = g["x"].astype(np.complex128)
+    assert orc.rel_l2(orc.dft_naive(x), g["fwd"]) < 1e-16
+    assert orc.rel_l2(orc.fft64(x), g["fwd"]) < 1e-15
+    assert orc.rel_l2(orc.fft64(x, inverse=True), g["inv"]) < 1e-15
+
+
+# ---------------------------------------------------------------- planted mistakes
+def test_checks_catch_planted_mistakes(orc):
+    """Each plausible mistake in a transform fails at least one check used
+    above: conjugated sign (tone test), dropped twiddle (naive comparison),
+    transposed/bit-reversed output order (impulse-shift test)."""
+    n = 64
+    k0 = 3
+    tone = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+    wrong_sign = orc.dft_naive(tone, inverse=True) * n  # sign flipped, unnormalised
+    assert np.abs(wrong_sign[k0]) < 1e-9          # energy lands at N-k0, not k0
+    rng = np.random.default_rng(1)
+    x = cx(rng, n)
+    ref = orc.dft_naive(x)
+    # six-step without the twiddle multiply
+    x2 = x.reshape(8, 8)
+    no_tw = np.fft.fft(np.fft.fft(x2.T.copy(), axis=1).T, axis=1).T.reshape(-1)
+    assert orc.rel_l2(no_tw, ref) > 0.1
+    # bit-reversed output order
+    rev = np.array([int(format(i, "06b")[::-1], 2) for i in range(n)])
+    assert orc.rel_l2(ref[rev], ref) > 0.1
