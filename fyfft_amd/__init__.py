@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libfyfft.so")
 
 min_input_length = 2          # foye_FFT.hpp:86
 max_input_length = 80_000     # foye_FFT.hpp:87 (the reference's limit)
-MAX_LENGTH = 1 << 26          # this library's limit (FYFFT_MAX_LENGTH)
+MAX_LENGTH = 1 << 24          # this library's limit (FYFFT_MAX_LENGTH)
 
 FORWARD, INVERSE, ADJOINT = 0, 1, 2
 F32, F64, BF16, F16 = 0, 1, 2, 3
@@ -68,6 +68,13 @@ def _load() -> ctypes.CDLL:
     with _lib_lock:
         if _lib is not None:
             return _lib
+        # torch ships its own libamdhip64.so.7; loading it first makes this
+        # library bind to the same HIP runtime instead of a second copy
+        # from /opt/rocm (two runtimes in one process do not share devices).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 f"fyfft_amd: native library not built ({LIB_PATH}); run "

@@ -19,7 +19,12 @@ namespace {
 constexpr uint32_t kMagic = 0xF1FF7A11u;
 thread_local std::string g_last_error;
 
-enum Kind { kTrivial = 0, kLds = 1 };
+enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6 };
+
+// Workspace of multi-pass plans is capped; larger batches run in chunks.
+constexpr size_t kWorkspaceCap = (size_t)1 << 30;
+// Longest factor of a four-step split (one column/row FFT in LDS).
+constexpr size_t kFactorCap = 4096;
 
 fyfft_status from_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return FYFFT_SUCCESS;
@@ -44,7 +49,22 @@ struct fyfft_plan {
   int strategy = kDirect;
   int device = 0;
   Kind kind = kTrivial;
-  LdsLaunch lds;
+  LdsLaunch lds;        // kLds, and the conv length of kRaderLds / kBlueLds
+  // four-step structure (kFourStep; conv length of kRaderFs / kBlueFs)
+  size_t n1 = 0, n2 = 0;
+  LdsGeom gcol, grow;
+  dev::PassDesc pdcol{}, pdrow{};
+  void *twcol = nullptr, *twrow = nullptr, *tw_hi = nullptr, *tw_lo = nullptr;
+  int lo_bits = 0;
+  size_t chunk = 0;     // transforms per workspace chunk
+  void* tmp = nullptr;  // chunk * conv length elements
+  void* x0buf = nullptr;
+  // convolution plans
+  size_t m = 0;                       // conv length (Rader N-1, Bluestein M)
+  void* kern[2] = {nullptr, nullptr}; // [0] forward, [1] inverse/adjoint spectra / m
+  void* chirp[2] = {nullptr, nullptr};
+  int* perm_in = nullptr;
+  int* perm_out = nullptr;
   std::vector<void*> allocs;
   size_t dev_bytes = 0;
   std::string desc;
@@ -81,21 +101,272 @@ static std::string radix_string(const dev::PassDesc& pd) {
   return s;
 }
 
-// Plan an LDS-resident smooth transform of length n; fills L and allocates
-// and generates its twiddle table.
-static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L) {
-  const size_t cap = lds_cap_bytes(p->device);
-  if (!lds_geometry(n, elem_bytes(p->prec), cap, &L->geom)) return FYFFT_UNSUPPORTED;
+static int ilog2_exact(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Pass list + FP64-generated Stockham twiddle table for a smooth length.
+static fyfft_status make_pass_table(fyfft_plan* p, size_t n, dev::PassDesc* pd, void** tw) {
   const std::vector<int> rad = radices_for(n);
   if (rad.empty() || (int)rad.size() > dev::kMaxPasses) return FYFFT_UNSUPPORTED;
-  L->pd.npass = (int)rad.size();
-  for (size_t i = 0; i < rad.size(); ++i) L->pd.radix[i] = rad[i];
-  L->fixed = find_fixed_kernel(p->prec, L->pd, &L->geom);
-  void* tw = nullptr;
-  fyfft_status st = p->alloc(std::max<size_t>(n, 2) * elem_bytes(p->prec), &tw);
+  pd->npass = (int)rad.size();
+  for (size_t i = 0; i < rad.size(); ++i) pd->radix[i] = rad[i];
+  fyfft_status st = p->alloc(std::max<size_t>(n, 2) * elem_bytes(p->prec), tw);
   if (st) return st;
-  HIP_TRY(launch_gen_stockham_twiddles(p->prec, tw, (int)n, L->pd, nullptr));
+  HIP_TRY(launch_gen_stockham_twiddles(p->prec, *tw, (int)n, *pd, nullptr));
+  return FYFFT_SUCCESS;
+}
+
+// Plan an LDS-resident smooth transform of length n.
+static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_fixed) {
+  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &L->geom)) return FYFFT_UNSUPPORTED;
+  void* tw = nullptr;
+  fyfft_status st = make_pass_table(p, n, &L->pd, &tw);
+  if (st) return st;
   L->tw = tw;
+  L->fixed = allow_fixed ? find_fixed_kernel(p->prec, L->pd, &L->geom) : -1;
+  return FYFFT_SUCCESS;
+}
+
+// Geometry of a four-step pass whose LDS FFT has length len: up to 16
+// sub-transforms (columns or rows) per workgroup, two workgroups per CU
+// where the LDS allows.
+static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g) {
+  if (!lds_geometry(len, elem, cap, g)) return false;
+  const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
+  int c = 16;
+  while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
+  if ((size_t)c * stride * elem > cap) return false;
+  g->per_block = c;
+  g->lds_bytes = (size_t)c * stride * elem;
+  return true;
+}
+
+static bool fs_possible(size_t n, size_t elem, size_t cap) {
+  size_t n1, n2;
+  LdsGeom a, b;
+  return four_step_split(n, kFactorCap, &n1, &n2) && fs_geometry(n1, elem, cap, &a) &&
+         fs_geometry(n2, elem, cap, &b);
+}
+
+// Four-step structure for smooth n (tables only; workspace separately).
+static fyfft_status plan_fourstep(fyfft_plan* p, size_t n) {
+  const size_t cap = lds_cap_bytes(p->device), elem = elem_bytes(p->prec);
+  if (!four_step_split(n, kFactorCap, &p->n1, &p->n2)) return FYFFT_UNSUPPORTED;
+  if (!fs_geometry(p->n1, elem, cap, &p->gcol) || !fs_geometry(p->n2, elem, cap, &p->grow))
+    return FYFFT_UNSUPPORTED;
+  fyfft_status st = make_pass_table(p, p->n1, &p->pdcol, &p->twcol);
+  if (!st) st = make_pass_table(p, p->n2, &p->pdrow, &p->twrow);
+  if (st) return st;
+  int lg = 0;
+  while (((size_t)1 << lg) < n) ++lg;
+  p->lo_bits = (lg + 1) / 2;
+  const size_t lo = (size_t)1 << p->lo_bits, hi = (n >> p->lo_bits) + 1;
+  if ((st = p->alloc(lo * elem, &p->tw_lo)) || (st = p->alloc(hi * elem, &p->tw_hi))) return st;
+  HIP_TRY(launch_gen_roots(p->prec, p->tw_lo, (long long)lo, 1, (long long)n, nullptr));
+  HIP_TRY(launch_gen_roots(p->prec, p->tw_hi, (long long)hi, (long long)lo, (long long)n, nullptr));
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status alloc_workspace(fyfft_plan* p, size_t len, bool x0) {
+  const size_t per = len * elem_bytes(p->prec);
+  p->chunk = std::max<size_t>(1, std::min(p->batch, kWorkspaceCap / per));
+  fyfft_status st = p->alloc(p->chunk * per, &p->tmp);
+  if (!st && x0) st = p->alloc(p->chunk * elem_bytes(p->prec), &p->x0buf);
+  return st;
+}
+
+// Forward FFT of a length-len FP64 sequence in place with a temporary F64
+// plan of this library (used to build kernel spectra from FP64 tables).
+static fyfft_status fft_f64_inplace(double2* buf, size_t len) {
+  fyfft_plan* sub = nullptr;
+  fyfft_status st = fyfft_plan_create(&sub, len, 1, FYFFT_F64);
+  if (st) return st;
+  st = fyfft_execute(sub, buf, buf, FYFFT_FORWARD, nullptr);
+  if (!st) {
+    hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) st = from_hip(e, "kernel spectrum FFT");
+  }
+  fyfft_plan_destroy(sub);
+  return st;
+}
+
+// Convolution structure of length m (LDS-fused or four-step) for a Rader or
+// Bluestein plan.
+static fyfft_status plan_conv_structure(fyfft_plan* p, size_t m, bool rader) {
+  p->m = m;
+  LdsGeom g;
+  if (lds_geometry(m, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) {
+    p->kind = rader ? kRaderLds : kBlueLds;
+    return plan_lds(p, m, &p->lds, false);
+  }
+  p->kind = rader ? kRaderFs : kBlueFs;
+  fyfft_status st = plan_fourstep(p, m);
+  if (!st) st = alloc_workspace(p, m, rader);
+  return st;
+}
+
+// Kernel spectrum K = FFT_m(b) / m from an FP64 sequence b (consumed),
+// stored as the plan precision, in [k1][k2] order for four-step plans.
+static fyfft_status store_spectrum(fyfft_plan* p, double2* b, int slot) {
+  fyfft_status st = fft_f64_inplace(b, p->m);
+  if (st) return st;
+  if ((st = p->alloc(p->m * elem_bytes(p->prec), &p->kern[slot]))) return st;
+  const bool fs = p->kind == kRaderFs || p->kind == kBlueFs;
+  HIP_TRY(launch_convert(p->prec, p->kern[slot], b, (long long)p->m, 1.0 / (double)p->m,
+                         (long long)p->n1, (long long)p->n2, fs, nullptr));
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status plan_rader(fyfft_plan* p) {
+  const size_t n = p->n, L = n - 1;
+  fyfft_status st = plan_conv_structure(p, L, true);
+  if (st) return st;
+  const uint64_t g = primitive_root(n), ginv = power_mod(g, n - 2, n);
+  std::vector<int> pin(L), pout(L);
+  uint64_t a = 1, b = 1;
+  for (size_t i = 0; i < L; ++i) {
+    pout[i] = (int)a;
+    pin[i] = (int)b;
+    a = a * g % n;
+    b = b * ginv % n;
+  }
+  void* d = nullptr;
+  if ((st = p->alloc(L * sizeof(int), &d))) return st;
+  p->perm_in = (int*)d;
+  if ((st = p->alloc(L * sizeof(int), &d))) return st;
+  p->perm_out = (int*)d;
+  HIP_TRY(hipMemcpy(p->perm_in, pin.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->perm_out, pout.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  double2* bb = nullptr;
+  HIP_TRY(hipMalloc(&bb, L * sizeof(double2)));
+  for (int slot = 0; slot < 2 && !st; ++slot) {
+    hipError_t e = launch_gen_rader_b(bb, p->perm_out, (long long)n, (long long)L, slot ? +1 : -1, nullptr);
+    st = e != hipSuccess ? from_hip(e, "rader kernel") : store_spectrum(p, bb, slot);
+  }
+  (void)hipFree(bb);
+  if (!st)
+    p->desc = std::string(p->kind == kRaderLds ? "rader-lds" : "rader-fourstep") + " N=" +
+              std::to_string(n) + " L=" + std::to_string(L) + " g=" + std::to_string(g);
+  return st;
+}
+
+static fyfft_status plan_bluestein(fyfft_plan* p) {
+  const size_t n = p->n, elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
+  size_t m = 0;
+  for (size_t c = 2 * n - 1; c < 4 * (2 * n - 1) + 64; ++c) {
+    if (!is_smooth7(c)) continue;
+    LdsGeom g;
+    if (lds_geometry(c, elem, cap, &g) || fs_possible(c, elem, cap)) { m = c; break; }
+  }
+  if (!m) return FYFFT_UNSUPPORTED;
+  fyfft_status st = plan_conv_structure(p, m, false);
+  if (st) return st;
+  double2* c = nullptr;
+  double2* bb = nullptr;
+  HIP_TRY(hipMalloc(&c, n * sizeof(double2)));
+  hipError_t e = hipMalloc(&bb, m * sizeof(double2));
+  if (e != hipSuccess) { (void)hipFree(c); return from_hip(e, "bluestein tables"); }
+  // chirp[0] = exp(-i pi n^2/N) (forward), chirp[1] = exp(+i pi n^2/N)
+  for (int slot = 0; slot < 2 && !st; ++slot) {
+    if ((st = p->alloc(n * elem, &p->chirp[slot]))) break;
+    e = launch_gen_chirp(c, (long long)n, (long long)n, slot ? +1 : -1, nullptr);
+    if (e == hipSuccess) e = launch_convert(p->prec, p->chirp[slot], c, (long long)n, 1.0, 0, 0, false, nullptr);
+    if (e != hipSuccess) st = from_hip(e, "bluestein chirp");
+  }
+  // c now holds exp(+i pi n^2/N); b (wrapped) and conj(b) give the spectra
+  for (int slot = 0; slot < 2 && !st; ++slot) {
+    e = launch_gen_bluestein_b(bb, c, (long long)n, (long long)m, slot == 1, nullptr);
+    st = e != hipSuccess ? from_hip(e, "bluestein kernel") : store_spectrum(p, bb, slot);
+  }
+  (void)hipFree(c);
+  (void)hipFree(bb);
+  if (!st)
+    p->desc = std::string(p->kind == kBlueLds ? "bluestein-lds" : "bluestein-fourstep") + " N=" +
+              std::to_string(n) + " M=" + std::to_string(m);
+  return st;
+}
+
+// ------------------------------------------------------------ execution
+template <typename V>
+static void fill_pass(FsArgs<V>& a, const LdsGeom& g, size_t len, const dev::PassDesc& pd,
+                      const void* tw) {
+  a.threads = g.threads;
+  a.log2c = ilog2_exact(g.per_block);
+  a.ps = g.pad_shift;
+  a.stride = dev::lds_len((int)len, g.pad_shift);
+  a.pd = pd;
+  a.tw = (const V*)tw;
+}
+
+template <typename V>
+static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int dir, bool inv_tables,
+                                  double scale, hipStream_t s) {
+  const size_t n = p->n, L = (p->kind == kFourStep) ? n : p->m;
+  for (size_t b0 = 0; b0 < p->batch; b0 += p->chunk) {
+    const size_t cb = std::min(p->chunk, p->batch - b0);
+    FsArgs<V> a;
+    a.in = in + b0 * n;
+    a.out = out + b0 * n;
+    a.tmp = (V*)p->tmp;
+    a.in_dist = (long long)n;
+    a.out_dist = (long long)n;
+    a.tmp_dist = (long long)L;
+    a.n1 = (int)p->n1;
+    a.n2 = (int)p->n2;
+    a.tw_hi = (const V*)p->tw_hi;
+    a.tw_lo = (const V*)p->tw_lo;
+    a.lo_bits = p->lo_bits;
+    a.n_orig = (int)n;
+    a.scale = (dev::S<V>)scale;
+    const int slot = inv_tables ? 1 : 0;
+    a.kern = (const V*)p->kern[slot];
+    a.chirp = (const V*)p->chirp[slot];
+    a.x0buf = (V*)p->x0buf;
+    int colA, colC = -1, rowB;
+    if (p->kind == kFourStep) { colA = kFsPlainA; rowB = kFsPlainB; }
+    else if (p->kind == kRaderFs) { colA = kFsRaderA; rowB = kFsConvB; colC = kFsRaderC; }
+    else { colA = kFsBlueA; rowB = kFsConvB; colC = kFsBlueC; }
+
+    fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
+    a.perm = p->perm_in;
+    HIP_TRY(launch_fs_pass(p->prec, true, colA, dir, &a, cb, p->gcol, s));
+    fill_pass(a, p->grow, p->n2, p->pdrow, p->twrow);
+    HIP_TRY(launch_fs_pass(p->prec, false, rowB, dir, &a, cb, p->grow, s));
+    if (colC >= 0) {
+      fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
+      a.perm = p->perm_out;
+      HIP_TRY(launch_fs_pass(p->prec, true, colC, +1, &a, cb, p->gcol, s));
+    }
+  }
+  return FYFFT_SUCCESS;
+}
+
+template <typename V>
+static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool inv_tables,
+                                  double scale, hipStream_t s) {
+  ConvArgs<V> a;
+  a.in = in;
+  a.out = out;
+  a.batch = (long long)p->batch;
+  a.dist = (long long)p->n;
+  a.n_orig = (int)p->n;
+  a.m = (int)p->m;
+  a.threads = p->lds.geom.threads;
+  a.per_block = p->lds.geom.per_block;
+  a.ps = p->lds.geom.pad_shift;
+  a.stride = dev::lds_len((int)p->m, a.ps);
+  a.pd = p->lds.pd;
+  a.tw = (const V*)p->lds.tw;
+  a.perm_in = p->perm_in;
+  a.perm_out = p->perm_out;
+  const int slot = inv_tables ? 1 : 0;
+  a.chirp = (const V*)p->chirp[slot];
+  a.kern = (const V*)p->kern[slot];
+  a.scale = (dev::S<V>)scale;
+  HIP_TRY(launch_conv_lds(p->prec, p->kind == kRaderLds, &a, p->lds.geom, s));
   return FYFFT_SUCCESS;
 }
 
@@ -157,17 +428,31 @@ fyfft_status fyfft_plan_create(fyfft_plan** out, size_t n, size_t batch, fyfft_p
     p->kind = kTrivial;
     p->desc = "trivial N=1";
   } else if (is_smooth7(n)) {
-    p->kind = kLds;
-    st = plan_lds(p, n, &p->lds);
-    if (!st) {
-      const LdsGeom& g = p->lds.geom;
-      p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") + " N=" +
-                std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
-                " T=" + std::to_string(g.threads) + " TPB=" + std::to_string(g.per_block) +
-                " pad=" + std::to_string(g.pad_shift);
+    LdsGeom g;
+    if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
+      p->kind = kLds;
+      st = plan_lds(p, n, &p->lds, true);
+      if (!st) {
+        const LdsGeom& q = p->lds.geom;
+        p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") + " N=" +
+                  std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
+                  " T=" + std::to_string(q.threads) + " TPB=" + std::to_string(q.per_block) +
+                  " pad=" + std::to_string(q.pad_shift);
+      }
+    } else {
+      p->kind = kFourStep;
+      st = plan_fourstep(p, n);
+      if (!st) st = alloc_workspace(p, n, false);
+      if (!st)
+        p->desc = "fourstep N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
+                  radix_string(p->pdcol) + ") x " + std::to_string(p->n2) + "(" +
+                  radix_string(p->pdrow) + ") C=" + std::to_string(p->gcol.per_block) + "/" +
+                  std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk);
     }
+  } else if (p->strategy == kRader) {
+    st = plan_rader(p);
   } else {
-    st = FYFFT_UNSUPPORTED;
+    st = plan_bluestein(p);
   }
   if (!st) {
     hipError_t e = hipDeviceSynchronize();
@@ -196,6 +481,7 @@ fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int d
   hipStream_t s = (hipStream_t)stream;
   const double scale = direction == FYFFT_INVERSE ? 1.0 / (double)p->n : 1.0;
   const int dir = direction == FYFFT_FORWARD ? kForward : (direction == FYFFT_INVERSE ? kInverse : kAdjoint);
+  const bool inv_tables = direction != FYFFT_FORWARD;
   switch (p->kind) {
     case kTrivial:
       if (in != out)
@@ -204,6 +490,19 @@ fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int d
     case kLds:
       HIP_TRY(launch_lds(p->prec, dir, p->lds, in, out, p->batch, p->n, scale, s));
       return FYFFT_SUCCESS;
+    case kFourStep:
+    case kRaderFs:
+    case kBlueFs: {
+      const int d = dir == kForward ? -1 : +1;
+      if (p->prec == kF64)
+        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, s);
+      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, s);
+    }
+    case kRaderLds:
+    case kBlueLds:
+      if (p->prec == kF64)
+        return exec_conv_lds<double2>(p, (const double2*)in, (double2*)out, inv_tables, scale, s);
+      return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, s);
   }
   return FYFFT_INVALID_PLAN;
 }

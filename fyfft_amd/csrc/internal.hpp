@@ -31,6 +31,81 @@ hipError_t launch_lds(int prec, int dir, const LdsLaunch& L, const void* in, voi
 // a hit the geometry is replaced by the kernel's own.
 int find_fixed_kernel(int prec, const dev::PassDesc& pd, LdsGeom* g);
 
+// --- kernels_fourstep.hip
+enum FsMode {
+  kFsPlainA = 0,  // column pass of a plain transform (x -> tmp, twiddled)
+  kFsRaderA = 1,  // column pass reading the Rader gather x[g^-j]
+  kFsBlueA = 2,   // column pass reading x * chirp, zero padded
+  kFsRaderC = 3,  // inverse column pass, Rader scatter + x0
+  kFsBlueC = 4,   // inverse column pass, post-chirp, first n_orig outputs
+  kFsPlainB = 5,  // row pass of a plain transform (tmp -> out, transposed)
+  kFsConvB = 6    // row pass of a convolution (fwd, * kernel, inv, conj twiddle)
+};
+
+template <typename V> struct FsArgs {
+  const V* in = nullptr;
+  V* out = nullptr;
+  V* tmp = nullptr;
+  long long in_dist = 0, out_dist = 0, tmp_dist = 0;  // per transform, elements
+  int n1 = 0, n2 = 0;          // column length, row length
+  int log2c = 0;               // columns (rows) per workgroup = 1 << log2c
+  int threads = 0;             // threads per sub-transform
+  int ps = 4, stride = 0;      // LDS pad shift and per-sub-transform stride
+  dev::PassDesc pd{};          // passes of the sub-length this kernel runs
+  const V* tw = nullptr;       // Stockham table of that sub-length
+  const V* tw_hi = nullptr;    // w_N^(i << lo_bits)
+  const V* tw_lo = nullptr;    // w_N^i, i < 2^lo_bits
+  int lo_bits = 0;
+  int n_orig = 0;              // Rader/Bluestein: the user length
+  const int* perm = nullptr;   // Rader gather (A) or scatter (C) permutation
+  const V* chirp = nullptr;    // Bluestein pre/post chirp
+  const V* kern = nullptr;     // kernel spectrum in [k1][k2] order, 1/L folded in
+  V* x0buf = nullptr;          // Rader: x[0] per transform
+  dev::S<V> scale = 1;
+};
+
+// One four-step pass; `args` points at FsArgs<float2> or FsArgs<double2>.
+// g gives threads per sub-transform, sub-transforms per workgroup and LDS
+// bytes.  dir (-1/+1) only matters for the plain modes.
+hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* args, size_t batch,
+                          const LdsGeom& g, hipStream_t s);
+
+// --- kernels_conv.hip
+template <typename V> struct ConvArgs {
+  const V* in = nullptr;
+  V* out = nullptr;
+  long long batch = 0, dist = 0;
+  int n_orig = 0;              // user length N
+  int m = 0;                   // convolution length (Rader: N-1, Bluestein: M)
+  int threads = 0, per_block = 1, ps = 4, stride = 0;
+  dev::PassDesc pd{};
+  const V* tw = nullptr;       // Stockham table of m
+  const int* perm_in = nullptr;   // Rader: g^-j mod N
+  const int* perm_out = nullptr;  // Rader: g^i mod N
+  const V* chirp = nullptr;    // Bluestein chirp for this direction
+  const V* kern = nullptr;     // kernel spectrum / m, natural order
+  dev::S<V> scale = 1;
+};
+// Fused single-kernel Rader (rader=true) or Bluestein, whole convolution in LDS.
+hipError_t launch_conv_lds(int prec, bool rader, const void* args, const LdsGeom& g, hipStream_t s);
+
+// Table generation (FP64 results in double2 buffers unless noted).
+// c[n] = exp(sign * i*pi*(n^2 mod 2N)/N), n < count
+hipError_t launch_gen_chirp(double2* c, long long n_orig, long long count, int sign, hipStream_t s);
+// Bluestein kernel sequence b of length M from c (optionally conjugated)
+hipError_t launch_gen_bluestein_b(double2* b, const double2* c, long long n_orig, long long m,
+                                  bool conj, hipStream_t s);
+// Rader kernel sequence b[j] = exp(sign*2*pi*i*perm_out[j]/N), j < L
+hipError_t launch_gen_rader_b(double2* b, const int* perm_out, long long n_orig, long long l,
+                              int sign, hipStream_t s);
+// out[i] = in[src(i)] * scale stored as prec; src(i) = i, or for a [k1][k2]
+// permutation of an N1 x N2 spectrum src(k1*N2 + k2) = k1 + N1*k2.
+hipError_t launch_convert(int prec, void* out, const double2* in, long long count, double scale,
+                          long long n1, long long n2, bool permute, hipStream_t s);
+// out[i] = exp(-2*pi*i * i*step / D) stored as prec, i < count
+hipError_t launch_gen_roots(int prec, void* out, long long count, long long step, long long d,
+                            hipStream_t s);
+
 // --- kernels_tables.hip
 // Stockham twiddles for pass list pd (N-1 entries) in FP64, stored as prec.
 hipError_t launch_gen_stockham_twiddles(int prec, void* tw, int n, const dev::PassDesc& pd,

@@ -131,14 +131,4 @@ bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2) {
   return true;
 }
 
-size_t bluestein_length(size_t n, size_t cap_elems) {
-  const size_t lo = 2 * n - 1;
-  for (size_t m = lo; m < 4 * lo + 64; ++m) {
-    if (!is_smooth7(m)) continue;
-    size_t a, b;
-    if (m <= cap_elems || four_step_split(m, cap_elems, &a, &b)) return m;
-  }
-  return 0;
-}
-
 }  // namespace fyfft

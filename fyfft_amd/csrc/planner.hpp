@@ -42,8 +42,4 @@ size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded
 // (n1 <= n2, as balanced as possible).  False if none exists.
 bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2);
 
-// Bluestein convolution length: the smallest 7-smooth M >= 2n-1 that is
-// either LDS-resident or has a four-step split.
-size_t bluestein_length(size_t n, size_t cap_elems);
-
 }  // namespace fyfft

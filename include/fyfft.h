@@ -38,7 +38,7 @@ extern "C" {
 /* Longest transform a plan accepts.  The reference stops at 80000
  * (foye_FFT.hpp:87); this ABI goes beyond it (Bluestein N = 1000003 and
  * smooth N = 2^20 are required workloads). */
-#define FYFFT_MAX_LENGTH ((size_t)1 << 26)
+#define FYFFT_MAX_LENGTH ((size_t)1 << 24)
 
 typedef enum fyfft_status {
   FYFFT_SUCCESS = 0,

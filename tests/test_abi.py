@@ -34,7 +34,7 @@ def test_argument_checks_without_gpu(fy):
     lib = fy.lib()
     h = ctypes.c_void_p()
     assert lib.fyfft_plan_create(ctypes.byref(h), 0, 1, 0) == 2          # invalid length
-    assert lib.fyfft_plan_create(ctypes.byref(h), (1 << 26) + 1, 1, 0) == 2
+    assert lib.fyfft_plan_create(ctypes.byref(h), (1 << 24) + 1, 1, 0) == 2
     assert lib.fyfft_plan_create(ctypes.byref(h), 64, 0, 0) == 1         # batch 0
     assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 2) == 3         # bf16 reserved
     assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 9) == 1         # unknown precision

@@ -1,0 +1,119 @@
+"""GPU parity of the four-step (N beyond one workgroup's LDS), Rader and
+Bluestein paths against the FP64 oracle, through the C ABI.
+
+Tolerances (relative L2 vs the oracle): FP32 1e-7*log2(N) + 1e-7,
+FP64 4e-16*log2(N) + 1e-15 (the convolution paths chain three FFTs)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(n, prec):
+    l2 = max(1.0, math.log2(n))
+    return 1e-7 * l2 + 1e-7 if prec == "f32" else 4e-16 * l2 + 1e-15
+
+
+def unit_phase(rng, batch, n, prec):
+    """unit-modulus random-phase inputs e^{i phi}, phi ~ U[0, 2pi)"""
+    x = np.exp(1j * rng.uniform(0, 2 * np.pi, (batch, n)))
+    return x.astype(np.complex64 if prec == "f32" else np.complex128)
+
+
+def gauss(rng, batch, n, prec):
+    x = rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))
+    return x.astype(np.complex64 if prec == "f32" else np.complex128)
+
+
+def check(fy, orc, x, expect_kind, inplace=False):
+    import torch
+    batch, n = x.shape
+    prec = "f32" if x.dtype == np.complex64 else "f64"
+    xd = torch.from_numpy(x).cuda()
+    with fy.Plan(n, batch, fy.F32 if prec == "f32" else fy.F64) as p:
+        desc = p.describe()
+        assert desc.startswith(expect_kind), desc
+        if inplace:
+            y = xd.clone()
+            p.execute(y, y, fy.FORWARD)
+        else:
+            y = p.execute(xd, None, fy.FORWARD)
+        z = p.execute(y, None, fy.INVERSE)
+        a = p.execute(xd, None, fy.ADJOINT)
+    torch.cuda.synchronize()
+    x64 = x.astype(np.complex128)
+    ref = orc.fft_rows(x64)
+    err = orc.rel_l2(y.cpu().numpy(), ref)
+    rt = orc.rel_l2(z.cpu().numpy(), x64)
+    adj = orc.rel_l2(a.cpu().numpy(), orc.fft_rows(x64, inverse=True) * n)
+    print(f"N={n} {prec} {desc}: fwd {err:.3e} round-trip {rt:.3e} adjoint {adj:.3e}")
+    t = tol(n, prec)
+    assert err < t and adj < t and rt < 2 * t, (err, rt, adj, t)
+
+
+@pytest.mark.parametrize("n,batch", [(32768, 3), (65536, 2), (78125, 2), (59049, 2),
+                                     (1 << 20, 2), (20000 * 7, 1)])
+def test_fourstep_f32(fy, orc, cuda, n, batch):
+    check(fy, orc, gauss(np.random.default_rng(n), batch, n, "f32"), "fourstep")
+
+
+@pytest.mark.parametrize("n,batch", [(16384, 2), (65536, 2)])
+def test_fourstep_f64(fy, orc, cuda, n, batch):
+    check(fy, orc, gauss(np.random.default_rng(n), batch, n, "f64"), "fourstep")
+
+
+def test_fourstep_inplace(fy, orc, cuda):
+    check(fy, orc, gauss(np.random.default_rng(1), 2, 65536, "f32"), "fourstep", inplace=True)
+
+
+@pytest.mark.parametrize("n", [11, 13, 17, 97, 257, 641, 12289])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rader_lds(fy, orc, cuda, n, prec):
+    check(fy, orc, unit_phase(np.random.default_rng(n), 5, n, prec), "rader-lds")
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rader_fourstep_65537(fy, orc, cuda, prec):
+    x = unit_phase(np.random.default_rng(65537), 3, 65537, prec)
+    check(fy, orc, x, "rader-fourstep")
+    check(fy, orc, x, "rader-fourstep", inplace=True)
+
+
+@pytest.mark.parametrize("n", [22, 23, 121, 1031, 10006, 10007])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_bluestein_lds(fy, orc, cuda, n, prec):
+    if prec == "f64" and n > 5000:
+        pytest.skip("covered by the four-step form in FP64")
+    check(fy, orc, unit_phase(np.random.default_rng(n), 4, n, prec), "bluestein-lds")
+
+
+@pytest.mark.parametrize("n,batch,prec", [(1000003, 2, "f32"), (100003, 2, "f32"),
+                                          (10007, 2, "f64")])
+def test_bluestein_fourstep(fy, orc, cuda, n, batch, prec):
+    x = unit_phase(np.random.default_rng(n), batch, n, prec)
+    check(fy, orc, x, "bluestein-fourstep")
+
+
+def test_bluestein_inplace(fy, orc, cuda):
+    x = unit_phase(np.random.default_rng(5), 3, 10007, "f32")
+    check(fy, orc, x, "bluestein-lds", inplace=True)
+
+
+def test_chunked_workspace(fy, orc, cuda):
+    """A four-step batch larger than the 1 GiB workspace cap runs in chunks;
+    spot-check the first and last transforms."""
+    import torch
+    n, batch = 1 << 20, 160   # 1.25 GiB of complex64 -> two chunks
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    x = torch.view_as_complex(torch.randn(batch, n, 2, device="cuda", generator=g))
+    with fy.Plan(n, batch, fy.F32) as p:
+        assert "chunk=128" in p.describe(), p.describe()
+        y = p.execute(x, None, fy.FORWARD)
+    torch.cuda.synchronize()
+    for r in (0, batch - 1):
+        xr = x[r].cpu().numpy().astype(np.complex128)
+        err = orc.rel_l2(y[r].cpu().numpy(), np.fft.fft(xr))
+        assert err < tol(n, "f32"), (r, err)
