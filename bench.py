@@ -25,8 +25,8 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")

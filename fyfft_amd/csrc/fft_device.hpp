@@ -55,6 +55,26 @@ template <int DIR, typename V> __device__ __forceinline__ V rotc(V a, double c, 
   return V{a.x * cc - a.y * ss, a.x * ss + a.y * cc};
 }
 
+// Streaming global accesses: every element is read once and written once, so
+// loads and stores are non-temporal (they do not displace the twiddle tables
+// from the caches).
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gload(const float2* p) {
+  const f32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p));
+  return float2{v.x, v.y};
+}
+__device__ __forceinline__ double2 gload(const double2* p) {
+  const f64x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p));
+  return double2{v.x, v.y};
+}
+__device__ __forceinline__ void gstore(float2* p, float2 v) {
+  __builtin_nontemporal_store(f32x2_t{v.x, v.y}, reinterpret_cast<f32x2_t*>(p));
+}
+__device__ __forceinline__ void gstore(double2* p, double2 v) {
+  __builtin_nontemporal_store(f64x2_t{v.x, v.y}, reinterpret_cast<f64x2_t*>(p));
+}
+
 // cos/sin(2*pi*j/R) for the odd prime radices, exact decimal expansions.
 template <int R> __device__ __forceinline__ constexpr double cos_r(int j) {
   j %= R;
@@ -207,6 +227,38 @@ template <int DIR, typename V> struct Dft<3, DIR, V> : DftOdd<3, DIR, V> {};
 template <int DIR, typename V> struct Dft<5, DIR, V> : DftOdd<5, DIR, V> {};
 template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
 
+// ---------------------------------------------------------------- twiddles
+// Multiply inputs r = 1..R-1 of one butterfly by w^(k*r), where t points at
+// the pass table slice [w^k, w^2k, ..., w^((R-1)k)].  For R = 4, 7, 8 and 16
+// only the power-of-two powers are loaded and the others are formed by at
+// most three complex products (a few ulp), which halves to quarters the
+// table traffic of the pass; R = 2, 3, 5 load every factor.
+template <int R, int DIR, typename V>
+__device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
+  if constexpr (R == 16) {
+    const V w1 = t[0], w2 = t[1], w4 = t[3], w8 = t[7];
+    const V w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+    const V w[16] = {V{1, 0}, w1, w2, w3, w4, w5, w6, w7, w8, cmul(w1, w8), cmul(w2, w8),
+                     cmul(w3, w8), cmul(w4, w8), cmul(w5, w8), cmul(w6, w8), cmul(w7, w8)};
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = twmul<DIR>(v[r], w[r]);
+  } else if constexpr (R == 8 || R == 7) {
+    const V w1 = t[0], w2 = t[1], w4 = t[3];
+    const V w3 = cmul(w1, w2);
+    const V w[8] = {V{1, 0}, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4)};
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r]);
+  } else if constexpr (R == 4) {
+    const V w1 = t[0], w2 = t[1];
+    v[1] = twmul<DIR>(v[1], w1);
+    v[2] = twmul<DIR>(v[2], w2);
+    v[3] = twmul<DIR>(v[3], cmul(w1, w2));
+  } else {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], t[r - 1]);
+  }
+}
+
 // ---------------------------------------------------------------- LDS layout
 // One pad element every 16 complex values breaks the power-of-two strides
 // of the pass read/write patterns across LDS banks.
@@ -243,13 +295,9 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int idx = j + r * nb;
-        v[b][r] = SRC_G ? src[idx] : src[idx + (idx >> ps)];
+        v[b][r] = SRC_G ? gload(src + idx) : src[idx + (idx >> ps)];
       }
-      if (Ls > 1) {
-        const V* t = tw + (Ls - 1) + k * (R - 1);
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = twmul<DIR>(v[b][r], t[r - 1]);
-      }
+      if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
       Dft<R, DIR, V>::run(v[b]);
     }
   }
@@ -263,7 +311,7 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const int idx = base + r * Ls;
-        if (DST_G) dst[idx] = cscale(v[b][r], scale);
+        if (DST_G) gstore(dst + idx, cscale(v[b][r], scale));
         else dst[idx + (idx >> ps)] = v[b][r];
       }
     }

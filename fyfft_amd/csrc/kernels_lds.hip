@@ -30,11 +30,11 @@ __global__ __launch_bounds__(1024) void k_lds_generic(const V* in, V* out, long 
   const V* gin = in + (active ? t : 0) * dist;
   V* gout = out + (active ? t : 0) * dist;
   if (active)
-    for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gin[i];
+    for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gload(gin + i);
   __syncthreads();
   lds_fft<V, PPT, DIR>(lds, N, T, tid, pd, tw, active, ps);
   if (active)
-    for (int i = tid; i < N; i += T) gout[i] = cscale(lds[i + (i >> ps)], scale);
+    for (int i = tid; i < N; i += T) gstore(gout + i, cscale(lds[i + (i >> ps)], scale));
 }
 
 // ------------------------------------------------------------ fixed kernels
@@ -68,42 +68,45 @@ __global__ __launch_bounds__(T * TPB) void k_lds_fixed(const V* in, V* out, long
 using FixedLauncher = hipError_t (*)(int dir, const void* in, void* out, size_t batch,
                                      size_t dist, const void* tw, double scale, hipStream_t s);
 
-template <typename V, int N, int T, int TPB, int... Rs>
+template <typename V, int N, int T, int TPB, int PS, int... Rs>
 static hipError_t launch_fixed(int dir, const void* in, void* out, size_t batch, size_t dist,
                                const void* tw, double scale, hipStream_t s) {
   const dim3 grid((unsigned)((batch + TPB - 1) / TPB)), block(T * TPB);
   if (dir == kForward)
-    hipLaunchKernelGGL((k_lds_fixed<V, -1, N, T, TPB, kPadShift, Rs...>), grid, block, 0, s,
+    hipLaunchKernelGGL((k_lds_fixed<V, -1, N, T, TPB, PS, Rs...>), grid, block, 0, s,
                        (const V*)in, (V*)out, (long long)batch, (long long)dist, (const V*)tw,
                        (S<V>)scale);
   else
-    hipLaunchKernelGGL((k_lds_fixed<V, +1, N, T, TPB, kPadShift, Rs...>), grid, block, 0, s,
+    hipLaunchKernelGGL((k_lds_fixed<V, +1, N, T, TPB, PS, Rs...>), grid, block, 0, s,
                        (const V*)in, (V*)out, (long long)batch, (long long)dist, (const V*)tw,
                        (S<V>)scale);
   return hipGetLastError();
 }
 
 struct FixedEntry {
-  int prec, n, threads, per_block, npass;
+  int prec, n, threads, per_block, pad_shift, npass;
   int radix[6];
   FixedLauncher launch;
 };
 
-#define FIXED(P, V, N, T, TPB, NP, ...) \
-  {P, N, T, TPB, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, __VA_ARGS__>}
+// PS = 30 stores the transform unpadded: for these power-of-two radix-8/16
+// pass orders the access patterns stay within two LDS cycles per wave, and
+// the smaller footprint lets five 4096-point workgroups share a CU.
+#define FIXED(P, V, N, T, TPB, PS, NP, ...) \
+  {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, PS, __VA_ARGS__>}
 
 // Radix lists here must equal radices_for(N) (planner.cpp) so that the plan's
 // twiddle table matches; find_fixed_kernel checks this at plan time.
 static const FixedEntry kFixed[] = {
-    FIXED(kF32, float2, 256, 16, 16, 2, 16, 16),
-    FIXED(kF32, float2, 512, 32, 8, 3, 8, 8, 8),
-    FIXED(kF32, float2, 1024, 64, 4, 3, 16, 8, 8),
-    FIXED(kF32, float2, 2048, 128, 2, 3, 16, 16, 8),
-    FIXED(kF32, float2, 4096, 256, 1, 3, 16, 16, 16),
-    FIXED(kF32, float2, 8192, 512, 1, 4, 16, 8, 8, 8),
-    FIXED(kF32, float2, 16384, 1024, 1, 4, 16, 16, 8, 8),
-    FIXED(kF64, double2, 1024, 64, 4, 3, 16, 8, 8),
-    FIXED(kF64, double2, 4096, 256, 1, 3, 16, 16, 16),
+    FIXED(kF32, float2, 256, 16, 16, 30, 2, 16, 16),
+    FIXED(kF32, float2, 512, 32, 8, 30, 3, 8, 8, 8),
+    FIXED(kF32, float2, 1024, 64, 4, 30, 3, 16, 8, 8),
+    FIXED(kF32, float2, 2048, 128, 2, 30, 3, 16, 16, 8),
+    FIXED(kF32, float2, 4096, 256, 1, 30, 3, 16, 16, 16),
+    FIXED(kF32, float2, 8192, 512, 1, 30, 4, 16, 8, 8, 8),
+    FIXED(kF32, float2, 16384, 1024, 1, 30, 4, 16, 16, 8, 8),
+    FIXED(kF64, double2, 1024, 64, 4, 30, 3, 16, 8, 8),
+    FIXED(kF64, double2, 4096, 256, 1, 30, 3, 16, 16, 16),
 };
 #undef FIXED
 
@@ -116,8 +119,8 @@ int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
     if (!same) continue;
     g->threads = e.threads;
     g->per_block = e.per_block;
-    g->pad_shift = kPadShift;
-    g->lds_bytes = (size_t)lds_len(e.n, kPadShift) * e.per_block * elem_bytes(prec);
+    g->pad_shift = e.pad_shift;
+    g->lds_bytes = (size_t)lds_len(e.n, e.pad_shift) * e.per_block * elem_bytes(prec);
     return i;
   }
   return -1;
