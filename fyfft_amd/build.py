@@ -63,5 +63,19 @@ def build(jobs: int | None = None) -> str:
     return LIB
 
 
+def build_dropin_test() -> str:
+    """Compile tests/cpp/dropin_main.cpp with plain g++ against the C ABI and
+    include/foye_FFT.hpp (no HIP headers needed on the caller's side)."""
+    exe = os.path.join(LIBDIR, "dropin_main")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+           os.path.join(ROOT, "tests", "cpp", "dropin_main.cpp"), "-o", exe,
+           f"-L{LIBDIR}", "-lfyfft", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in test build failed:\n{r.stderr}")
+    return exe
+
+
 if __name__ == "__main__":
     print(build())
+    print(build_dropin_test())
