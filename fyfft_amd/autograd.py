@@ -1,0 +1,108 @@
+"""Differentiable transforms over the C ABI.
+
+* :func:`fft` / :func:`ifft` -- C2C transforms whose backward is the
+  hand-written adjoint kernel: for y = F x the gradient is F^H g
+  (FYFFT_ADJOINT: conjugate twiddles, no 1/N); for y = F^{-1} x = F^H x / N it
+  is F g / N.
+* :class:`SpectralConv1d` -- the 1-D spectral convolution
+      y = irfft( (rfft(x)[..., :K]) mixed over channels by W )
+  computed as two fused transforms (real-input FFT keeping K modes, K-mode
+  real inverse) around a batched complex GEMM over the modes; its backward
+  is written by hand from the same kernels.
+
+Gradients follow torch's convention for complex tensors
+(dL/dRe z + i dL/dIm z).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import ADJOINT, FORWARD, INVERSE, _run, irfft_modes, rfft_modes
+
+
+class _FFT(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return _run(x, FORWARD)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _run(g.contiguous(), ADJOINT)
+
+
+class _IFFT(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return _run(x, INVERSE)
+
+    @staticmethod
+    def backward(ctx, g):
+        n = g.shape[-1]
+        return _run(g.contiguous(), FORWARD) / n
+
+
+def fft(x):
+    """Differentiable forward C2C FFT over the last dimension."""
+    return _FFT.apply(x)
+
+
+def ifft(x):
+    """Differentiable inverse C2C FFT (1/N) over the last dimension."""
+    return _IFFT.apply(x)
+
+
+def mix_modes(xm, w):
+    """out[b, o, k] = sum_i xm[b, i, k] * w[i, o, k] as one batched GEMM over
+    the modes ([K, B, Ci] @ [K, Ci, Co])."""
+    a = xm.permute(2, 0, 1).contiguous()
+    b = w.permute(2, 0, 1).contiguous()
+    return torch.matmul(a, b).permute(1, 2, 0).contiguous()
+
+
+class SpectralConv1dFunction(torch.autograd.Function):
+    """y = Re(IFFT_L(Z)), Z[0] = Y[0], Z[k] = 2 Y[k] (0 < k < K), Z = 0 above,
+    Y[b,o,k] = sum_i X[b,i,k] W[i,o,k], X = FFT_L(x)[..., :K]."""
+
+    @staticmethod
+    def forward(ctx, x, w):
+        L = x.shape[-1]
+        K = w.shape[-1]
+        if 2 * K > L:
+            raise ValueError("SpectralConv1d: modes must satisfy 2*K <= L")
+        xm = rfft_modes(x, K)                       # [B, Ci, K]
+        ym = mix_modes(xm, w)                       # [B, Co, K]
+        y = irfft_modes(ym, L, 1.0 / L, 2.0 / L)    # [B, Co, L]
+        ctx.save_for_backward(xm, w)
+        ctx.L = L
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        xm, w = ctx.saved_tensors
+        L, K = ctx.L, w.shape[-1]
+        # dL/dY = S * (1/L) * FFT(g) restricted to the kept modes (S = 1, 2, 2, ...)
+        gy = rfft_modes(g.contiguous(), K, 1.0 / L, 2.0 / L)      # [B, Co, K]
+        gx_modes = mix_modes(gy, w.conj().transpose(0, 1))          # [B, Ci, K]
+        gw = torch.matmul(xm.conj().permute(2, 1, 0).contiguous(),  # [K, Ci, B]
+                          gy.permute(2, 0, 1).contiguous())         # [K, B, Co]
+        gw = gw.permute(1, 2, 0).contiguous()                       # [Ci, Co, K]
+        # dL/dx = Re(F^H pad(gX)): the real part of the adjoint over K modes
+        gx = irfft_modes(gx_modes, L, 1.0, 1.0)
+        return gx, gw
+
+
+class SpectralConv1d(torch.nn.Module):
+    """1-D spectral convolution layer: in_channels -> out_channels keeping
+    `modes` Fourier modes; weights ~ N(0, 1/in_channels) complex."""
+
+    def __init__(self, in_channels: int, out_channels: int, modes: int, dtype=torch.complex64,
+                 device=None, generator=None):
+        super().__init__()
+        scale = (1.0 / in_channels) ** 0.5
+        real_dt = torch.float32 if dtype == torch.complex64 else torch.float64
+        w = torch.randn(in_channels, out_channels, modes, 2, dtype=real_dt, device=device,
+                        generator=generator) * (scale / 2 ** 0.5)
+        self.weight = torch.nn.Parameter(torch.view_as_complex(w))
+
+    def forward(self, x):
+        return SpectralConv1dFunction.apply(x, self.weight)

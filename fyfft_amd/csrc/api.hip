@@ -370,6 +370,43 @@ static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool
   return FYFFT_SUCCESS;
 }
 
+template <typename V>
+static fyfft_status exec_spectral(const fyfft_plan* p, bool inverse, const void* src, void* dst,
+                                  size_t k, double scale0, double scale, hipStream_t s) {
+  SpecArgs<V> a;
+  if (inverse) {
+    a.cplx_in = (const V*)src;
+    a.real_out = (dev::S<V>*)dst;
+  } else {
+    a.real_in = (const dev::S<V>*)src;
+    a.cplx_out = (V*)dst;
+  }
+  const LdsGeom& g = p->lds.geom;
+  a.rows = (long long)p->batch;
+  a.n = (int)p->n;
+  a.k = (int)k;
+  a.threads = g.threads;
+  a.per_block = g.per_block;
+  a.ps = g.pad_shift;
+  a.stride = dev::lds_len((int)p->n, g.pad_shift);
+  a.pd = p->lds.pd;
+  a.tw = (const V*)p->lds.tw;
+  a.a0 = (dev::S<V>)scale0;
+  a.a = (dev::S<V>)scale;
+  HIP_TRY(launch_spectral(p->prec, inverse, &a, g, s));
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src, void* dst,
+                             size_t k, double scale0, double scale, void* stream) {
+  if (!valid(p)) return FYFFT_INVALID_PLAN;
+  if (!src || !dst || k < 1 || k > p->n) return FYFFT_INVALID_ARGUMENT;
+  if (p->kind != kLds) return FYFFT_UNSUPPORTED;
+  if (p->prec == kF64)
+    return exec_spectral<double2>(p, inverse, src, dst, k, scale0, scale, (hipStream_t)stream);
+  return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, (hipStream_t)stream);
+}
+
 extern "C" {
 
 const char* fyfft_status_string(fyfft_status s) {
@@ -505,6 +542,16 @@ fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int d
       return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, s);
   }
   return FYFFT_INVALID_PLAN;
+}
+
+fyfft_status fyfft_rfft_modes(const fyfft_plan* p, const void* x, void* modes, size_t k,
+                              double scale0, double scale, void* stream) {
+  return spectral(p, false, x, modes, k, scale0, scale, stream);
+}
+
+fyfft_status fyfft_irfft_modes(const fyfft_plan* p, const void* modes, void* y, size_t k,
+                               double scale0, double scale, void* stream) {
+  return spectral(p, true, modes, y, k, scale0, scale, stream);
 }
 
 fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, int direction) {

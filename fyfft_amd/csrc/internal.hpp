@@ -106,6 +106,21 @@ hipError_t launch_convert(int prec, void* out, const double2* in, long long coun
 hipError_t launch_gen_roots(int prec, void* out, long long count, long long step, long long d,
                             hipStream_t s);
 
+// --- kernels_spectral.hip
+template <typename V> struct SpecArgs {
+  const dev::S<V>* real_in = nullptr;  // rfft_modes input, rows x n
+  V* cplx_out = nullptr;               // rfft_modes output, rows x k
+  const V* cplx_in = nullptr;          // irfft_modes input, rows x k
+  dev::S<V>* real_out = nullptr;       // irfft_modes output, rows x n
+  long long rows = 0;
+  int n = 0, k = 0;
+  int threads = 0, per_block = 1, ps = 4, stride = 0;
+  dev::PassDesc pd{};
+  const V* tw = nullptr;
+  dev::S<V> a0 = 1, a = 1;             // scale of mode 0 and of modes 1..k-1
+};
+hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGeom& g, hipStream_t s);
+
 // --- kernels_tables.hip
 // Stockham twiddles for pass list pd (N-1 entries) in FP64, stored as prec.
 hipError_t launch_gen_stockham_twiddles(int prec, void* tw, int n, const dev::PassDesc& pd,

@@ -119,6 +119,7 @@ int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
     if (!same) continue;
     g->threads = e.threads;
     g->per_block = e.per_block;
+    g->ppt = e.n / e.threads;  // the generic kernels reuse this geometry
     g->pad_shift = e.pad_shift;
     g->lds_bytes = (size_t)lds_len(e.n, e.pad_shift) * e.per_block * elem_bytes(prec);
     return i;

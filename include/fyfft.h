@@ -122,6 +122,25 @@ fyfft_status fyfft_length_strategy(size_t n, int* strategy);
  * (mix_radix.hpp:253-352, 577-598). */
 fyfft_status fyfft_length_radices(size_t n, int* radices, int max, int* count);
 
+/* Spectral-convolution transforms (no counterpart in the reference, which
+ * has neither real-input transforms nor differentiation; they serve the
+ * fused spectral convolution and its hand-written adjoint).  `plan` is an
+ * F32 or F64 plan of length L whose batch is the number of rows; L must fit
+ * one workgroup's LDS (FYFFT_UNSUPPORTED otherwise).  c_0 = scale0 and
+ * c_j = scale for j >= 1; 1 <= k <= L.  Asynchronous on `stream`.
+ *
+ * fyfft_rfft_modes:  x is rows x L real (float/double), modes is rows x k
+ *   complex:  modes[r][j] = c_j * sum_n x[r][n] exp(-2*pi*i*n*j/L).
+ * fyfft_irfft_modes: modes is rows x k complex, y is rows x L real:
+ *   y[r][n] = Re( sum_{j<k} c_j * modes[r][j] * exp(+2*pi*i*n*j/L) ).
+ *   scale0 = 1/L, scale = 2/L gives the real inverse of a Hermitian
+ *   spectrum holding the k kept modes (k <= L/2); scale0 = scale = 1 gives
+ *   the real part of the adjoint transform. */
+fyfft_status fyfft_rfft_modes(const fyfft_plan* plan, const void* x, void* modes, size_t k,
+                              double scale0, double scale, void* stream);
+fyfft_status fyfft_irfft_modes(const fyfft_plan* plan, const void* modes, void* y, size_t k,
+                               double scale0, double scale, void* stream);
+
 /* Device bytes the plan holds (tables + workspace). */
 fyfft_status fyfft_plan_workspace_bytes(const fyfft_plan* plan, size_t* bytes);
 
