@@ -93,6 +93,8 @@ def _load() -> ctypes.CDLL:
         dbl = ctypes.c_double
         L.fyfft_rfft_modes.argtypes = [vp, vp, vp, sz, dbl, dbl, vp]
         L.fyfft_irfft_modes.argtypes = [vp, vp, vp, sz, dbl, dbl, vp]
+        L.fyfft_plan_create_grouped.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(sz), sz, i, ctypes.c_uint]
+        L.fyfft_plan_elements.argtypes = [vp, ctypes.POINTER(sz)]
         L.fyfft_status_string.argtypes = [i]
         L.fyfft_status_string.restype = ctypes.c_char_p
         L.fyfft_last_error.argtypes = []
@@ -100,7 +102,8 @@ def _load() -> ctypes.CDLL:
         for name in ("fyfft_plan_create", "fyfft_plan_destroy", "fyfft_execute",
                      "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
-                     "fyfft_rfft_modes", "fyfft_irfft_modes"):
+                     "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_plan_create_grouped",
+                     "fyfft_plan_elements"):
             getattr(L, name).restype = i
         _lib = L
         return L
@@ -272,6 +275,30 @@ def irfft_modes(modes, L: int, scale0: float, scale: float, stream=None):
                                      k, float(scale0), float(scale),
                                      ctypes.c_void_p(_stream_handle(stream, modes.device))), "irfft_modes")
     return y
+
+
+GROUPED_GRAPH = 1
+GROUPED_PER_TRANSFORM = 2
+
+
+class GroupedPlan(Plan):
+    """Ragged batch: transforms of lengths `lengths`, packed back to back in
+    one 1-D device tensor; executed as one grouped launch per phase.
+    flags: GROUPED_GRAPH (capture/replay a hipGraph per (in, out, direction,
+    stream)), GROUPED_PER_TRANSFORM (baseline: one plain launch per
+    transform)."""
+
+    def __init__(self, lengths, precision: int = F32, flags: int = 0):
+        lens = [int(n) for n in lengths]
+        arr = (ctypes.c_size_t * len(lens))(*lens)
+        self.lengths = lens
+        self.precision = int(precision)
+        self.offsets = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.int64) if lens else []
+        self.total = int(sum(lens))
+        self.n, self.batch = self.total, 1
+        self._h = ctypes.c_void_p()
+        _check(_load().fyfft_plan_create_grouped(ctypes.byref(self._h), arr, len(lens), self.precision,
+                                                 int(flags)), f"plan_create_grouped(count={len(lens)})")
 
 
 _plan_cache: dict = {}

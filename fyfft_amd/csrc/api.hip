@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -19,7 +21,7 @@ namespace {
 constexpr uint32_t kMagic = 0xF1FF7A11u;
 thread_local std::string g_last_error;
 
-enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6 };
+enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7 };
 
 // Workspace of multi-pass plans is capped; larger batches run in chunks.
 constexpr size_t kWorkspaceCap = (size_t)1 << 30;
@@ -65,6 +67,19 @@ struct fyfft_plan {
   void* chirp[2] = {nullptr, nullptr};
   int* perm_in = nullptr;
   int* perm_out = nullptr;
+  // grouped plans
+  unsigned gflags = 0;
+  size_t count = 0, total = 0;               // transforms, sum of lengths
+  std::vector<fyfft_plan*> subs;             // one plan per distinct length
+  std::vector<int> sub_of;                   // transform -> index into subs
+  std::vector<size_t> h_off;                 // transform -> packed element offset
+  void *d_tasks = nullptr, *d_lens = nullptr, *d_off = nullptr, *d_woff = nullptr;
+  void *gws = nullptr, *gx0 = nullptr;
+  int task_base[3] = {0, 0, 0}, task_count[3] = {0, 0, 0};
+  size_t phase_lds[3] = {0, 0, 0};
+  unsigned phase_grid[3] = {0, 0, 0};
+  mutable std::mutex graph_mu;
+  mutable std::map<std::tuple<const void*, void*, int, hipStream_t>, hipGraphExec_t> graphs;
   std::vector<void*> allocs;
   size_t dev_bytes = 0;
   std::string desc;
@@ -80,6 +95,8 @@ struct fyfft_plan {
     return FYFFT_SUCCESS;
   }
   ~fyfft_plan() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (fyfft_plan* sp : subs) delete sp;
     for (void* p : allocs) (void)hipFree(p);
     if (d_io) (void)hipFree(d_io);
     magic = 0;
@@ -180,9 +197,11 @@ static fyfft_status alloc_workspace(fyfft_plan* p, size_t len, bool x0) {
 
 // Forward FFT of a length-len FP64 sequence in place with a temporary F64
 // plan of this library (used to build kernel spectra from FP64 tables).
+static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, bool);
+
 static fyfft_status fft_f64_inplace(double2* buf, size_t len) {
   fyfft_plan* sub = nullptr;
-  fyfft_status st = fyfft_plan_create(&sub, len, 1, FYFFT_F64);
+  fyfft_status st = create_plan_impl(&sub, len, 1, FYFFT_F64, true);
   if (st) return st;
   st = fyfft_execute(sub, buf, buf, FYFFT_FORWARD, nullptr);
   if (!st) {
@@ -407,6 +426,277 @@ static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src,
   return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, (hipStream_t)stream);
 }
 
+static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
+                                     bool allow_fixed) {
+  if (!out) return FYFFT_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (batch == 0) return FYFFT_INVALID_ARGUMENT;
+  if (precision != FYFFT_F32 && precision != FYFFT_F64) {
+    if (precision == FYFFT_BF16 || precision == FYFFT_F16) return FYFFT_UNSUPPORTED;
+    return FYFFT_INVALID_ARGUMENT;
+  }
+  if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  fyfft_plan* p = new (std::nothrow) fyfft_plan();
+  if (!p) return FYFFT_ALLOC_FAILED;
+  p->n = n;
+  p->batch = batch;
+  p->prec = precision == FYFFT_F64 ? kF64 : kF32;
+  p->device = dev;
+  p->strategy = strategy_for(n);
+  fyfft_status st = FYFFT_SUCCESS;
+  if (n == 1) {
+    p->kind = kTrivial;
+    p->desc = "trivial N=1";
+  } else if (is_smooth7(n)) {
+    LdsGeom g;
+    if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
+      p->kind = kLds;
+      st = plan_lds(p, n, &p->lds, allow_fixed);
+      if (!st) {
+        const LdsGeom& q = p->lds.geom;
+        p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") + " N=" +
+                  std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
+                  " T=" + std::to_string(q.threads) + " TPB=" + std::to_string(q.per_block) +
+                  " pad=" + std::to_string(q.pad_shift);
+      }
+    } else {
+      p->kind = kFourStep;
+      st = plan_fourstep(p, n);
+      if (!st) st = alloc_workspace(p, n, false);
+      if (!st)
+        p->desc = "fourstep N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
+                  radix_string(p->pdcol) + ") x " + std::to_string(p->n2) + "(" +
+                  radix_string(p->pdrow) + ") C=" + std::to_string(p->gcol.per_block) + "/" +
+                  std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk);
+    }
+  } else if (p->strategy == kRader) {
+    st = plan_rader(p);
+  } else {
+    st = plan_bluestein(p);
+  }
+  if (!st) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) st = from_hip(e, "plan table generation");
+  }
+  if (st) {
+    delete p;
+    return st;
+  }
+  *out = p;
+  return FYFFT_SUCCESS;
+}
+
+
+// ------------------------------------------------------------ grouped plans
+template <typename V>
+static GLen<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, int* per_block) {
+  GLen<V> g;
+  g.n = (int)s->n;
+  *per_block = 1;
+  auto fill_lds = [&](size_t m) {
+    const LdsGeom& q = s->lds.geom;
+    g.lds.n = (int)s->n;
+    g.lds.m = (int)m;
+    g.lds.threads = q.threads;
+    g.lds.ps = q.pad_shift;
+    g.lds.stride = dev::lds_len((int)m, q.pad_shift);
+    g.lds.pd = s->lds.pd;
+    g.lds.tw = (const V*)s->lds.tw;
+    g.lds_ppt = q.ppt;
+    const size_t per = (size_t)g.lds.stride * sizeof(V);
+    size_t pb = std::min<size_t>(1024 / q.threads, cap / per);
+    *per_block = (int)std::max<size_t>(1, std::min(pb, count_len));
+  };
+  auto fill_fs = [&](FsArgs<V>& a, const LdsGeom& q, size_t len, const dev::PassDesc& pd, const void* tw) {
+    a.n1 = (int)s->n1;
+    a.n2 = (int)s->n2;
+    a.log2c = ilog2_exact(q.per_block);
+    a.threads = q.threads;
+    a.ps = q.pad_shift;
+    a.stride = dev::lds_len((int)len, q.pad_shift);
+    a.pd = pd;
+    a.tw = (const V*)tw;
+    a.tw_hi = (const V*)s->tw_hi;
+    a.tw_lo = (const V*)s->tw_lo;
+    a.lo_bits = s->lo_bits;
+    a.n_orig = (int)s->n;
+    a.perm = s->perm_in;
+    a.chirp = (const V*)s->chirp[0];
+    a.kern = (const V*)s->kern[0];
+  };
+  g.lds.perm_in = s->perm_in;
+  g.lds.perm_out = s->perm_out;
+  g.lds.chirp = (const V*)s->chirp[0];
+  g.lds.kern = (const V*)s->kern[0];
+  g.chirp_inv = (const V*)s->chirp[1];
+  g.kern_inv = (const V*)s->kern[1];
+  switch (s->kind) {
+    case kTrivial:
+      g.kind = kGkLds;
+      g.lds.n = g.lds.m = 1;
+      g.lds.threads = 1;
+      g.lds.ps = 30;
+      g.lds.stride = 1;
+      g.lds.pd.npass = 0;
+      *per_block = (int)std::max<size_t>(1, std::min<size_t>(1024, count_len));
+      break;
+    case kLds: g.kind = kGkLds; fill_lds(s->n); break;
+    case kRaderLds: g.kind = kGkRader; fill_lds(s->m); break;
+    case kBlueLds: g.kind = kGkBlue; fill_lds(s->m); break;
+    default:
+      g.kind = s->kind == kFourStep ? kGkFsPlain : (s->kind == kRaderFs ? kGkFsRader : kGkFsBlue);
+      fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol);
+      fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow);
+      g.col_ppt = s->gcol.ppt;
+      g.row_ppt = s->grow.ppt;
+      break;
+  }
+  return g;
+}
+
+template <typename V>
+static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vector<size_t>>& by_len) {
+  const size_t cap = lds_cap_bytes(p->device), elem = sizeof(V);
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || ncu <= 0) ncu = 256;
+  std::vector<GLen<V>> lens;
+  std::vector<long long> off, woff;
+  struct T { GTask t; double cost; };
+  std::vector<T> phase[3];
+  long long wsize = 0;
+  int li = 0;
+  for (const auto& kv : by_len) {
+    const fyfft_plan* s = p->subs[li];
+    const std::vector<size_t>& idx = kv.second;
+    int per_block = 1;
+    lens.push_back(make_glen<V>(s, idx.size(), cap, &per_block));
+    const GLen<V>& g = lens.back();
+    const int first = (int)off.size();
+    for (size_t i : idx) {
+      off.push_back((long long)p->h_off[i]);
+      const bool fs = g.kind >= kGkFsPlain;
+      woff.push_back(fs ? wsize : 0);
+      if (fs) wsize += (long long)(g.col.n1) * g.col.n2;
+    }
+    const double lg = std::log2((double)std::max<size_t>(2, kv.first));
+    if (g.kind <= kGkBlue) {
+      const size_t m = (size_t)g.lds.m;
+      const double conv = g.kind == kGkLds ? 1.0 : 2.0;
+      for (size_t i = 0; i < idx.size(); i += per_block) {
+        const int c = (int)std::min<size_t>(per_block, idx.size() - i);
+        phase[0].push_back({GTask{li, 0, first + (int)i, c}, conv * c * m * lg});
+        p->phase_lds[0] = std::max(p->phase_lds[0], (size_t)per_block * g.lds.stride * elem);
+      }
+    } else {
+      const int C = 1 << g.col.log2c, R = 1 << g.row.log2c;
+      const double conv = g.kind == kGkFsPlain ? 1.0 : 2.0;
+      for (size_t i = 0; i < idx.size(); ++i) {
+        for (int c0 = 0; c0 < g.col.n2; c0 += C) {
+          phase[0].push_back({GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg});
+          if (g.kind != kGkFsPlain) phase[2].push_back({GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg});
+        }
+        for (int r0 = 0; r0 < g.col.n1; r0 += R)
+          phase[1].push_back({GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg});
+      }
+      p->phase_lds[0] = std::max(p->phase_lds[0], s->gcol.lds_bytes);
+      p->phase_lds[1] = std::max(p->phase_lds[1], s->grow.lds_bytes);
+      if (g.kind != kGkFsPlain) p->phase_lds[2] = std::max(p->phase_lds[2], s->gcol.lds_bytes);
+    }
+    ++li;
+  }
+  std::vector<GTask> tasks;
+  for (int ph = 0; ph < 3; ++ph) {
+    std::stable_sort(phase[ph].begin(), phase[ph].end(), [](const T& a, const T& b) { return a.cost > b.cost; });
+    p->task_base[ph] = (int)tasks.size();
+    p->task_count[ph] = (int)phase[ph].size();
+    for (const T& t : phase[ph]) tasks.push_back(t.t);
+    const size_t lds = std::max<size_t>(p->phase_lds[ph], 16);
+    const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(2, cap / lds));
+    p->phase_grid[ph] = (unsigned)std::min<size_t>(phase[ph].size(), (size_t)ncu * per_cu);
+  }
+  fyfft_status st;
+  if ((st = p->alloc(std::max<size_t>(1, tasks.size()) * sizeof(GTask), &p->d_tasks)) ||
+      (st = p->alloc(lens.size() * sizeof(GLen<V>), &p->d_lens)) ||
+      (st = p->alloc(off.size() * sizeof(long long), &p->d_off)) ||
+      (st = p->alloc(woff.size() * sizeof(long long), &p->d_woff)) ||
+      (st = p->alloc(std::max<long long>(1, wsize) * elem, &p->gws)) ||
+      (st = p->alloc(p->count * elem, &p->gx0)))
+    return st;
+  HIP_TRY(hipMemcpy(p->d_tasks, tasks.data(), tasks.size() * sizeof(GTask), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->d_lens, lens.data(), lens.size() * sizeof(GLen<V>), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->d_off, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->d_woff, woff.data(), woff.size() * sizeof(long long), hipMemcpyHostToDevice));
+  HIP_TRY(prepare_grouped(p->prec, std::max({p->phase_lds[0], p->phase_lds[1], p->phase_lds[2]})));
+  p->desc = "grouped count=" + std::to_string(p->count) + " lengths=" + std::to_string(lens.size()) +
+            " total=" + std::to_string(p->total) + " tasks=" + std::to_string(p->task_count[0]) + "/" +
+            std::to_string(p->task_count[1]) + "/" + std::to_string(p->task_count[2]) +
+            " workspace=" + std::to_string(wsize);
+  return FYFFT_SUCCESS;
+}
+
+template <typename V>
+static fyfft_status launch_grouped_phases(const fyfft_plan* p, const void* in, void* out, int direction,
+                                          hipStream_t s) {
+  GroupedArgs<V> a;
+  a.lens = (const GLen<V>*)p->d_lens;
+  a.off = (const long long*)p->d_off;
+  a.woff = (const long long*)p->d_woff;
+  a.in = (const V*)in;
+  a.out = (V*)out;
+  a.ws = (V*)p->gws;
+  a.x0buf = (V*)p->gx0;
+  a.tables = direction != FYFFT_FORWARD;
+  a.normalize = direction == FYFFT_INVERSE;
+  const int dir = direction == FYFFT_FORWARD ? -1 : +1;
+  for (int ph = 0; ph < 3; ++ph) {
+    if (!p->task_count[ph]) continue;
+    a.tasks = (const GTask*)p->d_tasks + p->task_base[ph];
+    a.ntasks = p->task_count[ph];
+    HIP_TRY(launch_grouped(p->prec, dir, &a, p->phase_grid[ph], std::max<size_t>(p->phase_lds[ph], 16), s));
+  }
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status exec_grouped(const fyfft_plan* p, const void* in, void* out, int direction, hipStream_t s) {
+  const size_t elem = elem_bytes(p->prec);
+  if (p->gflags & FYFFT_GROUPED_PER_TRANSFORM) {
+    for (size_t i = 0; i < p->count; ++i) {
+      const fyfft_status st = fyfft_execute(p->subs[p->sub_of[i]], (const char*)in + p->h_off[i] * elem,
+                                            (char*)out + p->h_off[i] * elem, direction, s);
+      if (st) return st;
+    }
+    return FYFFT_SUCCESS;
+  }
+  auto run = [&]() {
+    return p->prec == kF64 ? launch_grouped_phases<double2>(p, in, out, direction, s)
+                           : launch_grouped_phases<float2>(p, in, out, direction, s);
+  };
+  if (!(p->gflags & FYFFT_GROUPED_GRAPH) || s == nullptr) return run();
+  std::lock_guard<std::mutex> lock(p->graph_mu);
+  const auto key = std::make_tuple(in, out, direction, s);
+  auto it = p->graphs.find(key);
+  if (it == p->graphs.end()) {
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const fyfft_status st = run();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    if (e != hipSuccess) return from_hip(e, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) return from_hip(e2, "hipGraphInstantiate");
+    it = p->graphs.emplace(key, exec).first;
+  }
+  HIP_TRY(hipGraphLaunch(it->second, s));
+  return FYFFT_SUCCESS;
+}
+
 extern "C" {
 
 const char* fyfft_status_string(fyfft_status s) {
@@ -443,63 +733,59 @@ fyfft_status fyfft_length_radices(size_t n, int* radices, int max, int* count) {
 }
 
 fyfft_status fyfft_plan_create(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision) {
+  return create_plan_impl(out, n, batch, precision, true);
+}
+
+fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, size_t count,
+                                       fyfft_precision precision, unsigned flags) {
   if (!out) return FYFFT_INVALID_ARGUMENT;
   *out = nullptr;
-  if (batch == 0) return FYFFT_INVALID_ARGUMENT;
-  if (precision != FYFFT_F32 && precision != FYFFT_F64) {
-    if (precision == FYFFT_BF16 || precision == FYFFT_F16) return FYFFT_UNSUPPORTED;
-    return FYFFT_INVALID_ARGUMENT;
-  }
-  if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+  if (!lengths || count == 0 || count > (size_t)INT32_MAX / 2) return FYFFT_INVALID_ARGUMENT;
+  if (flags & ~(unsigned)(FYFFT_GROUPED_GRAPH | FYFFT_GROUPED_PER_TRANSFORM)) return FYFFT_INVALID_ARGUMENT;
+  if (precision != FYFFT_F32 && precision != FYFFT_F64)
+    return (precision == FYFFT_BF16 || precision == FYFFT_F16) ? FYFFT_UNSUPPORTED : FYFFT_INVALID_ARGUMENT;
+  for (size_t i = 0; i < count; ++i)
+    if (lengths[i] < 1 || lengths[i] > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   fyfft_plan* p = new (std::nothrow) fyfft_plan();
   if (!p) return FYFFT_ALLOC_FAILED;
-  p->n = n;
-  p->batch = batch;
+  p->kind = kGrouped;
   p->prec = precision == FYFFT_F64 ? kF64 : kF32;
   p->device = dev;
-  p->strategy = strategy_for(n);
+  p->gflags = flags;
+  p->count = count;
+  p->batch = 1;
+  p->h_off.resize(count);
+  p->sub_of.resize(count);
+  std::map<size_t, std::vector<size_t>> by_len;
+  for (size_t i = 0; i < count; ++i) {
+    p->h_off[i] = p->total;
+    p->total += lengths[i];
+    by_len[lengths[i]].push_back(i);
+  }
+  p->n = p->total;
   fyfft_status st = FYFFT_SUCCESS;
-  if (n == 1) {
-    p->kind = kTrivial;
-    p->desc = "trivial N=1";
-  } else if (is_smooth7(n)) {
-    LdsGeom g;
-    if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
-      p->kind = kLds;
-      st = plan_lds(p, n, &p->lds, true);
-      if (!st) {
-        const LdsGeom& q = p->lds.geom;
-        p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") + " N=" +
-                  std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
-                  " T=" + std::to_string(q.threads) + " TPB=" + std::to_string(q.per_block) +
-                  " pad=" + std::to_string(q.pad_shift);
-      }
-    } else {
-      p->kind = kFourStep;
-      st = plan_fourstep(p, n);
-      if (!st) st = alloc_workspace(p, n, false);
-      if (!st)
-        p->desc = "fourstep N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
-                  radix_string(p->pdcol) + ") x " + std::to_string(p->n2) + "(" +
-                  radix_string(p->pdrow) + ") C=" + std::to_string(p->gcol.per_block) + "/" +
-                  std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk);
-    }
-  } else if (p->strategy == kRader) {
-    st = plan_rader(p);
-  } else {
-    st = plan_bluestein(p);
+  for (const auto& kv : by_len) {
+    fyfft_plan* sp = nullptr;
+    st = create_plan_impl(&sp, kv.first, 1, precision, false);
+    if (st) break;
+    for (size_t i : kv.second) p->sub_of[i] = (int)p->subs.size();
+    p->subs.push_back(sp);
   }
-  if (!st) {
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) st = from_hip(e, "plan table generation");
-  }
+  if (!st) st = p->prec == kF64 ? build_grouped<double2>(p, by_len) : build_grouped<float2>(p, by_len);
   if (st) {
     delete p;
     return st;
   }
   *out = p;
+  return FYFFT_SUCCESS;
+}
+
+fyfft_status fyfft_plan_elements(const fyfft_plan* p, size_t* elements) {
+  if (!valid(p)) return FYFFT_INVALID_PLAN;
+  if (!elements) return FYFFT_INVALID_ARGUMENT;
+  *elements = p->kind == kGrouped ? p->total : p->n * p->batch;
   return FYFFT_SUCCESS;
 }
 
@@ -540,6 +826,8 @@ fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int d
       if (p->prec == kF64)
         return exec_conv_lds<double2>(p, (const double2*)in, (double2*)out, inv_tables, scale, s);
       return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, s);
+    case kGrouped:
+      return exec_grouped(p, in, out, direction, s);
   }
   return FYFFT_INVALID_PLAN;
 }
@@ -558,7 +846,7 @@ fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, 
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   if (!in || !out) return FYFFT_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(p->host_mu);
-  const size_t bytes = p->n * p->batch * elem_bytes(p->prec);
+  const size_t bytes = (p->kind == kGrouped ? p->total : p->n * p->batch) * elem_bytes(p->prec);
   if (p->d_io_bytes < bytes) {
     if (p->d_io) (void)hipFree(p->d_io);
     p->d_io = nullptr;

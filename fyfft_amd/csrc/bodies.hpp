@@ -1,0 +1,184 @@
+// Workgroup-level bodies of every transform kernel, shared by the batched
+// kernels (one launch per plan) and the grouped ragged kernel (one task per
+// workgroup from a device task table).  Each body takes pointers already
+// positioned at its transform, so the callers only differ in how they find
+// the transform: batch index * distance, or a per-transform offset table.
+//
+// All bodies call __syncthreads(); every thread of the workgroup must call
+// them, with `active` false for threads that have no transform.
+#pragma once
+
+#include "internal.hpp"
+
+namespace fyfft {
+namespace dev {
+
+// ------------------------------------------------------------ smooth
+template <typename V, int PPT, int DIR>
+__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds,
+                                                int tid, bool active) {
+  const int N = a.m, T = a.threads, ps = a.ps;
+  if (active)
+    for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gload(x + i);
+  __syncthreads();
+  lds_fft<V, PPT, DIR>(lds, N, T, tid, a.pd, a.tw, active, ps);
+  if (active)
+    for (int i = tid; i < N; i += T) gstore(y + i, cscale(lds[i + (i >> ps)], a.scale));
+}
+
+// ------------------------------------------------------------ Rader
+// X[0] = x0 + sum_{n>=1} x[n];  X[g^i] = x0 + (a (*) b)[i],  a[j] = x[g^-j]
+template <typename V, int PPT>
+__device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y, V* lds, int tid,
+                                           bool active) {
+  const int T = a.threads, L = a.m, ps = a.ps;
+  V x0 = V{0, 0};
+  if (active) {
+    x0 = x[0];
+    for (int j = tid; j < L; j += T) lds[j + (j >> ps)] = x[a.perm_in[j]];
+  }
+  __syncthreads();
+  lds_fft<V, PPT, -1>(lds, L, T, tid, a.pd, a.tw, active, ps);
+  const V sum_rest = lds[0];
+  __syncthreads();
+  if (active)
+    for (int j = tid; j < L; j += T) {
+      const int li = j + (j >> ps);
+      lds[li] = cmul(lds[li], a.kern[j]);
+    }
+  __syncthreads();
+  lds_fft<V, PPT, +1>(lds, L, T, tid, a.pd, a.tw, active, ps);
+  if (active) {
+    for (int i = tid; i < L; i += T) y[a.perm_out[i]] = cscale(cadd(lds[i + (i >> ps)], x0), a.scale);
+    if (tid == 0) y[0] = cscale(cadd(x0, sum_rest), a.scale);
+  }
+}
+
+// ------------------------------------------------------------ Bluestein
+// X[k] = chirp[k] * IFFT(FFT(x * chirp, zero-padded to M) * kernel)[k]
+template <typename V, int PPT>
+__device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds, int tid,
+                                          bool active) {
+  const int T = a.threads, M = a.m, N = a.n, ps = a.ps;
+  if (active)
+    for (int n = tid; n < M; n += T) lds[n + (n >> ps)] = n < N ? cmul(x[n], a.chirp[n]) : V{0, 0};
+  __syncthreads();
+  lds_fft<V, PPT, -1>(lds, M, T, tid, a.pd, a.tw, active, ps);
+  if (active)
+    for (int j = tid; j < M; j += T) {
+      const int li = j + (j >> ps);
+      lds[li] = cmul(lds[li], a.kern[j]);
+    }
+  __syncthreads();
+  lds_fft<V, PPT, +1>(lds, M, T, tid, a.pd, a.tw, active, ps);
+  if (active)
+    for (int k = tid; k < N; k += T) y[k] = cscale(cmul(lds[k + (k >> ps)], a.chirp[k]), a.scale);
+}
+
+// ------------------------------------------------------------ four-step
+template <int DIR, typename V>
+__device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
+  const V w = cmul(a.tw_hi[m >> a.lo_bits], a.tw_lo[m & ((1LL << a.lo_bits) - 1)]);
+  return DIR < 0 ? w : V{w.x, -w.y};
+}
+
+// Column pass over columns [c0, c0 + C) of one transform (C = 1 << log2c).
+// x: the transform's input, tmp: its workspace, out: its output, x0: its
+// Rader x[0] slot.
+template <typename V, int PPT, int DIR, int MODE>
+__device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* tmp, V* out, V* x0,
+                                            int c0, V* lds_all) {
+  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
+  const int slot = threadIdx.x / a.threads;
+  const int tid = threadIdx.x - slot * a.threads;
+  const int ps = a.ps, stride = a.stride;
+  const int total = N1 << a.log2c;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int n1 = i >> a.log2c, c = i & (C - 1), n2 = c0 + c;
+    if (n2 >= N2) continue;
+    const int n = n1 * N2 + n2;
+    V v;
+    if constexpr (MODE == kFsPlainA) v = gload(x + n);
+    else if constexpr (MODE == kFsRaderA) v = x[a.perm[n]];
+    else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
+    else v = tmp[n];
+    lds_all[c * stride + n1 + (n1 >> ps)] = v;
+  }
+  if constexpr (MODE == kFsRaderA) {
+    if (c0 == 0 && threadIdx.x == 0) *x0 = x[0];
+  }
+  __syncthreads();
+  const bool active = slot < C && c0 + slot < N2;
+  lds_fft<V, PPT, DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, a.tw, active, ps);
+  V xz = V{0, 0};
+  if constexpr (MODE == kFsRaderC) xz = *x0;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int k1 = i >> a.log2c, c = i & (C - 1), n2 = c0 + c;
+    if (n2 >= N2) continue;
+    const V v = lds_all[c * stride + k1 + (k1 >> ps)];
+    if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
+      tmp[(long long)k1 * N2 + n2] = cmul(v, big_twiddle<DIR>(a, (long long)n2 * k1));
+    } else if constexpr (MODE == kFsRaderC) {
+      const int n = k1 * N2 + n2;  // k1 runs over n1 of the convolution output here
+      out[a.perm[n]] = cscale(cadd(v, xz), a.scale);
+    } else {  // kFsBlueC
+      const int n = k1 * N2 + n2;
+      if (n < a.n_orig) out[n] = cscale(cmul(v, a.chirp[n]), a.scale);
+    }
+  }
+}
+
+// Row pass over rows [r0, r0 + C) of one transform's workspace.
+template <typename V, int PPT, int DIR, int MODE>
+__device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, const V* x0,
+                                            int r0, V* lds_all) {
+  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
+  const int slot = threadIdx.x / a.threads;
+  const int tid = threadIdx.x - slot * a.threads;
+  const int ps = a.ps, stride = a.stride;
+  const int rows = min(C, N1 - r0);
+  const int total = rows * N2;
+  const V* src = tmp + (long long)r0 * N2;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int c = i / N2, n2 = i - c * N2;
+    lds_all[c * stride + n2 + (n2 >> ps)] = src[i];
+  }
+  __syncthreads();
+  const bool active = slot < rows;
+  V* lds = lds_all + (slot < C ? slot : 0) * stride;
+  const int k1 = r0 + slot;
+  if constexpr (MODE == kFsPlainB) {
+    lds_fft<V, PPT, DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
+      const int k2 = i >> a.log2c, c = i & (C - 1);
+      if (c >= rows) continue;
+      gstore(out + (long long)(r0 + c) + (long long)N1 * k2,
+             cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale));
+    }
+  } else {  // kFsConvB
+    lds_fft<V, PPT, -1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    if (x0 != nullptr && r0 == 0 && threadIdx.x == 0) {
+      // Rader: X[0] = x0 + sum_{n>=1} x[n] = x0 + A[0] (row k1 = 0, k2 = 0)
+      out[0] = cscale(cadd(*x0, lds_all[0]), a.scale);
+    }
+    __syncthreads();
+    if (active) {
+      const V* kr = a.kern + (long long)k1 * N2;
+      for (int k2 = tid; k2 < N2; k2 += a.threads) {
+        const int li = k2 + (k2 >> ps);
+        lds[li] = cmul(lds[li], kr[k2]);
+      }
+    }
+    __syncthreads();
+    lds_fft<V, PPT, +1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    V* dst = tmp + (long long)r0 * N2;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+      const int c = i / N2, n2 = i - c * N2;
+      const V v = lds_all[c * stride + n2 + (n2 >> ps)];
+      dst[i] = cmul(v, big_twiddle<+1>(a, (long long)n2 * (r0 + c)));
+    }
+  }
+}
+
+}  // namespace dev
+}  // namespace fyfft

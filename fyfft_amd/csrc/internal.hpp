@@ -8,6 +8,24 @@
 #include "planner.hpp"
 
 namespace fyfft {
+namespace dev {
+// Parameters of an LDS-resident single-kernel transform (smooth, Rader or
+// Bluestein) of one length.
+template <typename V> struct LdsBody {
+  int n = 0;          // user length
+  int m = 0;          // LDS FFT length: n (smooth), n-1 (Rader), M (Bluestein)
+  int threads = 0;    // threads per transform
+  int ps = 30;        // LDS pad shift
+  int stride = 0;     // LDS elements per transform slot
+  PassDesc pd{};
+  const V* tw = nullptr;
+  const int* perm_in = nullptr;
+  const int* perm_out = nullptr;
+  const V* chirp = nullptr;
+  const V* kern = nullptr;
+  S<V> scale = 1;
+};
+}  // namespace dev
 
 enum Prec { kF32 = 0, kF64 = 1 };
 enum Dir { kForward = 0, kInverse = 1, kAdjoint = 2 };
@@ -120,6 +138,49 @@ template <typename V> struct SpecArgs {
   dev::S<V> a0 = 1, a = 1;             // scale of mode 0 and of modes 1..k-1
 };
 hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGeom& g, hipStream_t s);
+
+// --- kernels_grouped.hip
+enum GroupedKind { kGkLds = 0, kGkRader = 1, kGkBlue = 2, kGkFsPlain = 3, kGkFsRader = 4, kGkFsBlue = 5 };
+
+// One workgroup's work: phase 0 LDS tasks run `count` transforms of one
+// length starting at list position `first`; four-step tasks run the column
+// (or row) group starting at `count` of the transform at list position
+// `first`.
+struct GTask {
+  int len;    // index into the GLen table
+  int phase;  // 0, 1, 2 (see kernels_grouped.hip)
+  int first;
+  int count;
+};
+
+// Device descriptor of one distinct length of a grouped plan; forward tables
+// are stored in lds / col / row, the inverse chirp and kernel spectrum apart.
+template <typename V> struct GLen {
+  int kind = kGkLds;
+  int n = 0;
+  int lds_ppt = 16, col_ppt = 16, row_ppt = 16;
+  dev::LdsBody<V> lds;
+  FsArgs<V> col, row;
+  const V* chirp_inv = nullptr;
+  const V* kern_inv = nullptr;
+};
+
+template <typename V> struct GroupedArgs {
+  const GTask* tasks = nullptr;
+  int ntasks = 0;
+  const GLen<V>* lens = nullptr;
+  const long long* off = nullptr;   // element offset of each transform (list order)
+  const long long* woff = nullptr;  // workspace offset of each transform (list order)
+  const V* in = nullptr;
+  V* out = nullptr;
+  V* ws = nullptr;
+  V* x0buf = nullptr;
+  int tables = 0;     // 0 forward, 1 inverse/adjoint tables
+  int normalize = 0;  // scale by 1/n
+};
+hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, size_t lds, hipStream_t s);
+// Sets the dynamic-LDS limit of the grouped kernels (call at plan time).
+hipError_t prepare_grouped(int prec, size_t lds);
 
 // --- kernels_tables.hip
 // Stockham twiddles for pass list pd (N-1 entries) in FP64, stored as prec.

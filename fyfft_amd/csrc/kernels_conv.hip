@@ -14,68 +14,48 @@
 // One kernel: pre-chirp and zero-pad into LDS, FFT_M, multiply, inverse
 // FFT_M, post-chirp the first N outputs (bluestein.hpp:235-354 does the same
 // steps as separate passes over host memory).
-#include "internal.hpp"
+#include "bodies.hpp"
 
 namespace fyfft {
 using namespace dev;
 
+template <typename V> __device__ __forceinline__ LdsBody<V> conv_body(const ConvArgs<V>& a) {
+  LdsBody<V> b;
+  b.n = a.n_orig;
+  b.m = a.m;
+  b.threads = a.threads;
+  b.ps = a.ps;
+  b.stride = a.stride;
+  b.pd = a.pd;
+  b.tw = a.tw;
+  b.perm_in = a.perm_in;
+  b.perm_out = a.perm_out;
+  b.chirp = a.chirp;
+  b.kern = a.kern;
+  b.scale = a.scale;
+  return b;
+}
+
 template <typename V, int PPT>
 __global__ __launch_bounds__(1024) void k_rader_lds(ConvArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds_all = reinterpret_cast<V*>(smem_raw);
-  const int T = a.threads, L = a.m, ps = a.ps;
-  const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
+  const int slot = threadIdx.x / a.threads, tid = threadIdx.x - slot * a.threads;
   const long long t = (long long)blockIdx.x * a.per_block + slot;
   const bool active = t < a.batch;
-  V* lds = lds_all + slot * a.stride;
-  const V* x = a.in + (active ? t : 0) * a.dist;
-  V* out = a.out + (active ? t : 0) * a.dist;
-  V x0 = V{0, 0};
-  if (active) {
-    x0 = x[0];
-    for (int j = tid; j < L; j += T) lds[j + (j >> ps)] = x[a.perm_in[j]];
-  }
-  __syncthreads();
-  lds_fft<V, PPT, -1>(lds, L, T, tid, a.pd, a.tw, active, ps);
-  const V sum_rest = lds[0];
-  __syncthreads();
-  if (active)
-    for (int j = tid; j < L; j += T) {
-      const int li = j + (j >> ps);
-      lds[li] = cmul(lds[li], a.kern[j]);
-    }
-  __syncthreads();
-  lds_fft<V, PPT, +1>(lds, L, T, tid, a.pd, a.tw, active, ps);
-  if (active) {
-    for (int i = tid; i < L; i += T) out[a.perm_out[i]] = cscale(cadd(lds[i + (i >> ps)], x0), a.scale);
-    if (tid == 0) out[0] = cscale(cadd(x0, sum_rest), a.scale);
-  }
+  const long long off = (active ? t : 0) * a.dist;
+  body_rader<V, PPT>(conv_body(a), a.in + off, a.out + off,
+                     reinterpret_cast<V*>(smem_raw) + slot * a.stride, tid, active);
 }
 
 template <typename V, int PPT>
 __global__ __launch_bounds__(1024) void k_blue_lds(ConvArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds_all = reinterpret_cast<V*>(smem_raw);
-  const int T = a.threads, M = a.m, N = a.n_orig, ps = a.ps;
-  const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
+  const int slot = threadIdx.x / a.threads, tid = threadIdx.x - slot * a.threads;
   const long long t = (long long)blockIdx.x * a.per_block + slot;
   const bool active = t < a.batch;
-  V* lds = lds_all + slot * a.stride;
-  const V* x = a.in + (active ? t : 0) * a.dist;
-  V* out = a.out + (active ? t : 0) * a.dist;
-  if (active)
-    for (int n = tid; n < M; n += T) lds[n + (n >> ps)] = n < N ? cmul(x[n], a.chirp[n]) : V{0, 0};
-  __syncthreads();
-  lds_fft<V, PPT, -1>(lds, M, T, tid, a.pd, a.tw, active, ps);
-  if (active)
-    for (int j = tid; j < M; j += T) {
-      const int li = j + (j >> ps);
-      lds[li] = cmul(lds[li], a.kern[j]);
-    }
-  __syncthreads();
-  lds_fft<V, PPT, +1>(lds, M, T, tid, a.pd, a.tw, active, ps);
-  if (active)
-    for (int k = tid; k < N; k += T) out[k] = cscale(cmul(lds[k + (k >> ps)], a.chirp[k]), a.scale);
+  const long long off = (active ? t : 0) * a.dist;
+  body_blue<V, PPT>(conv_body(a), a.in + off, a.out + off,
+                    reinterpret_cast<V*>(smem_raw) + slot * a.stride, tid, active);
 }
 
 template <typename V, int PPT>

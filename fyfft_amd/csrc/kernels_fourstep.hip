@@ -19,138 +19,31 @@
 // Twiddles w_N^m (m < N) come from two FP64-generated tables,
 // w_N^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)], each about sqrt(N)
 // entries, so they stay cache resident for any N.
-#include "internal.hpp"
+#include "bodies.hpp"
 
 namespace fyfft {
 using namespace dev;
 
-template <int DIR, typename V>
-__device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
-  const V w = cmul(a.tw_hi[m >> a.lo_bits], a.tw_lo[m & ((1LL << a.lo_bits) - 1)]);
-  return DIR < 0 ? w : V{w.x, -w.y};
-}
-
-// ------------------------------------------------------------ column pass
 template <typename V, int PPT, int DIR, int MODE>
 __global__ __launch_bounds__(1024) void k_fs_col(FsArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds_all = reinterpret_cast<V*>(smem_raw);
-  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
-  const int groups = (N2 + C - 1) >> a.log2c;
+  const int groups = (a.n2 + (1 << a.log2c) - 1) >> a.log2c;
   const long long t = blockIdx.x / groups;
   const int c0 = (int)(blockIdx.x - t * groups) << a.log2c;
-  const int slot = threadIdx.x / a.threads;
-  const int tid = threadIdx.x - slot * a.threads;
-  const int ps = a.ps, stride = a.stride;
-  const int total = N1 << a.log2c;
-
-  const V* x = a.in + t * a.in_dist;
-  V* tmp = a.tmp + t * a.tmp_dist;
-  V* out = a.out + t * a.out_dist;
-
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int n1 = i >> a.log2c, c = i & (C - 1), n2 = c0 + c;
-    if (n2 >= N2) continue;
-    const int n = n1 * N2 + n2;
-    V v;
-    if constexpr (MODE == kFsPlainA) {
-      v = x[n];
-    } else if constexpr (MODE == kFsRaderA) {
-      v = x[a.perm[n]];
-    } else if constexpr (MODE == kFsBlueA) {
-      v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
-    } else {  // inverse column pass of a convolution: input is tmp
-      v = tmp[n];
-    }
-    lds_all[c * stride + n1 + (n1 >> ps)] = v;
-  }
-  if constexpr (MODE == kFsRaderA) {
-    if (c0 == 0 && threadIdx.x == 0) a.x0buf[t] = x[0];
-  }
-  __syncthreads();
-  lds_fft<V, PPT, DIR>(lds_all + slot * stride, N1, a.threads, tid, a.pd, a.tw,
-                       c0 + slot < N2, ps);
-
-  V x0 = V{0, 0};
-  if constexpr (MODE == kFsRaderC) x0 = a.x0buf[t];
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int k1 = i >> a.log2c, c = i & (C - 1), n2 = c0 + c;
-    if (n2 >= N2) continue;
-    V v = lds_all[c * stride + k1 + (k1 >> ps)];
-    if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
-      tmp[(long long)k1 * N2 + n2] = cmul(v, big_twiddle<DIR>(a, (long long)n2 * k1));
-    } else if constexpr (MODE == kFsRaderC) {
-      // here k1 plays the role of n1 of the convolution output
-      const int n = k1 * N2 + n2;
-      out[a.perm[n]] = cscale(cadd(v, x0), a.scale);
-    } else {  // kFsBlueC
-      const int n = k1 * N2 + n2;
-      if (n < a.n_orig) out[n] = cscale(cmul(v, a.chirp[n]), a.scale);
-    }
-  }
+  body_fs_col<V, PPT, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
+                                 a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
+                                 reinterpret_cast<V*>(smem_raw));
 }
 
-// ------------------------------------------------------------ row pass
 template <typename V, int PPT, int DIR, int MODE>
 __global__ __launch_bounds__(1024) void k_fs_row(FsArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds_all = reinterpret_cast<V*>(smem_raw);
-  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
-  const int groups = (N1 + C - 1) >> a.log2c;
+  const int groups = (a.n1 + (1 << a.log2c) - 1) >> a.log2c;
   const long long t = blockIdx.x / groups;
   const int r0 = (int)(blockIdx.x - t * groups) << a.log2c;
-  const int slot = threadIdx.x / a.threads;
-  const int tid = threadIdx.x - slot * a.threads;
-  const int ps = a.ps, stride = a.stride;
-  const int rows = min(C, N1 - r0);
-  const int total = rows * N2;
-
-  V* tmp = a.tmp + t * a.tmp_dist;
-  V* out = a.out + t * a.out_dist;
-  const V* src = tmp + (long long)r0 * N2;
-
-  // rows r0 .. r0+rows-1 are one contiguous span of tmp
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int c = i / N2, n2 = i - c * N2;
-    lds_all[c * stride + n2 + (n2 >> ps)] = src[i];
-  }
-  __syncthreads();
-  const bool active = slot < rows;
-  V* lds = lds_all + slot * stride;
-  const int k1 = r0 + slot;
-
-  if constexpr (MODE == kFsPlainB) {
-    lds_fft<V, PPT, DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
-    // transposed store X[k1 + N1*k2]: C consecutive k1 per k2
-    for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
-      const int k2 = i >> a.log2c, c = i & (C - 1);
-      if (c >= rows) continue;
-      out[(long long)(r0 + c) + (long long)N1 * k2] = cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale);
-    }
-  } else {  // kFsConvB: forward, multiply by kernel spectrum, inverse, conj twiddle
-    lds_fft<V, PPT, -1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
-    if (a.x0buf != nullptr && r0 == 0 && threadIdx.x == 0) {
-      // Rader: X[0] = x0 + sum_{n>=1} x[n] = x0 + A[0] (row k1 = 0, k2 = 0)
-      const V x0 = a.x0buf[t];
-      a.out[t * a.out_dist] = cscale(cadd(x0, lds_all[0]), a.scale);
-    }
-    __syncthreads();
-    if (active) {
-      const V* kr = a.kern + (long long)k1 * N2;
-      for (int k2 = tid; k2 < N2; k2 += a.threads) {
-        const int li = k2 + (k2 >> ps);
-        lds[li] = cmul(lds[li], kr[k2]);
-      }
-    }
-    __syncthreads();
-    lds_fft<V, PPT, +1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
-    V* dst = tmp + (long long)r0 * N2;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-      const int c = i / N2, n2 = i - c * N2;
-      const V v = lds_all[c * stride + n2 + (n2 >> ps)];
-      dst[i] = cmul(v, big_twiddle<+1>(a, (long long)n2 * (r0 + c)));
-    }
-  }
+  body_fs_row<V, PPT, DIR, MODE>(a, a.tmp + t * a.tmp_dist, a.out + t * a.out_dist,
+                                 a.x0buf ? a.x0buf + t : nullptr, r0,
+                                 reinterpret_cast<V*>(smem_raw));
 }
 
 // ------------------------------------------------------------ launch

@@ -10,7 +10,7 @@
 //   registers, so a P-pass transform costs P-1 LDS round trips.
 #include <cstdio>
 
-#include "internal.hpp"
+#include "bodies.hpp"
 
 namespace fyfft {
 using namespace dev;
@@ -21,20 +21,22 @@ __global__ __launch_bounds__(1024) void k_lds_generic(const V* in, V* out, long 
                                                       int ps, int lds_stride, PassDesc pd,
                                                       const V* __restrict__ tw, S<V> scale) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  V* lds_all = reinterpret_cast<V*>(smem_raw);
   const int slot = threadIdx.x / T;
   const int tid = threadIdx.x - slot * T;
   const long long t = (long long)blockIdx.x * TPB + slot;
   const bool active = t < batch;
-  V* lds = lds_all + slot * lds_stride;
-  const V* gin = in + (active ? t : 0) * dist;
-  V* gout = out + (active ? t : 0) * dist;
-  if (active)
-    for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gload(gin + i);
-  __syncthreads();
-  lds_fft<V, PPT, DIR>(lds, N, T, tid, pd, tw, active, ps);
-  if (active)
-    for (int i = tid; i < N; i += T) gstore(gout + i, cscale(lds[i + (i >> ps)], scale));
+  LdsBody<V> b;
+  b.n = N;
+  b.m = N;
+  b.threads = T;
+  b.ps = ps;
+  b.stride = lds_stride;
+  b.pd = pd;
+  b.tw = tw;
+  b.scale = scale;
+  const long long off = (active ? t : 0) * dist;
+  body_lds_smooth<V, PPT, DIR>(b, in + off, out + off,
+                               reinterpret_cast<V*>(smem_raw) + slot * lds_stride, tid, active);
 }
 
 // ------------------------------------------------------------ fixed kernels

@@ -91,6 +91,25 @@ typedef struct fyfft_plan fyfft_plan;
 fyfft_status fyfft_plan_create(fyfft_plan** plan, size_t n, size_t batch,
                                fyfft_precision precision);
 
+/* Grouped ragged plans (no counterpart in the reference, which plans one
+ * length at a time): `count` transforms of lengths lengths[0..count-1],
+ * packed back to back (transform i starts at element sum_{j<i} lengths[j]),
+ * executed by fyfft_execute as ONE persistent grouped launch per phase
+ * (up to three phases) that reads a device-side task table.
+ *   FYFFT_GROUPED_GRAPH: capture the phases into a hipGraph the first time
+ *     a (in, out, direction, stream) combination runs on a non-null stream,
+ *     and replay the graph afterwards;
+ *   FYFFT_GROUPED_PER_TRANSFORM: the baseline form -- one launch of the
+ *     plain (non-specialised) kernels per transform. */
+#define FYFFT_GROUPED_GRAPH 1u
+#define FYFFT_GROUPED_PER_TRANSFORM 2u
+fyfft_status fyfft_plan_create_grouped(fyfft_plan** plan, const size_t* lengths, size_t count,
+                                       fyfft_precision precision, unsigned flags);
+
+/* Elements (complex values) one execution reads and writes: N * batch, or
+ * the sum of the lengths of a grouped plan. */
+fyfft_status fyfft_plan_elements(const fyfft_plan* plan, size_t* elements);
+
 /* Releases every device allocation of the plan; NULL is accepted. */
 fyfft_status fyfft_plan_destroy(fyfft_plan* plan);
 
