@@ -1,11 +1,17 @@
 """Batched FP32 C2C FFT throughput on one MI355X.
 
-Workload: forward complex<float> FFT, N=4096, 65536 contiguous transforms
-(2 GiB in, 2 GiB out), re/im i.i.d. N(0,1) from seed 0, resident in HBM before
-the timed region.  A step is one batched launch through the C ABI
-(fyfft_execute).  Prints ONE JSON line.
+Default workload (the headline): forward complex<float> FFT, N=4096, 65536
+contiguous transforms (2 GiB in, 2 GiB out), re/im i.i.d. N(0,1) from seed 0,
+resident in HBM before the timed region.  A step is one batched launch
+through the C ABI (fyfft_execute).  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--batch 65536]
+
+Other configurations (one JSON line each, same fields where they apply):
+  --workload mixed     N=6720 complex<float> fwd+inv, batch 16384 (--precision f64: batch 4096)
+  --workload prime     Rader N=65537 x 2048, Bluestein N=10007 x 1024 and N=1000003 x 16
+  --workload ragged    20000 ragged transforms in one grouped launch vs one launch per transform
+  --workload spectral  spectral convolution [256, 64, 4096], 512 modes, forward + backward
 """
 from __future__ import annotations
 
@@ -33,6 +39,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="transforms in the CPU baseline sample")
+    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral"],
+                    default="c2c")
     return ap.parse_args()
 
 
@@ -61,12 +69,181 @@ def cpu_baseline(n, sample, prec):
                       f"reference ladder (oracle/), {threads} threads, {sec:.2f} s"}
 
 
+def time_steps(fn, steps, warmup, stream):
+    """ms per step of fn() on `stream` (HIP events on that stream)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def line(metric_extra, **kw):
+    base = {"metric": "batched C2C GFLOP/s (5NlogN) + % HBM roofline vs plain GPU form, rel-L2<=1e-6",
+            "unit": "GFLOP/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "data": "synthetic, seeded, resident in HBM"}
+    base.update(metric_extra)
+    base.update(kw)
+    return base
+
+
+def flops_of(n):
+    return 5.0 * n * math.log2(n) if n > 1 else 0.0
+
+
+def bench_transforms(a, cases, label):
+    """cases: list of (n, batch, prec, directions).  One JSON line with a
+    per-case table."""
+    import numpy as np
+    import torch
+    import fyfft_amd as fy
+    import oracle
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    rows, tot_flops, tot_ms, tot_bytes = [], 0.0, 0.0, 0.0
+    for n, batch, prec, dirs in cases:
+        rdt = torch.float32 if prec == fy.F32 else torch.float64
+        g = torch.Generator(device=dev).manual_seed(0)
+        if label == "prime":
+            ph = torch.rand(batch, n, dtype=rdt, device=dev, generator=g) * (2 * math.pi)
+            x = torch.polar(torch.ones_like(ph), ph)
+        else:
+            x = torch.view_as_complex(torch.randn(batch, n, 2, dtype=rdt, device=dev, generator=g))
+        y = torch.empty_like(x)
+        plan = fy.Plan(n, batch, prec)
+        h = stream.cuda_stream
+        for d in dirs:
+            ms = time_steps(lambda: plan.execute_ptr(x.data_ptr(), y.data_ptr(), d, h), a.steps, a.warmup, stream)
+            r = [0, batch - 1]
+            xs = x[r].cpu().numpy().astype(np.complex128)
+            err = oracle.rel_l2(y[r].cpu().numpy(), oracle.fft_rows(xs, inverse=d != fy.FORWARD))
+            elem = 8 if prec == fy.F32 else 16
+            byt = 2.0 * elem * n * batch
+            rows.append({"n": n, "batch": batch, "dtype": "f32" if prec == fy.F32 else "f64",
+                         "direction": ["forward", "inverse"][d], "plan": plan.describe(),
+                         "ms": round(ms, 4), "GFLOP/s": round(flops_of(n) * batch / ms / 1e6, 1),
+                         "hbm_frac": round(byt / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                         "rel_l2_vs_fp64": err})
+            tot_flops += flops_of(n) * batch
+            tot_ms += ms
+            tot_bytes += byt
+        plan.close()
+        del x, y
+        torch.cuda.empty_cache()
+    return line({"config": {"workload": label, "cases": rows}}, value=round(tot_flops / tot_ms / 1e6, 2),
+                steps=a.steps, warmup=a.warmup, ms_per_step=round(tot_ms, 4),
+                dtype="/".join(sorted({r["dtype"] for r in rows})),
+                roofline={"bound": "hbm", "achieved": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                          "traffic": None})
+
+
+def bench_ragged(a):
+    import numpy as np
+    import torch
+    import fyfft_amd as fy
+    import oracle
+    from fyfft_amd.workloads import ragged_lengths, l_max_for
+    dev = torch.device("cuda:0")
+    lens = ragged_lengths(count=20000, seed=0)
+    tot = int(lens.sum())
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.view_as_complex(torch.randn(tot, 2, device=dev, generator=g))
+    y = torch.empty_like(x)
+    s = torch.cuda.Stream(dev)
+    t0 = time.perf_counter()
+    gp = fy.GroupedPlan(lens, fy.F32, fy.GROUPED_GRAPH)
+    t_plan = time.perf_counter() - t0
+    with torch.cuda.stream(s):
+        ms = time_steps(lambda: gp.execute(x, y, fy.FORWARD, stream=s), a.steps, a.warmup, s)
+    # accuracy on a sample of transforms of every class
+    offs = gp.offsets
+    pick = sorted(set([0, len(lens) - 1] + list(np.argsort(lens)[-3:]) + list(range(0, len(lens), 997))))
+    errs = []
+    yh = y.cpu().numpy()
+    xh = x.cpu().numpy()
+    for i in pick:
+        n, o = int(lens[i]), int(offs[i])
+        errs.append(oracle.rel_l2(yh[o:o + n], oracle.fft_ext(xh[o:o + n].astype(np.complex128))))
+    bp = fy.GroupedPlan(lens, fy.F32, fy.GROUPED_PER_TRANSFORM)
+    ms_base = time_steps(lambda: bp.execute(x, y, fy.FORWARD), 2, 1, torch.cuda.current_stream(dev))
+    flops = sum(flops_of(int(n)) for n in lens)
+    byt = 16.0 * tot
+    out = line({"config": {"workload": "ragged grouped C2C, 20000 complex<float> transforms, lengths "
+                                       "log-uniform in [16, %d] snapped to 60%% smooth / 25%% Rader / 15%% "
+                                       "Bluestein (seed 0)" % round(l_max_for(20000, 2.0 ** 28)),
+                           "total_elements": tot, "distinct_lengths": len(set(lens.tolist())),
+                           "plan": gp.describe(), "plan_seconds": round(t_plan, 2)}},
+               value=round(flops / ms / 1e6, 2), steps=a.steps, warmup=a.warmup,
+               ms_per_step=round(ms, 4), dtype="f32",
+               rel_l2_vs_fp64_max=max(errs),
+               roofline={"bound": "hbm", "achieved": round(byt / (ms * 1e-3) / 1e9, 1),
+                         "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(byt / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": None},
+               per_transform_baseline={"ms_per_step": round(ms_base, 3),
+                                       "speedup_grouped": round(ms_base / ms, 2)})
+    gp.close()
+    bp.close()
+    return out
+
+
+def bench_spectral(a):
+    import torch
+    from fyfft_amd.autograd import SpectralConv1d
+    dev = torch.device("cuda:0")
+    B, C, L, K = 256, 64, 4096, 512
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.randn(B, C, L, device=dev, generator=g, requires_grad=True)
+    layer = SpectralConv1d(C, C, K, device=dev, generator=g)
+    gy = torch.randn(B, C, L, device=dev, generator=g)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        x.grad = None
+        layer.weight.grad = None
+        y = layer(x)
+        y.backward(gy)
+    ms = time_steps(step, a.steps, a.warmup, stream)
+    fft_flops = 2 * 2 * B * C * flops_of(L)          # fwd: 2 transforms, bwd: 2 transforms
+    mix_flops = 3 * 8.0 * B * C * C * K              # one forward and two backward GEMMs
+    return line({"config": {"workload": f"spectral convolution forward+backward, x float32 "
+                                        f"[{B}, {C}, {L}], W complex64 [{C}, {C}, {K}]"}},
+                value=round((fft_flops + mix_flops) / ms / 1e6, 2), steps=a.steps,
+                warmup=a.warmup, ms_per_step=round(ms, 4), dtype="f32",
+                unit="GFLOP/s")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload != "c2c":
+        import torch
+        import fyfft_amd as fy
+        torch.cuda.set_device(0)
+        if a.workload == "mixed":
+            cases = [(6720, 16384, fy.F32, (fy.FORWARD, fy.INVERSE)),
+                     (6720, 4096, fy.F64, (fy.FORWARD, fy.INVERSE))]
+            out = bench_transforms(a, cases, "mixed")
+        elif a.workload == "prime":
+            cases = [(65537, 2048, fy.F32, (fy.FORWARD,)), (10007, 1024, fy.F32, (fy.FORWARD,)),
+                     (1000003, 16, fy.F32, (fy.FORWARD,))]
+            out = bench_transforms(a, cases, "prime")
+        elif a.workload == "ragged":
+            out = bench_ragged(a)
+        else:
+            out = bench_spectral(a)
+        print(json.dumps(out), flush=True)
+        return
     import numpy as np
     import torch
     import fyfft_amd as fy

@@ -76,6 +76,7 @@ struct fyfft_plan {
   void *d_tasks = nullptr, *d_lens = nullptr, *d_off = nullptr, *d_woff = nullptr;
   void *gws = nullptr, *gx0 = nullptr;
   int task_base[3] = {0, 0, 0}, task_count[3] = {0, 0, 0};
+  int task_xbase[3][dev::kXcds + 1] = {};
   size_t phase_lds[3] = {0, 0, 0};
   unsigned phase_grid[3] = {0, 0, 0};
   mutable std::mutex graph_mu;
@@ -606,15 +607,45 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
     }
     ++li;
   }
+  // Bin-pack each phase onto the XCDs: all tasks of one transform (four-step
+  // groups) stay together, greedy by decreasing group cost onto the least
+  // loaded XCD; within an XCD, tasks by decreasing cost.
   std::vector<GTask> tasks;
   for (int ph = 0; ph < 3; ++ph) {
-    std::stable_sort(phase[ph].begin(), phase[ph].end(), [](const T& a, const T& b) { return a.cost > b.cost; });
+    std::map<std::pair<int, int>, std::vector<T>> groups;  // (len, transform or task) -> tasks
+    for (size_t i = 0; i < phase[ph].size(); ++i) {
+      const T& t = phase[ph][i];
+      const bool fs = lens[t.t.len].kind >= kGkFsPlain;
+      groups[{t.t.len, fs ? t.t.first : -1 - (int)i}].push_back(t);
+    }
+    std::vector<std::pair<double, std::vector<T>*>> order;
+    for (auto& kv : groups) {
+      double c = 0;
+      for (const T& t : kv.second) c += t.cost;
+      order.push_back({c, &kv.second});
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<T> per_x[dev::kXcds];
+    double load[dev::kXcds] = {0};
+    for (auto& g : order) {
+      const int x = (int)(std::min_element(load, load + dev::kXcds) - load);
+      load[x] += g.first;
+      per_x[x].insert(per_x[x].end(), g.second->begin(), g.second->end());
+    }
     p->task_base[ph] = (int)tasks.size();
     p->task_count[ph] = (int)phase[ph].size();
-    for (const T& t : phase[ph]) tasks.push_back(t.t);
+    for (int x = 0; x < dev::kXcds; ++x) {
+      std::stable_sort(per_x[x].begin(), per_x[x].end(), [](const T& a, const T& b) { return a.cost > b.cost; });
+      p->task_xbase[ph][x] = (int)tasks.size() - p->task_base[ph];
+      for (const T& t : per_x[x]) tasks.push_back(t.t);
+    }
+    p->task_xbase[ph][dev::kXcds] = (int)tasks.size() - p->task_base[ph];
     const size_t lds = std::max<size_t>(p->phase_lds[ph], 16);
     const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(2, cap / lds));
-    p->phase_grid[ph] = (unsigned)std::min<size_t>(phase[ph].size(), (size_t)ncu * per_cu);
+    // a multiple of the XCD count, at most the resident capacity
+    size_t grid = std::min<size_t>(phase[ph].size(), (size_t)ncu * per_cu);
+    grid = std::max<size_t>(dev::kXcds, (grid + dev::kXcds - 1) / dev::kXcds * dev::kXcds);
+    p->phase_grid[ph] = (unsigned)grid;
   }
   fyfft_status st;
   if ((st = p->alloc(std::max<size_t>(1, tasks.size()) * sizeof(GTask), &p->d_tasks)) ||
@@ -653,7 +684,7 @@ static fyfft_status launch_grouped_phases(const fyfft_plan* p, const void* in, v
   for (int ph = 0; ph < 3; ++ph) {
     if (!p->task_count[ph]) continue;
     a.tasks = (const GTask*)p->d_tasks + p->task_base[ph];
-    a.ntasks = p->task_count[ph];
+    for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = p->task_xbase[ph][x];
     HIP_TRY(launch_grouped(p->prec, dir, &a, p->phase_grid[ph], std::max<size_t>(p->phase_lds[ph], 16), s));
   }
   return FYFFT_SUCCESS;

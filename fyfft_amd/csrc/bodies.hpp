@@ -15,8 +15,9 @@ namespace dev {
 
 // ------------------------------------------------------------ smooth
 template <typename V, int PPT, int DIR>
-__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds,
+__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds_in,
                                                 int tid, bool active) {
+  V* lds = as_lds(lds_in);
   const int N = a.m, T = a.threads, ps = a.ps;
   if (active)
     for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gload(x + i);
@@ -29,8 +30,9 @@ __device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x,
 // ------------------------------------------------------------ Rader
 // X[0] = x0 + sum_{n>=1} x[n];  X[g^i] = x0 + (a (*) b)[i],  a[j] = x[g^-j]
 template <typename V, int PPT>
-__device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y, V* lds, int tid,
+__device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                            bool active) {
+  V* lds = as_lds(lds_in);
   const int T = a.threads, L = a.m, ps = a.ps;
   V x0 = V{0, 0};
   if (active) {
@@ -57,8 +59,9 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
 // ------------------------------------------------------------ Bluestein
 // X[k] = chirp[k] * IFFT(FFT(x * chirp, zero-padded to M) * kernel)[k]
 template <typename V, int PPT>
-__device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds, int tid,
+__device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                           bool active) {
+  V* lds = as_lds(lds_in);
   const int T = a.threads, M = a.m, N = a.n, ps = a.ps;
   if (active)
     for (int n = tid; n < M; n += T) lds[n + (n >> ps)] = n < N ? cmul(x[n], a.chirp[n]) : V{0, 0};
@@ -87,7 +90,8 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
 // Rader x[0] slot.
 template <typename V, int PPT, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* tmp, V* out, V* x0,
-                                            int c0, V* lds_all) {
+                                            int c0, V* lds_in) {
+  V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
@@ -131,7 +135,8 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
 // Row pass over rows [r0, r0 + C) of one transform's workspace.
 template <typename V, int PPT, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, const V* x0,
-                                            int r0, V* lds_all) {
+                                            int r0, V* lds_in) {
+  V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;

@@ -259,6 +259,28 @@ __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
   }
 }
 
+// ---------------------------------------------------------------- XCD mapping
+// The dispatcher hands workgroup b of a launch to XCD (b mod 8).  For passes
+// where several consecutive logical workgroups work on one transform (the
+// four-step column/row groups, the Rader gather and scatter), remapping gives
+// each XCD one contiguous range of logical workgroups, so a transform's
+// cache lines are fetched into one L2 instead of eight.  It is a bijection
+// on [0, total) for any total.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_remap(int b, int total) {
+  const int per = total / kXcds, rem = total % kXcds;
+  const int x = b % kXcds, i = b / kXcds;
+  return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+// ---------------------------------------------------------------- address spaces
+// Pointers that cross a call boundary lose their address space; passing LDS
+// as address_space(3) and tables as address_space(1) keeps ds_* / global_*
+// instructions instead of flat ones in out-of-line code.
+template <typename V> using lds_ptr = __attribute__((address_space(3))) V*;
+template <typename V> using gbl_cptr = const __attribute__((address_space(1))) V*;
+template <typename V> __device__ __forceinline__ V* as_lds(V* p) { return (V*)(lds_ptr<V>)p; }
+
 // ---------------------------------------------------------------- LDS layout
 // One pad element every 16 complex values breaks the power-of-two strides
 // of the pass read/write patterns across LDS banks.
@@ -287,32 +309,54 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
                                               int ps) {
   const int nb = N / R;
   V v[MAXB][R];
+  // k = j mod Ls and q = j / Ls for j = tid + b*T, stepped without a
+  // division per butterfly: two divisions per pass.
+  int q0 = 0, k0 = 0, tq = 0, tk = 0;
+  if (Ls > 1) {
+    q0 = tid / Ls;
+    k0 = tid - q0 * Ls;
+    tq = T / Ls;
+    tk = T - tq * Ls;
+  }
+  {
+    int k = k0;
 #pragma unroll
-  for (int b = 0; b < MAXB; ++b) {
-    const int j = tid + b * T;
-    if (active && j < nb) {
-      const int k = (Ls == 1) ? 0 : j % Ls;
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = tid + b * T;
+      if (active && j < nb) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int idx = j + r * nb;
-        v[b][r] = SRC_G ? gload(src + idx) : src[idx + (idx >> ps)];
+        for (int r = 0; r < R; ++r) {
+          const int idx = j + r * nb;
+          v[b][r] = SRC_G ? gload(src + idx) : src[idx + (idx >> ps)];
+        }
+        if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
+        Dft<R, DIR, V>::run(v[b]);
       }
-      if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
-      Dft<R, DIR, V>::run(v[b]);
+      if (Ls > 1) {
+        k += tk;
+        if (k >= Ls) k -= Ls;
+      }
     }
   }
   if (!SRC_G && !DST_G) __syncthreads();
+  {
+    int q = q0, k = k0;
 #pragma unroll
-  for (int b = 0; b < MAXB; ++b) {
-    const int j = tid + b * T;
-    if (active && j < nb) {
-      const int k = (Ls == 1) ? 0 : j % Ls;
-      const int base = (j - k) * R + k;
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = tid + b * T;
+      if (active && j < nb) {
+        const int base = (Ls > 1) ? q * Ls * R + k : j * R;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int idx = base + r * Ls;
-        if (DST_G) gstore(dst + idx, cscale(v[b][r], scale));
-        else dst[idx + (idx >> ps)] = v[b][r];
+        for (int r = 0; r < R; ++r) {
+          const int idx = base + r * Ls;
+          if (DST_G) gstore(dst + idx, cscale(v[b][r], scale));
+          else dst[idx + (idx >> ps)] = v[b][r];
+        }
+      }
+      if (Ls > 1) {
+        q += tq;
+        k += tk;
+        if (k >= Ls) { k -= Ls; ++q; }
       }
     }
   }
@@ -323,16 +367,19 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
 // pass loop of lds_fft makes the register allocator keep all cases' arrays
 // live at once and spill; as separate functions each gets its own budget.
 template <typename V, int R, int MAXB, int DIR>
-__device__ __noinline__ void lds_pass_ool(V* lds, int N, int Ls, int T, int tid,
-                                          const V* __restrict__ tw, bool active, int ps) {
-  stockham_pass<V, R, MAXB, DIR, false, false>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
+__device__ __noinline__ void lds_pass_ool(lds_ptr<V> lds_s, int N, int Ls, int T, int tid,
+                                          gbl_cptr<V> tw_g, bool active, int ps) {
+  V* lds = (V*)lds_s;
+  stockham_pass<V, R, MAXB, DIR, false, false>(lds, lds, N, Ls, T, tid, (const V*)tw_g, active, 1, ps);
 }
 
 // Run every pass of `pd` on an LDS-resident transform (LDS -> LDS).
 // PPT bounds the points per thread (N <= PPT * T).
 template <typename V, int PPT, int DIR>
-__device__ __forceinline__ void lds_fft(V* lds, int N, int T, int tid, const PassDesc& pd,
-                                        const V* __restrict__ tw, bool active, int ps) {
+__device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const PassDesc& pd,
+                                        const V* __restrict__ tw_gen, bool active, int ps) {
+  const lds_ptr<V> lds = (lds_ptr<V>)lds_gen;
+  const gbl_cptr<V> tw = (gbl_cptr<V>)tw_gen;
   int Ls = 1;
   for (int p = 0; p < pd.npass; ++p) {
     const int R = pd.radix[p];

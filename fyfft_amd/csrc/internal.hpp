@@ -167,7 +167,7 @@ template <typename V> struct GLen {
 
 template <typename V> struct GroupedArgs {
   const GTask* tasks = nullptr;
-  int ntasks = 0;
+  int xbase[dev::kXcds + 1] = {0};  // tasks of XCD x are [xbase[x], xbase[x+1])
   const GLen<V>* lens = nullptr;
   const long long* off = nullptr;   // element offset of each transform (list order)
   const long long* woff = nullptr;  // workspace offset of each transform (list order)

@@ -28,8 +28,9 @@ template <typename V, int PPT, int DIR, int MODE>
 __global__ __launch_bounds__(1024) void k_fs_col(FsArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int groups = (a.n2 + (1 << a.log2c) - 1) >> a.log2c;
-  const long long t = blockIdx.x / groups;
-  const int c0 = (int)(blockIdx.x - t * groups) << a.log2c;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const long long t = lb / groups;
+  const int c0 = (int)(lb - t * groups) << a.log2c;
   body_fs_col<V, PPT, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
                                  a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
                                  reinterpret_cast<V*>(smem_raw));
@@ -39,8 +40,9 @@ template <typename V, int PPT, int DIR, int MODE>
 __global__ __launch_bounds__(1024) void k_fs_row(FsArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int groups = (a.n1 + (1 << a.log2c) - 1) >> a.log2c;
-  const long long t = blockIdx.x / groups;
-  const int r0 = (int)(blockIdx.x - t * groups) << a.log2c;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const long long t = lb / groups;
+  const int r0 = (int)(lb - t * groups) << a.log2c;
   body_fs_row<V, PPT, DIR, MODE>(a, a.tmp + t * a.tmp_dist, a.out + t * a.out_dist,
                                  a.x0buf ? a.x0buf + t : nullptr, r0,
                                  reinterpret_cast<V*>(smem_raw));

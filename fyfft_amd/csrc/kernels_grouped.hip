@@ -6,8 +6,11 @@
 //          column pass of every four-step transform.
 // Phase 1: the row pass of every four-step transform.
 // Phase 2: the closing column pass of four-step Rader/Bluestein transforms.
-// Each workgroup takes tasks blockIdx.x, blockIdx.x + gridDim.x, ... from a
-// table sorted by decreasing cost, so the long tasks start first.  The task
+// The plan bin-packs the tasks onto the eight XCDs, keeping all tasks of one
+// transform on one XCD (so its cache lines are shared in one L2), each XCD's
+// list sorted by decreasing cost.  Workgroup b runs on XCD b mod 8 and takes
+// every (gridDim/8)-th task of that XCD's list, so the long tasks start
+// first.  The task
 // bodies are the same device code as the batched kernels (bodies.hpp); each
 // runs out of line so that the register budget of one body does not add to
 // the others'.
@@ -90,7 +93,8 @@ __global__ __launch_bounds__(1024) void k_grouped(GroupedArgs<V> ga) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TaskCtx<V> c{ga.off, ga.woff, ga.in, ga.out, ga.ws, ga.x0buf,
                     reinterpret_cast<V*>(smem_raw), ga.tables, ga.normalize};
-  for (int ti = blockIdx.x; ti < ga.ntasks; ti += gridDim.x) {
+  const int x = blockIdx.x % kXcds, step = gridDim.x / kXcds;
+  for (int ti = ga.xbase[x] + blockIdx.x / kXcds; ti < ga.xbase[x + 1]; ti += step) {
     const GTask t = ga.tasks[ti];
     const GLen<V>& g = ga.lens[t.len];
     __syncthreads();  // the previous task's LDS reads are complete

@@ -107,8 +107,10 @@ static const FixedEntry kFixed[] = {
     FIXED(kF32, float2, 4096, 256, 1, 30, 3, 16, 16, 16),
     FIXED(kF32, float2, 8192, 512, 1, 30, 4, 16, 8, 8, 8),
     FIXED(kF32, float2, 16384, 1024, 1, 30, 4, 16, 16, 8, 8),
+    FIXED(kF32, float2, 6720, 420, 1, 4, 5, 8, 8, 7, 5, 3),
     FIXED(kF64, double2, 1024, 64, 4, 30, 3, 16, 8, 8),
     FIXED(kF64, double2, 4096, 256, 1, 30, 3, 16, 16, 16),
+    FIXED(kF64, double2, 6720, 840, 1, 4, 5, 8, 8, 7, 5, 3),
 };
 #undef FIXED
 
