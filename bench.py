@@ -41,6 +41,8 @@ def parse():
                     help="transforms in the CPU baseline sample")
     ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral"],
                     default="c2c")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="3 FFT launches and 3 plain copies of the same size, for rocprofv3 --pmc")
     return ap.parse_args()
 
 
@@ -261,6 +263,16 @@ def main():
     plan = fy.Plan(n, batch, prec)
     stream = torch.cuda.current_stream(dev)
     h = stream.cuda_stream
+    if a.pmc_probe:
+        # counter calibration: the FFT kernel and an elementwise copy kernel
+        # (read + write of the same 2*elem*n*batch bytes)
+        xr, yr = torch.view_as_real(x), torch.view_as_real(y)
+        for _ in range(3):
+            plan.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h)
+            torch.mul(xr, 1.0, out=yr)
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": plan.describe(), "bytes_each_way": x.numel() * x.element_size()}))
+        return
 
     for _ in range(a.warmup):
         plan.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h)
@@ -309,6 +321,13 @@ def main():
                      "traffic": traffic},
         "wall_s": round(wall, 3),
     }
+    # the same workload on the plain (generic) kernels
+    with fy.Plan(n, batch, prec, fy.FORM_PLAIN) as pp:
+        ms_plain = time_steps(lambda: pp.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h),
+                              max(10, a.steps // 4), max(3, a.warmup // 4), stream)
+        out["plain_form"] = {"plan": pp.describe(), "ms_per_step": round(ms_plain, 5),
+                             "hbm_frac": round(alg_bytes / (ms_plain * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                             "speedup_of_value": round(ms_plain / ms, 3)}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, min(a.cpu_sample, batch), a.precision)
     plan.close()

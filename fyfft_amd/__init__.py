@@ -31,6 +31,11 @@ MAX_LENGTH = 1 << 24          # this library's limit (FYFFT_MAX_LENGTH)
 
 FORWARD, INVERSE, ADJOINT = 0, 1, 2
 F32, F64, BF16, F16 = 0, 1, 2, 3
+FORM_AUTO, FORM_PLAIN, FORM_TUNED = 0, 1, 2
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
 
 
 class FFT_execution_strategy(enum.IntEnum):
@@ -95,6 +100,10 @@ def _load() -> ctypes.CDLL:
         L.fyfft_irfft_modes.argtypes = [vp, vp, vp, sz, dbl, dbl, vp]
         L.fyfft_plan_create_grouped.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(sz), sz, i, ctypes.c_uint]
         L.fyfft_plan_elements.argtypes = [vp, ctypes.POINTER(sz)]
+        L.fyfft_plan_create_ex.argtypes = [ctypes.POINTER(vp), sz, sz, i, ctypes.POINTER(_Options)]
+        L.fyfft_wisdom_export.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+        L.fyfft_wisdom_import.argtypes = [ctypes.c_char_p]
+        L.fyfft_wisdom_forget.argtypes = []
         L.fyfft_status_string.argtypes = [i]
         L.fyfft_status_string.restype = ctypes.c_char_p
         L.fyfft_last_error.argtypes = []
@@ -103,7 +112,8 @@ def _load() -> ctypes.CDLL:
                      "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
                      "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_plan_create_grouped",
-                     "fyfft_plan_elements"):
+                     "fyfft_plan_elements", "fyfft_plan_create_ex", "fyfft_wisdom_export",
+                     "fyfft_wisdom_import", "fyfft_wisdom_forget"):
             getattr(L, name).restype = i
         _lib = L
         return L
@@ -117,6 +127,24 @@ def lib() -> ctypes.CDLL:
 def _check(status: int, what: str) -> None:
     if status != 0:
         raise FyfftError(status, what)
+
+
+# ------------------------------------------------------------------ wisdom
+def wisdom_export() -> str:
+    """Autotuner choices of this process as text."""
+    need = ctypes.c_size_t()
+    _check(_load().fyfft_wisdom_export(None, 0, ctypes.byref(need)), "wisdom_export")
+    buf = ctypes.create_string_buffer(need.value)
+    _check(_load().fyfft_wisdom_export(buf, need.value, None), "wisdom_export")
+    return buf.value.decode()
+
+
+def wisdom_import(text: str) -> None:
+    _check(_load().fyfft_wisdom_import(text.encode()), "wisdom_import")
+
+
+def wisdom_forget() -> None:
+    _check(_load().fyfft_wisdom_forget(), "wisdom_forget")
 
 
 # ------------------------------------------------------------------ queries
@@ -151,11 +179,13 @@ class Plan:
     torch stream (or ``stream``) and does not synchronise.
     """
 
-    def __init__(self, n: int, batch: int = 1, precision: int = F32):
+    def __init__(self, n: int, batch: int = 1, precision: int = F32, form: int = FORM_AUTO):
         self.n, self.batch, self.precision = int(n), int(batch), int(precision)
         self._h = ctypes.c_void_p()
-        _check(_load().fyfft_plan_create(ctypes.byref(self._h), self.n, self.batch, self.precision),
-               f"plan_create(N={n}, batch={batch}, precision={precision})")
+        opts = _Options(int(form))
+        _check(_load().fyfft_plan_create_ex(ctypes.byref(self._h), self.n, self.batch, self.precision,
+                                            ctypes.byref(opts)),
+               f"plan_create(N={n}, batch={batch}, precision={precision}, form={form})")
 
     def close(self) -> None:
         if self._h:

@@ -198,11 +198,249 @@ static fyfft_status alloc_workspace(fyfft_plan* p, size_t len, bool x0) {
 
 // Forward FFT of a length-len FP64 sequence in place with a temporary F64
 // plan of this library (used to build kernel spectra from FP64 tables).
-static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, bool);
+
+// ------------------------------------------------------------ autotuner
+// FYFFT_FORM_TUNED plans of LDS-resident smooth lengths time a set of
+// candidate configurations on the device at plan time and keep the fastest:
+// the compile-time kernel when one exists, and the generic kernel over
+// several radix orders (power-of-two split and placement of the odd
+// radices), LDS padding on/off and two points-per-thread classes, each with
+// its own FP64 twiddle table.  Choices are remembered per (N, precision) as
+// "wisdom", which can be exported and imported as text.
+struct LdsChoice {
+  bool fixed = false;
+  int ppt = 16, ps = 4, tpb = 1;
+  std::vector<int> rad;
+};
+static std::mutex g_wisdom_mu;
+static std::map<std::pair<size_t, int>, LdsChoice> g_wisdom;
+
+static std::vector<std::vector<int>> radix_orders(size_t n) {
+  const std::vector<int> base = radices_for(n);
+  std::vector<int> pow2, odd;
+  int bits = 0;
+  for (int r : base) {
+    if (r & (r - 1)) odd.push_back(r);
+    else { pow2.push_back(r); int b = 0; while ((1 << b) < r) ++b; bits += b; }
+  }
+  std::vector<std::vector<int>> p2s = {pow2};
+  if (bits > 0) {
+    std::vector<int> greedy;
+    int left = bits;
+    while (left > 0) { const int b = std::min(4, left); greedy.push_back(1 << b); left -= b; }
+    p2s.push_back(greedy);
+    std::vector<int> rev(greedy.rbegin(), greedy.rend());
+    p2s.push_back(rev);
+  }
+  std::vector<std::vector<int>> odds = {odd};
+  if (odd.size() > 1) odds.push_back(std::vector<int>(odd.rbegin(), odd.rend()));
+  std::vector<std::vector<int>> out;
+  for (const auto& a : p2s)
+    for (const auto& o : odds)
+      for (int odd_first = 0; odd_first < 2; ++odd_first) {
+        if (odd_first && (o.empty() || a.empty())) continue;
+        std::vector<int> r = odd_first ? o : a;
+        const auto& tail = odd_first ? a : o;
+        r.insert(r.end(), tail.begin(), tail.end());
+        if (std::find(out.begin(), out.end(), r) == out.end()) out.push_back(r);
+      }
+  return out;
+}
+
+// Geometry + table for one candidate; the table is a plain hipMalloc owned by
+// the caller (*tw_alloc).
+static fyfft_status make_candidate(fyfft_plan* p, size_t n, const LdsChoice& c, LdsLaunch* L,
+                                   void** tw_alloc) {
+  const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
+  *tw_alloc = nullptr;
+  if (c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses) return FYFFT_UNSUPPORTED;
+  L->pd.npass = (int)c.rad.size();
+  for (size_t i = 0; i < c.rad.size(); ++i) L->pd.radix[i] = c.rad[i];
+  LdsGeom g;
+  g.n = (int)n;
+  g.ppt = c.ppt;
+  g.threads = (int)((n + c.ppt - 1) / c.ppt);
+  g.pad_shift = c.ps;
+  g.per_block = c.tpb;
+  const size_t stride = (size_t)dev::lds_len((int)n, c.ps);
+  g.lds_bytes = stride * c.tpb * elem;
+  if (g.threads > 1024 || g.threads * c.tpb > 1024 || g.lds_bytes > cap) return FYFFT_UNSUPPORTED;
+  L->geom = g;
+  L->fixed = -1;
+  if (c.fixed) {
+    L->fixed = find_fixed_kernel(p->prec, L->pd, &L->geom);
+    if (L->fixed < 0) return FYFFT_UNSUPPORTED;
+  }
+  HIP_TRY(hipMalloc(tw_alloc, std::max<size_t>(n, 2) * elem));
+  HIP_TRY(launch_gen_stockham_twiddles(p->prec, *tw_alloc, (int)n, L->pd, nullptr));
+  L->tw = *tw_alloc;
+  return FYFFT_SUCCESS;
+}
+
+static std::vector<LdsChoice> lds_candidates(fyfft_plan* p, size_t n) {
+  std::vector<LdsChoice> out;
+  LdsGeom g;
+  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) return out;
+  const bool f64 = p->prec == kF64;
+  const int alt_ppt = f64 ? (g.ppt == 8 ? 16 : 8) : (g.ppt == 16 ? 32 : 16);
+  auto tpb_for = [&](int ppt, int ps) {
+    const int T = (int)((n + ppt - 1) / ppt);
+    const size_t per = (size_t)dev::lds_len((int)n, ps) * elem_bytes(p->prec);
+    int t = std::max(1, 256 / T);
+    t = (int)std::min<size_t>(t, lds_cap_bytes(p->device) / per);
+    return std::max(1, std::min(t, 1024 / T));
+  };
+  const auto orders = radix_orders(n);
+  {
+    LdsChoice c;
+    c.fixed = true;
+    c.rad = orders[0];
+    c.ppt = g.ppt;
+    c.ps = g.pad_shift;
+    c.tpb = 1;
+    out.push_back(c);  // kept only if a fixed kernel exists (checked in make_candidate)
+  }
+  for (size_t o = 0; o < orders.size() && out.size() < 16; ++o)
+    for (int ps : {g.pad_shift, 30}) {
+      LdsChoice c;
+      c.rad = orders[o];
+      c.ppt = g.ppt;
+      c.ps = ps;
+      c.tpb = tpb_for(c.ppt, ps);
+      out.push_back(c);
+      if (g.pad_shift == 30) break;
+    }
+  LdsChoice c;
+  c.rad = orders[0];
+  c.ppt = alt_ppt;
+  c.ps = g.pad_shift;
+  if ((n + alt_ppt - 1) / alt_ppt <= 1024) {
+    c.tpb = tpb_for(alt_ppt, c.ps);
+    out.push_back(c);
+  }
+  return out;
+}
+
+static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n) {
+  const std::pair<size_t, int> key{n, p->prec};
+  LdsChoice best;
+  bool have = false;
+  {
+    std::lock_guard<std::mutex> lock(g_wisdom_mu);
+    auto it = g_wisdom.find(key);
+    if (it != g_wisdom.end()) { best = it->second; have = true; }
+  }
+  if (!have) {
+    const size_t elem = elem_bytes(p->prec);
+    const size_t b = std::max<size_t>(1, std::min<size_t>(p->batch, ((size_t)64 << 20) / (n * elem)));
+    void *in = nullptr, *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipMalloc(&in, b * n * elem));
+    hipError_t e = hipMalloc(&out, b * n * elem);
+    if (e == hipSuccess) e = hipMemset(in, 0, b * n * elem);
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best_ms = 1e30f;
+    for (const LdsChoice& c : lds_candidates(p, n)) {
+      if (e != hipSuccess) break;
+      LdsLaunch L;
+      void* tw = nullptr;
+      if (make_candidate(p, n, c, &L, &tw) == FYFFT_SUCCESS) {
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_lds(p->prec, kForward, L, in, out, b, n, 1.0, st);
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int i = 0; i < 5 && e == hipSuccess; ++i) e = launch_lds(p->prec, kForward, L, in, out, b, n, 1.0, st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e == hipSuccess && ms < best_ms) {
+          best_ms = ms;
+          best = c;
+          have = true;
+        }
+      }
+      if (tw) (void)hipFree(tw);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(in);
+    if (out) (void)hipFree(out);
+    if (e != hipSuccess) return from_hip(e, "autotune");
+    if (!have) return FYFFT_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_wisdom_mu);
+    g_wisdom[key] = best;
+  }
+  void* tw = nullptr;
+  fyfft_status st2 = make_candidate(p, n, best, &p->lds, &tw);
+  if (tw) {
+    p->allocs.push_back(tw);
+    p->dev_bytes += std::max<size_t>(n, 2) * elem_bytes(p->prec);
+  }
+  return st2;
+}
+
+static std::string wisdom_text() {
+  std::lock_guard<std::mutex> lock(g_wisdom_mu);
+  std::string s = "fyfft-wisdom 1\n";
+  for (const auto& kv : g_wisdom) {
+    const LdsChoice& c = kv.second;
+    s += "lds " + std::to_string(kv.first.first) + " " + std::to_string(kv.first.second) + " " +
+         std::to_string(c.fixed ? 1 : 0) + " " + std::to_string(c.ppt) + " " + std::to_string(c.ps) +
+         " " + std::to_string(c.tpb) + " ";
+    for (size_t i = 0; i < c.rad.size(); ++i) s += (i ? "x" : "") + std::to_string(c.rad[i]);
+    s += "\n";
+  }
+  return s;
+}
+
+static fyfft_status wisdom_parse(const char* text) {
+  std::string t(text);
+  size_t pos = t.find('\n');
+  if (t.compare(0, 15, "fyfft-wisdom 1\n") != 0 || pos == std::string::npos) return FYFFT_INVALID_ARGUMENT;
+  std::map<std::pair<size_t, int>, LdsChoice> parsed;
+  size_t start = pos + 1;
+  while (start < t.size()) {
+    size_t end = t.find('\n', start);
+    if (end == std::string::npos) end = t.size();
+    const std::string line = t.substr(start, end - start);
+    start = end + 1;
+    if (line.empty()) continue;
+    unsigned long long n = 0;
+    int prec = 0, fixed = 0, ppt = 0, ps = 0, tpb = 0;
+    char rad[256] = {0};
+    if (std::sscanf(line.c_str(), "lds %llu %d %d %d %d %d %255s", &n, &prec, &fixed, &ppt, &ps, &tpb, rad) != 7)
+      return FYFFT_INVALID_ARGUMENT;
+    LdsChoice c;
+    c.fixed = fixed != 0;
+    c.ppt = ppt;
+    c.ps = ps;
+    c.tpb = tpb;
+    size_t prod = 1;
+    for (char* tok = std::strtok(rad, "x"); tok; tok = std::strtok(nullptr, "x")) {
+      const int r = std::atoi(tok);
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 16) return FYFFT_INVALID_ARGUMENT;
+      c.rad.push_back(r);
+      prod *= (size_t)r;
+    }
+    const bool ppt_ok = prec == kF64 ? (ppt == 8 || ppt == 16) : (ppt == 16 || ppt == 32);
+    if (prod != n || (prec != kF32 && prec != kF64) || !ppt_ok || (ps != 30 && (ps < 2 || ps > 8)) ||
+        tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses)
+      return FYFFT_INVALID_ARGUMENT;
+    parsed[{(size_t)n, prec}] = c;
+  }
+  std::lock_guard<std::mutex> lock(g_wisdom_mu);
+  for (auto& kv : parsed) g_wisdom[kv.first] = kv.second;
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, int);
 
 static fyfft_status fft_f64_inplace(double2* buf, size_t len) {
   fyfft_plan* sub = nullptr;
-  fyfft_status st = create_plan_impl(&sub, len, 1, FYFFT_F64, true);
+  fyfft_status st = create_plan_impl(&sub, len, 1, FYFFT_F64, FYFFT_FORM_AUTO);
   if (st) return st;
   st = fyfft_execute(sub, buf, buf, FYFFT_FORWARD, nullptr);
   if (!st) {
@@ -428,7 +666,7 @@ static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src,
 }
 
 static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
-                                     bool allow_fixed) {
+                                     int form) {
   if (!out) return FYFFT_INVALID_ARGUMENT;
   *out = nullptr;
   if (batch == 0) return FYFFT_INVALID_ARGUMENT;
@@ -437,6 +675,8 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     return FYFFT_INVALID_ARGUMENT;
   }
   if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+  if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED)
+    return FYFFT_INVALID_ARGUMENT;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   fyfft_plan* p = new (std::nothrow) fyfft_plan();
@@ -454,10 +694,11 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     LdsGeom g;
     if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
       p->kind = kLds;
-      st = plan_lds(p, n, &p->lds, allow_fixed);
+      st = form == FYFFT_FORM_TUNED ? plan_lds_tuned(p, n) : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO);
       if (!st) {
         const LdsGeom& q = p->lds.geom;
-        p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") + " N=" +
+        p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") +
+                  (form == FYFFT_FORM_TUNED ? "(tuned)" : "") + " N=" +
                   std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
                   " T=" + std::to_string(q.threads) + " TPB=" + std::to_string(q.per_block) +
                   " pad=" + std::to_string(q.pad_shift);
@@ -764,7 +1005,38 @@ fyfft_status fyfft_length_radices(size_t n, int* radices, int max, int* count) {
 }
 
 fyfft_status fyfft_plan_create(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision) {
-  return create_plan_impl(out, n, batch, precision, true);
+  return create_plan_impl(out, n, batch, precision, FYFFT_FORM_AUTO);
+}
+
+fyfft_status fyfft_plan_create_ex(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
+                                  const fyfft_options* options) {
+  int form = FYFFT_FORM_AUTO;
+  if (options) {
+    for (int r : options->reserved)
+      if (r != 0) return FYFFT_INVALID_ARGUMENT;
+    form = options->form;
+  }
+  return create_plan_impl(out, n, batch, precision, form);
+}
+
+fyfft_status fyfft_wisdom_export(char* buf, size_t len, size_t* needed) {
+  const std::string t = wisdom_text();
+  if (needed) *needed = t.size() + 1;
+  if (!buf) return needed ? FYFFT_SUCCESS : FYFFT_INVALID_ARGUMENT;
+  if (len < t.size() + 1) return FYFFT_INVALID_ARGUMENT;
+  std::memcpy(buf, t.c_str(), t.size() + 1);
+  return FYFFT_SUCCESS;
+}
+
+fyfft_status fyfft_wisdom_import(const char* text) {
+  if (!text) return FYFFT_INVALID_ARGUMENT;
+  return wisdom_parse(text);
+}
+
+fyfft_status fyfft_wisdom_forget(void) {
+  std::lock_guard<std::mutex> lock(g_wisdom_mu);
+  g_wisdom.clear();
+  return FYFFT_SUCCESS;
 }
 
 fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, size_t count,
@@ -799,7 +1071,7 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
   fyfft_status st = FYFFT_SUCCESS;
   for (const auto& kv : by_len) {
     fyfft_plan* sp = nullptr;
-    st = create_plan_impl(&sp, kv.first, 1, precision, false);
+    st = create_plan_impl(&sp, kv.first, 1, precision, FYFFT_FORM_PLAIN);
     if (st) break;
     for (size_t i : kv.second) p->sub_of[i] = (int)p->subs.size();
     p->subs.push_back(sp);

@@ -80,6 +80,23 @@ typedef enum fyfft_strategy {
   FYFFT_DFT = 4
 } fyfft_strategy;
 
+/* Kernel form of a plan (fyfft_plan_create_ex):
+ *   AUTO  - a compile-time specialised kernel where one exists for the
+ *           length, else the generic kernels (what fyfft_plan_create does);
+ *   PLAIN - always the generic ("plain GPU form") kernels, the baseline the
+ *           other forms are measured against;
+ *   TUNED - time candidate configurations (radix order, LDS padding, points
+ *           per thread, specialised vs generic) on the device at plan time
+ *           and keep the fastest; the choice is remembered as wisdom per
+ *           (N, precision).  Applies to lengths held in one workgroup's LDS;
+ *           other lengths plan as AUTO. */
+typedef enum fyfft_form { FYFFT_FORM_AUTO = 0, FYFFT_FORM_PLAIN = 1, FYFFT_FORM_TUNED = 2 } fyfft_form;
+
+typedef struct fyfft_options {
+  int form;         /* fyfft_form */
+  int reserved[7];  /* must be zero */
+} fyfft_options;
+
 typedef struct fyfft_plan fyfft_plan;
 
 /* Replaces FFT1D_C2C_invoker::FFT1D_C2C_invoker(std::size_t)
@@ -109,6 +126,20 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** plan, const size_t* lengths,
 /* Elements (complex values) one execution reads and writes: N * batch, or
  * the sum of the lengths of a grouped plan. */
 fyfft_status fyfft_plan_elements(const fyfft_plan* plan, size_t* elements);
+
+/* fyfft_plan_create with options (NULL = defaults, same as
+ * fyfft_plan_create).  No counterpart in the reference. */
+fyfft_status fyfft_plan_create_ex(fyfft_plan** plan, size_t n, size_t batch,
+                                  fyfft_precision precision, const fyfft_options* options);
+
+/* Autotuner wisdom as text ("fyfft-wisdom 1" and one line per (N,
+ * precision)): export writes it to buf (NUL-terminated; *needed receives the
+ * size including the NUL, buf may be NULL to query), import merges lines
+ * into the process's wisdom (FYFFT_INVALID_ARGUMENT on malformed text),
+ * forget clears it.  The reference has no plan serialisation. */
+fyfft_status fyfft_wisdom_export(char* buf, size_t len, size_t* needed);
+fyfft_status fyfft_wisdom_import(const char* text);
+fyfft_status fyfft_wisdom_forget(void);
 
 /* Releases every device allocation of the plan; NULL is accepted. */
 fyfft_status fyfft_plan_destroy(fyfft_plan* plan);

@@ -83,3 +83,28 @@ def test_invoker_mirror_strategy(fy):
     assert S(65537) == fy.FFT_execution_strategy.rader
     with pytest.raises(ValueError):
         fy.FFT1D_C2C_invoker(1)
+
+
+def test_wisdom_text_roundtrip_and_validation(fy):
+    fy.wisdom_forget()
+    assert fy.wisdom_export() == "fyfft-wisdom 1\n"
+    line = "lds 6720 0 0 16 4 1 8x8x7x5x3\n"
+    fy.wisdom_import("fyfft-wisdom 1\n" + line)
+    assert fy.wisdom_export() == "fyfft-wisdom 1\n" + line
+    for bad in ["nonsense\n", "fyfft-wisdom 1\nlds 6720 0 0 16 4 1 8x8x7x5\n",   # product != N
+                "fyfft-wisdom 1\nlds 64 0 0 12 4 1 8x8\n",                      # bad points/thread
+                "fyfft-wisdom 1\nlds 64 0 0 16 4 1 6x8\n"]:                     # radix 6 not a codelet
+        with pytest.raises(fy.FyfftError):
+            fy.wisdom_import(bad)
+    assert fy.wisdom_export() == "fyfft-wisdom 1\n" + line
+    fy.wisdom_forget()
+
+
+def test_options_reserved_must_be_zero(fy):
+    import ctypes
+    opts = fy._Options(0)
+    opts.reserved[3] = 1
+    h = ctypes.c_void_p()
+    assert fy.lib().fyfft_plan_create_ex(ctypes.byref(h), 64, 1, 0, ctypes.byref(opts)) == 1
+    opts = fy._Options(7)
+    assert fy.lib().fyfft_plan_create_ex(ctypes.byref(h), 64, 1, 0, ctypes.byref(opts)) == 1
