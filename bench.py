@@ -12,6 +12,7 @@ Other configurations (one JSON line each, same fields where they apply):
   --workload prime     Rader N=65537 x 2048, Bluestein N=10007 x 1024 and N=1000003 x 16
   --workload ragged    20000 ragged transforms in one grouped launch vs one launch per transform
   --workload spectral  spectral convolution [256, 64, 4096], 512 modes, forward + backward
+  --workload lowprec   BF16 and FP16 storage, N=4096 x 65536, radix-16 stages on MFMA
 """
 from __future__ import annotations
 
@@ -39,7 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample", type=int, default=16384,
                     help="transforms in the CPU baseline sample")
-    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral"],
+    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral", "lowprec"],
                     default="c2c")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="3 FFT launches and 3 plain copies of the same size, for rocprofv3 --pmc")
@@ -222,6 +223,50 @@ def bench_spectral(a):
                 unit="GFLOP/s")
 
 
+def bench_lowprec(a):
+    import numpy as np
+    import torch
+    import fyfft_amd as fy
+    import oracle
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    n, batch = a.n, a.batch
+    rows, tot_ms, tot_bytes, tot_flops = [], 0.0, 0.0, 0.0
+    for name, tdt, prec in (("bf16", torch.bfloat16, fy.BF16), ("fp16", torch.float16, fy.F16)):
+        g = torch.Generator(device=dev).manual_seed(0)
+        x = torch.randn(batch, n, 2, device=dev, generator=g).to(tdt)
+        y = torch.empty_like(x)
+        with fy.Plan(n, batch, prec) as p:
+            h = stream.cuda_stream
+            ms = time_steps(lambda: p.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h),
+                            a.steps, a.warmup, stream)
+            desc = p.describe()
+        r = [0, batch - 1]
+        xq = x[r].float().cpu().numpy().astype(np.float64)
+        ex = oracle.fft_rows(xq[..., 0] + 1j * xq[..., 1])
+        ext = torch.from_numpy(np.stack([ex.real, ex.imag], -1))
+        err = float((y[r].cpu().double() - ext).norm() / ext.norm())
+        floor = float((ext.to(tdt).double() - ext).norm() / ext.norm())
+        byt = 2.0 * 4 * n * batch
+        rows.append({"dtype": name, "plan": desc, "ms": round(ms, 4),
+                     "GFLOP/s": round(flops_of(n) * batch / ms / 1e6, 1),
+                     "hbm_frac": round(byt / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                     "rel_l2_vs_fp64": err, "storage_rounding_floor": floor})
+        tot_ms += ms
+        tot_bytes += byt
+        tot_flops += flops_of(n) * batch
+        del x, y
+    return line({"config": {"workload": f"low-precision storage, forward C2C N={n} x {batch}, "
+                                        "radix-16 stages on MFMA with split hi/lo fp16 operands",
+                            "cases": rows}},
+                value=round(tot_flops / tot_ms / 1e6, 2), steps=a.steps, warmup=a.warmup,
+                ms_per_step=round(tot_ms, 4), dtype="bf16/fp16",
+                roofline={"bound": "hbm", "achieved": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                          "traffic": None})
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -242,6 +287,8 @@ def main():
             out = bench_transforms(a, cases, "prime")
         elif a.workload == "ragged":
             out = bench_ragged(a)
+        elif a.workload == "lowprec":
+            out = bench_lowprec(a)
         else:
             out = bench_spectral(a)
         print(json.dumps(out), flush=True)

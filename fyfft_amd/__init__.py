@@ -162,12 +162,25 @@ def length_radices(n: int) -> list[int]:
 
 
 def _precision_of(dtype) -> int:
+    """complex64 -> F32, complex128 -> F64; bfloat16 / float16 tensors whose
+    last dimension is (re, im) -> BF16 / F16 storage."""
     import torch
     if dtype == torch.complex64:
         return F32
     if dtype == torch.complex128:
         return F64
-    raise TypeError(f"fyfft_amd: unsupported dtype {dtype} (complex64 / complex128)")
+    if dtype == torch.bfloat16:
+        return BF16
+    if dtype == torch.float16 or dtype == torch.complex32:
+        return F16
+    raise TypeError(f"fyfft_amd: unsupported dtype {dtype}")
+
+
+def _complex_numel(t) -> int:
+    import torch
+    if t.dtype in (torch.bfloat16, torch.float16):
+        return t.numel() // 2
+    return t.numel()
 
 
 # ------------------------------------------------------------------ plan
@@ -229,8 +242,9 @@ class Plan:
             raise TypeError(f"{name} dtype {t.dtype} does not match the plan precision")
         if not t.is_contiguous():
             raise ValueError(f"{name} must be contiguous")
-        if t.numel() != self.n * self.batch:
-            raise ValueError(f"{name} has {t.numel()} elements, plan needs {self.n * self.batch}")
+        if _complex_numel(t) != self.n * self.batch:
+            raise ValueError(f"{name} has {_complex_numel(t)} complex elements, plan needs "
+                             f"{self.n * self.batch}")
 
     def execute_ptr(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0) -> None:
         _check(_load().fyfft_execute(self._h, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
@@ -249,6 +263,8 @@ class Plan:
         return out
 
     def execute_host(self, x: np.ndarray, out: np.ndarray | None = None, direction: int = FORWARD):
+        if self.precision not in (F32, F64):
+            raise TypeError("execute_host: use device tensors for BF16/F16 plans")
         dt = np.complex64 if self.precision == F32 else np.complex128
         x = np.ascontiguousarray(x, dtype=dt)
         if x.size != self.n * self.batch:

@@ -21,7 +21,7 @@ namespace {
 constexpr uint32_t kMagic = 0xF1FF7A11u;
 thread_local std::string g_last_error;
 
-enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7 };
+enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7, kMfma = 8 };
 
 // Workspace of multi-pass plans is capped; larger batches run in chunks.
 constexpr size_t kWorkspaceCap = (size_t)1 << 30;
@@ -670,24 +670,43 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
   if (!out) return FYFFT_INVALID_ARGUMENT;
   *out = nullptr;
   if (batch == 0) return FYFFT_INVALID_ARGUMENT;
-  if (precision != FYFFT_F32 && precision != FYFFT_F64) {
-    if (precision == FYFFT_BF16 || precision == FYFFT_F16) return FYFFT_UNSUPPORTED;
+  if (precision != FYFFT_F32 && precision != FYFFT_F64 && precision != FYFFT_BF16 &&
+      precision != FYFFT_F16)
     return FYFFT_INVALID_ARGUMENT;
-  }
   if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
   if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED)
     return FYFFT_INVALID_ARGUMENT;
+  const bool low = precision == FYFFT_BF16 || precision == FYFFT_F16;
+  int r0 = 1;
+  if (low && !mfma16_supported(n, &r0)) return FYFFT_UNSUPPORTED;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   fyfft_plan* p = new (std::nothrow) fyfft_plan();
   if (!p) return FYFFT_ALLOC_FAILED;
   p->n = n;
   p->batch = batch;
-  p->prec = precision == FYFFT_F64 ? kF64 : kF32;
+  p->prec = precision == FYFFT_F64 ? kF64 : (precision == FYFFT_BF16 ? kBF16 : (precision == FYFFT_F16 ? kF16 : kF32));
   p->device = dev;
   p->strategy = strategy_for(n);
   fyfft_status st = FYFFT_SUCCESS;
-  if (n == 1) {
+  if (low) {
+    // FP32 Stockham table for the pass list [R0,] 16, ..., 16
+    p->kind = kMfma;
+    dev::PassDesc& pd = p->lds.pd;
+    pd.npass = 0;
+    if (r0 > 1) pd.radix[pd.npass++] = r0;
+    for (size_t m = r0; m < n; m *= 16) pd.radix[pd.npass++] = 16;
+    void* tw = nullptr;
+    st = p->alloc(n * sizeof(float2), &tw);
+    if (!st) {
+      const hipError_t e = launch_gen_stockham_twiddles(kF32, tw, (int)n, pd, nullptr);
+      if (e != hipSuccess) st = from_hip(e, "twiddles");
+    }
+    p->lds.tw = tw;
+    p->lds.geom.n = (int)n;
+    p->desc = std::string("mfma16 ") + (precision == FYFFT_BF16 ? "bf16" : "fp16") + " N=" +
+              std::to_string(n) + " radices=" + radix_string(pd) + " split-hi/lo fp16 MFMA";
+  } else if (n == 1) {
     p->kind = kTrivial;
     p->desc = "trivial N=1";
   } else if (is_smooth7(n)) {
@@ -1131,6 +1150,10 @@ fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int d
       return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, s);
     case kGrouped:
       return exec_grouped(p, in, out, direction, s);
+    case kMfma:
+      HIP_TRY(launch_mfma16(p->prec == kBF16, dir == kForward ? -1 : +1, p->n, in, out, p->batch,
+                            p->lds.tw, scale, s));
+      return FYFFT_SUCCESS;
   }
   return FYFFT_INVALID_PLAN;
 }

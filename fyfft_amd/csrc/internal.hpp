@@ -27,10 +27,10 @@ template <typename V> struct LdsBody {
 };
 }  // namespace dev
 
-enum Prec { kF32 = 0, kF64 = 1 };
+enum Prec { kF32 = 0, kF64 = 1, kBF16 = 2, kF16 = 3 };
 enum Dir { kForward = 0, kInverse = 1, kAdjoint = 2 };
 
-inline size_t elem_bytes(int prec) { return prec == kF64 ? 16 : 8; }
+inline size_t elem_bytes(int prec) { return prec == kF64 ? 16 : (prec >= kBF16 ? 4 : 8); }
 
 // Everything one LDS-resident launch needs.
 struct LdsLaunch {
@@ -181,6 +181,13 @@ template <typename V> struct GroupedArgs {
 hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, size_t lds, hipStream_t s);
 // Sets the dynamic-LDS limit of the grouped kernels (call at plan time).
 hipError_t prepare_grouped(int prec, size_t lds);
+
+// --- kernels_mfma.hip
+// BF16/FP16-storage transforms with radix-16 stages on MFMA: N = R0 * 16^P
+// for the supported powers of two (256 .. 16384); *r0 receives R0.
+bool mfma16_supported(size_t n, int* r0);
+hipError_t launch_mfma16(bool bf16, int dir, size_t n, const void* in, void* out, size_t batch,
+                         const void* tw, double scale, hipStream_t s);
 
 // --- kernels_tables.hip
 // Stockham twiddles for pass list pd (N-1 entries) in FP64, stored as prec.

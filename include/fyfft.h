@@ -54,8 +54,8 @@ typedef enum fyfft_status {
 typedef enum fyfft_precision {
   FYFFT_F32 = 0,  /* complex<float>  (the reference's only precision) */
   FYFFT_F64 = 1,  /* complex<double>                                   */
-  FYFFT_BF16 = 2, /* reserved: bf16 storage                            */
-  FYFFT_F16 = 3   /* reserved: fp16 storage                            */
+  FYFFT_BF16 = 2, /* bf16 storage: complex as two bf16 (4 bytes)       */
+  FYFFT_F16 = 3   /* fp16 storage: complex as two fp16 (4 bytes)       */
 } fyfft_precision;
 
 typedef enum fyfft_direction {
@@ -98,6 +98,12 @@ typedef struct fyfft_options {
 } fyfft_options;
 
 typedef struct fyfft_plan fyfft_plan;
+
+/* BF16 / FP16 plans store data as 16-bit complex pairs and run the
+ * radix-16 stages on MFMA with split hi/lo operands and FP32 accumulation
+ * (internal error at FP32 level; the stored result is rounded once).  They
+ * are available for N = 256 .. 16384 powers of two; other lengths return
+ * FYFFT_UNSUPPORTED.  The reference has only complex<float>. */
 
 /* Replaces FFT1D_C2C_invoker::FFT1D_C2C_invoker(std::size_t)
  * (foye_FFT.hpp:101-126), extended with a batch of `batch` contiguous

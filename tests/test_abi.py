@@ -36,7 +36,7 @@ def test_argument_checks_without_gpu(fy):
     assert lib.fyfft_plan_create(ctypes.byref(h), 0, 1, 0) == 2          # invalid length
     assert lib.fyfft_plan_create(ctypes.byref(h), (1 << 24) + 1, 1, 0) == 2
     assert lib.fyfft_plan_create(ctypes.byref(h), 64, 0, 0) == 1         # batch 0
-    assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 2) == 3         # bf16 reserved
+    assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 2) == 3         # bf16: MFMA lengths only
     assert lib.fyfft_plan_create(ctypes.byref(h), 64, 1, 9) == 1         # unknown precision
     assert lib.fyfft_plan_create(None, 64, 1, 0) == 1
     assert lib.fyfft_execute(None, None, None, 0, None) == 6             # invalid plan
