@@ -74,9 +74,12 @@ __device__ __forceinline__ void mfma_dft16(const Mfma16A& A, const float2 x[4], 
   for (int i = 0; i < 4; ++i) mx = fmaxf(mx, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
   mx = fmaxf(mx, __shfl_xor(mx, 16));
   mx = fmaxf(mx, __shfl_xor(mx, 32));
-  int e = 0;
-  if (mx > 0.f) (void)frexpf(mx, &e);  // mx = f * 2^e, f in [0.5, 1)
-  const float s = ldexpf(1.f, -e), inv = ldexpf(1.f, e);
+  // mx = f * 2^e with f in [0.5, 1), read from the exponent field without
+  // branches (zero and subnormal maxima give e = -126, which is harmless);
+  // s = 2^-e and inv = 2^e are built directly as IEEE powers of two.
+  const int e = min((int)((__float_as_uint(mx) >> 23) & 0xff) - 126, 126);
+  const float s = __uint_as_float((unsigned)(127 - e) << 23);
+  const float inv = __uint_as_float((unsigned)(127 + e) << 23);
   f16x8_t bh, bl;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -133,8 +136,10 @@ __device__ __forceinline__ void mfma_pass16(const cplx16<H>* gsrc, cplx16<H>* gd
       const float2* t = tw + (Ls - 1) + k * 15;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int r = 4 * g + i;
-        if (r) x[i] = twmul<DIR>(x[i], t[r - 1]);
+        const int r = 4 * g + i;  // r = 0 only on lanes g = 0: select, no branch
+        const float2 w = t[r > 0 ? r - 1 : 0];
+        const float2 xt = twmul<DIR>(x[i], w);
+        x[i] = r > 0 ? xt : x[i];
       }
     }
     mfma_dft16(A, x, y[q]);
