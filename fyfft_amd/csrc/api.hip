@@ -58,6 +58,7 @@ struct fyfft_plan {
   dev::PassDesc pdcol{}, pdrow{};
   void *twcol = nullptr, *twrow = nullptr, *tw_hi = nullptr, *tw_lo = nullptr;
   int lo_bits = 0;
+  int fixed_col = 0, fixed_row = 0;  // compile-time sub-transform lengths (0 = generic)
   size_t chunk = 0;     // transforms per workspace chunk
   void* tmp = nullptr;  // chunk * conv length elements
   void* x0buf = nullptr;
@@ -154,8 +155,13 @@ static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_f
 static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g) {
   if (!lds_geometry(len, elem, cap, g)) return false;
   const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
+  // 16 sub-transforms per workgroup make 128-byte row segments for FP32
+  // columns; keep 16 if the whole LDS allows it, otherwise the largest
+  // count that leaves room for two workgroups per CU.
   int c = 16;
-  while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
+  if (!((size_t)c * g->threads <= 1024 && (size_t)c * stride * elem <= cap)) {
+    while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
+  }
   if ((size_t)c * stride * elem > cap) return false;
   g->per_block = c;
   g->lds_bytes = (size_t)c * stride * elem;
@@ -178,6 +184,8 @@ static fyfft_status plan_fourstep(fyfft_plan* p, size_t n) {
   fyfft_status st = make_pass_table(p, p->n1, &p->pdcol, &p->twcol);
   if (!st) st = make_pass_table(p, p->n2, &p->pdrow, &p->twrow);
   if (st) return st;
+  p->fixed_col = find_fs_fixed(p->prec, p->pdcol, (int)p->n1, p->gcol.threads);
+  p->fixed_row = find_fs_fixed(p->prec, p->pdrow, (int)p->n2, p->grow.threads);
   int lg = 0;
   while (((size_t)1 << lg) < n) ++lg;
   p->lo_bits = (lg + 1) / 2;
@@ -590,13 +598,13 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
 
     fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
     a.perm = p->perm_in;
-    HIP_TRY(launch_fs_pass(p->prec, true, colA, dir, &a, cb, p->gcol, s));
+    HIP_TRY(launch_fs_pass(p->prec, true, colA, dir, &a, cb, p->gcol, p->fixed_col, s));
     fill_pass(a, p->grow, p->n2, p->pdrow, p->twrow);
-    HIP_TRY(launch_fs_pass(p->prec, false, rowB, dir, &a, cb, p->grow, s));
+    HIP_TRY(launch_fs_pass(p->prec, false, rowB, dir, &a, cb, p->grow, p->fixed_row, s));
     if (colC >= 0) {
       fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
       a.perm = p->perm_out;
-      HIP_TRY(launch_fs_pass(p->prec, true, colC, +1, &a, cb, p->gcol, s));
+      HIP_TRY(launch_fs_pass(p->prec, true, colC, +1, &a, cb, p->gcol, p->fixed_col, s));
     }
   }
   return FYFFT_SUCCESS;

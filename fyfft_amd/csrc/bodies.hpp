@@ -13,6 +13,33 @@
 namespace fyfft {
 namespace dev {
 
+// FFT policies for the bodies: how the LDS-resident sub-transform is run.
+// GenericFft: runtime pass list (any smooth length), passes out of line.
+template <typename V, int PPT> struct GenericFft {
+  template <int D>
+  static __device__ __forceinline__ void run(V* lds, int n, int t, int tid, const PassDesc& pd,
+                                             const V* tw, bool active, int ps) {
+    lds_fft<V, PPT, D>(lds, n, t, tid, pd, tw, active, ps);
+  }
+};
+
+template <typename V, int D, int N, int T, int Ls, int R, int... Rest>
+__device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, bool active, int ps) {
+  constexpr int MAXB = cdiv(N / R, T);
+  stockham_pass<V, R, MAXB, D, false, false>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
+  if constexpr (sizeof...(Rest) > 0) fixed_lds_passes<V, D, N, T, Ls * R, Rest...>(lds, tid, tw, active, ps);
+}
+
+// FixedFft: length, threads per transform and radices known at compile time,
+// passes inlined.  The plan must use exactly this geometry and pass list.
+template <typename V, int N, int T, int... Rs> struct FixedFft {
+  template <int D>
+  static __device__ __forceinline__ void run(V* lds, int, int, int tid, const PassDesc&,
+                                             const V* tw, bool active, int ps) {
+    fixed_lds_passes<V, D, N, T, 1, Rs...>(as_lds(lds), tid, tw, active, ps);
+  }
+};
+
 // ------------------------------------------------------------ smooth
 template <typename V, int PPT, int DIR>
 __device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds_in,
@@ -88,7 +115,7 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
 // Column pass over columns [c0, c0 + C) of one transform (C = 1 << log2c).
 // x: the transform's input, tmp: its workspace, out: its output, x0: its
 // Rader x[0] slot.
-template <typename V, int PPT, int DIR, int MODE>
+template <typename V, typename F, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* tmp, V* out, V* x0,
                                             int c0, V* lds_in) {
   V* lds_all = as_lds(lds_in);
@@ -113,7 +140,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
   }
   __syncthreads();
   const bool active = slot < C && c0 + slot < N2;
-  lds_fft<V, PPT, DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, a.tw, active, ps);
+  F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, a.tw, active, ps);
   V xz = V{0, 0};
   if constexpr (MODE == kFsRaderC) xz = *x0;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
@@ -133,7 +160,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
 }
 
 // Row pass over rows [r0, r0 + C) of one transform's workspace.
-template <typename V, int PPT, int DIR, int MODE>
+template <typename V, typename F, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, const V* x0,
                                             int r0, V* lds_in) {
   V* lds_all = as_lds(lds_in);
@@ -153,7 +180,7 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
   V* lds = lds_all + (slot < C ? slot : 0) * stride;
   const int k1 = r0 + slot;
   if constexpr (MODE == kFsPlainB) {
-    lds_fft<V, PPT, DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    F::template run<DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
       if (c >= rows) continue;
@@ -161,7 +188,7 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
              cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale));
     }
   } else {  // kFsConvB
-    lds_fft<V, PPT, -1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    F::template run<-1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     if (x0 != nullptr && r0 == 0 && threadIdx.x == 0) {
       // Rader: X[0] = x0 + sum_{n>=1} x[n] = x0 + A[0] (row k1 = 0, k2 = 0)
       out[0] = cscale(cadd(*x0, lds_all[0]), a.scale);
@@ -175,7 +202,7 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
       }
     }
     __syncthreads();
-    lds_fft<V, PPT, +1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    F::template run<+1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     V* dst = tmp + (long long)r0 * N2;
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
       const int c = i / N2, n2 = i - c * N2;

@@ -85,8 +85,12 @@ template <typename V> struct FsArgs {
 // One four-step pass; `args` points at FsArgs<float2> or FsArgs<double2>.
 // g gives threads per sub-transform, sub-transforms per workgroup and LDS
 // bytes.  dir (-1/+1) only matters for the plain modes.
+// fixed_n: a length from find_fs_fixed selects the compile-time
+// sub-transform, 0 the generic one.
 hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* args, size_t batch,
-                          const LdsGeom& g, hipStream_t s);
+                          const LdsGeom& g, int fixed_n, hipStream_t s);
+// Length of a compile-time sub-transform matching (prec, passes, n, threads), or 0.
+int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads);
 
 // --- kernels_conv.hip
 template <typename V> struct ConvArgs {

@@ -24,84 +24,128 @@
 namespace fyfft {
 using namespace dev;
 
-template <typename V, int PPT, int DIR, int MODE>
+template <typename V, typename F, int DIR, int MODE>
 __global__ __launch_bounds__(1024) void k_fs_col(FsArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int groups = (a.n2 + (1 << a.log2c) - 1) >> a.log2c;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const long long t = lb / groups;
   const int c0 = (int)(lb - t * groups) << a.log2c;
-  body_fs_col<V, PPT, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
-                                 a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
-                                 reinterpret_cast<V*>(smem_raw));
+  body_fs_col<V, F, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
+                               a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
+                               reinterpret_cast<V*>(smem_raw));
 }
 
-template <typename V, int PPT, int DIR, int MODE>
+template <typename V, typename F, int DIR, int MODE>
 __global__ __launch_bounds__(1024) void k_fs_row(FsArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int groups = (a.n1 + (1 << a.log2c) - 1) >> a.log2c;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const long long t = lb / groups;
   const int r0 = (int)(lb - t * groups) << a.log2c;
-  body_fs_row<V, PPT, DIR, MODE>(a, a.tmp + t * a.tmp_dist, a.out + t * a.out_dist,
-                                 a.x0buf ? a.x0buf + t : nullptr, r0,
-                                 reinterpret_cast<V*>(smem_raw));
+  body_fs_row<V, F, DIR, MODE>(a, a.tmp + t * a.tmp_dist, a.out + t * a.out_dist,
+                               a.x0buf ? a.x0buf + t : nullptr, r0,
+                               reinterpret_cast<V*>(smem_raw));
 }
 
 // ------------------------------------------------------------ launch
-template <typename V, int PPT, int DIR, int MODE>
+template <typename V, typename F, int DIR, int MODE>
 static hipError_t go_col(const FsArgs<V>& a, size_t batch, size_t lds, int threads, hipStream_t s) {
-  auto k = k_fs_col<V, PPT, DIR, MODE>;
+  auto k = k_fs_col<V, F, DIR, MODE>;
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const size_t groups = ((size_t)a.n2 + (1 << a.log2c) - 1) >> a.log2c;
   hipLaunchKernelGGL(k, dim3((unsigned)(groups * batch)), dim3(threads), lds, s, a);
   return hipGetLastError();
 }
-template <typename V, int PPT, int DIR, int MODE>
+template <typename V, typename F, int DIR, int MODE>
 static hipError_t go_row(const FsArgs<V>& a, size_t batch, size_t lds, int threads, hipStream_t s) {
-  auto k = k_fs_row<V, PPT, DIR, MODE>;
+  auto k = k_fs_row<V, F, DIR, MODE>;
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const size_t groups = ((size_t)a.n1 + (1 << a.log2c) - 1) >> a.log2c;
   hipLaunchKernelGGL(k, dim3((unsigned)(groups * batch)), dim3(threads), lds, s, a);
   return hipGetLastError();
 }
 
-template <typename V, int PPT>
+template <typename V, typename F>
 static hipError_t dispatch_fs(bool col, int mode, int dir, const FsArgs<V>& a, size_t batch,
                               size_t lds, int threads, hipStream_t s) {
   if (col) {
     switch (mode) {
       case kFsPlainA:
-        return dir < 0 ? go_col<V, PPT, -1, kFsPlainA>(a, batch, lds, threads, s)
-                       : go_col<V, PPT, +1, kFsPlainA>(a, batch, lds, threads, s);
-      case kFsRaderA: return go_col<V, PPT, -1, kFsRaderA>(a, batch, lds, threads, s);
-      case kFsBlueA: return go_col<V, PPT, -1, kFsBlueA>(a, batch, lds, threads, s);
-      case kFsRaderC: return go_col<V, PPT, +1, kFsRaderC>(a, batch, lds, threads, s);
-      case kFsBlueC: return go_col<V, PPT, +1, kFsBlueC>(a, batch, lds, threads, s);
+        return dir < 0 ? go_col<V, F, -1, kFsPlainA>(a, batch, lds, threads, s)
+                       : go_col<V, F, +1, kFsPlainA>(a, batch, lds, threads, s);
+      case kFsRaderA: return go_col<V, F, -1, kFsRaderA>(a, batch, lds, threads, s);
+      case kFsBlueA: return go_col<V, F, -1, kFsBlueA>(a, batch, lds, threads, s);
+      case kFsRaderC: return go_col<V, F, +1, kFsRaderC>(a, batch, lds, threads, s);
+      case kFsBlueC: return go_col<V, F, +1, kFsBlueC>(a, batch, lds, threads, s);
     }
   } else {
     switch (mode) {
       case kFsPlainB:
-        return dir < 0 ? go_row<V, PPT, -1, kFsPlainB>(a, batch, lds, threads, s)
-                       : go_row<V, PPT, +1, kFsPlainB>(a, batch, lds, threads, s);
-      case kFsConvB: return go_row<V, PPT, -1, kFsConvB>(a, batch, lds, threads, s);
+        return dir < 0 ? go_row<V, F, -1, kFsPlainB>(a, batch, lds, threads, s)
+                       : go_row<V, F, +1, kFsPlainB>(a, batch, lds, threads, s);
+      case kFsConvB: return go_row<V, F, -1, kFsConvB>(a, batch, lds, threads, s);
     }
   }
   return hipErrorInvalidValue;
 }
 
+// Compile-time sub-transforms for the power-of-two factor lengths (radices
+// as planner.cpp radices_for, threads = n / points-per-thread).
+struct FsFixed {
+  int prec, n, threads, npass;
+  int radix[3];
+};
+static const FsFixed kFsFixed[] = {
+    {kF32, 64, 4, 2, {8, 8}},       {kF32, 128, 8, 2, {16, 8}},      {kF32, 256, 16, 2, {16, 16}},
+    {kF32, 512, 32, 3, {8, 8, 8}},  {kF32, 1024, 64, 3, {16, 8, 8}}, {kF32, 2048, 128, 3, {16, 16, 8}},
+    {kF32, 4096, 256, 3, {16, 16, 16}},
+    {kF64, 64, 8, 2, {8, 8}},       {kF64, 128, 16, 2, {16, 8}},     {kF64, 256, 32, 2, {16, 16}},
+    {kF64, 512, 64, 3, {8, 8, 8}},  {kF64, 1024, 128, 3, {16, 8, 8}}, {kF64, 2048, 256, 3, {16, 16, 8}},
+};
+
+int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads) {
+  for (int i = 0; i < (int)(sizeof(kFsFixed) / sizeof(kFsFixed[0])); ++i) {
+    const FsFixed& e = kFsFixed[i];
+    if (e.prec != prec || e.n != n || e.threads != threads || e.npass != pd.npass) continue;
+    bool same = true;
+    for (int p = 0; p < pd.npass; ++p) same = same && e.radix[p] == pd.radix[p];
+    if (same) return e.n;
+  }
+  return 0;
+}
+
+template <typename V, int PPT>
+static hipError_t dispatch_len(int fixed_n, bool col, int mode, int dir, const FsArgs<V>& a,
+                               size_t batch, size_t lds, int threads, hipStream_t s) {
+  constexpr int P = sizeof(V) == 8 ? 16 : 8;  // points per thread of the fixed forms
+  switch (fixed_n) {
+    case 64: return dispatch_fs<V, FixedFft<V, 64, 64 / P, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
+    case 128: return dispatch_fs<V, FixedFft<V, 128, 128 / P, 16, 8>>(col, mode, dir, a, batch, lds, threads, s);
+    case 256: return dispatch_fs<V, FixedFft<V, 256, 256 / P, 16, 16>>(col, mode, dir, a, batch, lds, threads, s);
+    case 512: return dispatch_fs<V, FixedFft<V, 512, 512 / P, 8, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
+    case 1024: return dispatch_fs<V, FixedFft<V, 1024, 1024 / P, 16, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
+    case 2048: return dispatch_fs<V, FixedFft<V, 2048, 2048 / P, 16, 16, 8>>(col, mode, dir, a, batch, lds, threads, s);
+    case 4096:
+      if constexpr (sizeof(V) == 8)
+        return dispatch_fs<V, FixedFft<V, 4096, 4096 / P, 16, 16, 16>>(col, mode, dir, a, batch, lds, threads, s);
+      break;
+  }
+  return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);
+}
+
 hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* args, size_t batch,
-                          const LdsGeom& g, hipStream_t s) {
+                          const LdsGeom& g, int fixed_n, hipStream_t s) {
   if (batch == 0) return hipSuccess;
   const int threads = g.threads * g.per_block;
   if (prec == kF64) {
     const auto& a = *static_cast<const FsArgs<double2>*>(args);
-    return g.ppt <= 8 ? dispatch_fs<double2, 8>(col, mode, dir, a, batch, g.lds_bytes, threads, s)
-                      : dispatch_fs<double2, 16>(col, mode, dir, a, batch, g.lds_bytes, threads, s);
+    return g.ppt <= 8 ? dispatch_len<double2, 8>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s)
+                      : dispatch_len<double2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s);
   }
   const auto& a = *static_cast<const FsArgs<float2>*>(args);
-  return g.ppt <= 16 ? dispatch_fs<float2, 16>(col, mode, dir, a, batch, g.lds_bytes, threads, s)
-                     : dispatch_fs<float2, 32>(col, mode, dir, a, batch, g.lds_bytes, threads, s);
+  return g.ppt <= 16 ? dispatch_len<float2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s)
+                     : dispatch_len<float2, 32>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s);
 }
 
 }  // namespace fyfft

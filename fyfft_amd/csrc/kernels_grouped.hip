@@ -62,9 +62,9 @@ __device__ __noinline__ void task_fs(const GLen<V>& g, const GTask& t, const Tas
   const long long o = c.off[t.first], w = c.woff[t.first];
   V* x0 = c.x0buf + t.first;
   if constexpr (col)
-    body_fs_col<V, PPT, DIR, MODE>(a, c.in + o, c.ws + w, c.out + o, x0, t.count, c.lds);
+    body_fs_col<V, GenericFft<V, PPT>, DIR, MODE>(a, c.in + o, c.ws + w, c.out + o, x0, t.count, c.lds);
   else
-    body_fs_row<V, PPT, DIR, MODE>(a, c.ws + w, c.out + o, g.kind == kGkFsRader ? x0 : nullptr,
+    body_fs_row<V, GenericFft<V, PPT>, DIR, MODE>(a, c.ws + w, c.out + o, g.kind == kGkFsRader ? x0 : nullptr,
                                    t.count, c.lds);
 }
 
