@@ -76,10 +76,11 @@ struct fyfft_plan {
   std::vector<size_t> h_off;                 // transform -> packed element offset
   void *d_tasks = nullptr, *d_lens = nullptr, *d_off = nullptr, *d_woff = nullptr;
   void *gws = nullptr, *gx0 = nullptr;
-  int task_base[3] = {0, 0, 0}, task_count[3] = {0, 0, 0};
-  int task_xbase[3][dev::kXcds + 1] = {};
-  size_t phase_lds[3] = {0, 0, 0};
-  unsigned phase_grid[3] = {0, 0, 0};
+  // per (phase, workgroup-size class): task range, XCD split, LDS, grid
+  int task_base[3][kGroupClasses] = {}, task_count[3][kGroupClasses] = {};
+  int task_xbase[3][kGroupClasses][dev::kXcds + 1] = {};
+  size_t phase_lds[3][kGroupClasses] = {};
+  unsigned phase_grid[3][kGroupClasses] = {};
   mutable std::mutex graph_mu;
   mutable std::map<std::tuple<const void*, void*, int, hipStream_t>, hipGraphExec_t> graphs;
   std::vector<void*> allocs;
@@ -759,11 +760,41 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
 
 
 // ------------------------------------------------------------ grouped plans
+// Workgroup-size class of a grouped task: the smallest class whose thread
+// count and LDS budget (cap * threads / 1024) hold it.
+static size_t class_lds_cap(size_t cap, int k) { return cap * (size_t)kGroupClassThreads[k] / 1024; }
+
+// Class and column count of a four-step sub-pass: 8 to 16 sub-transforms
+// per workgroup keep row segments at 64 bytes or more; below that, the
+// largest class takes whatever fits.
+static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t cap, int* cls, int* log2c) {
+  const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem;
+  for (int k = 0; k < kGroupClasses; ++k) {
+    int c = 16;
+    while (c > 1 && ((size_t)c * q.threads > (size_t)kGroupClassThreads[k] || c * per > class_lds_cap(cap, k))) c >>= 1;
+    if ((size_t)c * q.threads > (size_t)kGroupClassThreads[k] || c * per > class_lds_cap(cap, k)) continue;
+    if (c >= 8 || k == kGroupClasses - 1) {
+      *cls = k;
+      *log2c = ilog2_exact(c);
+      return;
+    }
+  }
+  *cls = kGroupClasses - 1;
+  *log2c = 0;
+}
+
 template <typename V>
-static GLen<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, int* per_block) {
+struct GLenPlan {
   GLen<V> g;
+  int per_block = 1;                   // LDS kinds: transforms per task
+  int lds_cls = 0, col_cls = 0, row_cls = 0;
+};
+
+template <typename V>
+static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap) {
+  GLenPlan<V> r;
+  GLen<V>& g = r.g;
   g.n = (int)s->n;
-  *per_block = 1;
   auto fill_lds = [&](size_t m) {
     const LdsGeom& q = s->lds.geom;
     g.lds.n = (int)s->n;
@@ -775,13 +806,18 @@ static GLen<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, int*
     g.lds.tw = (const V*)s->lds.tw;
     g.lds_ppt = q.ppt;
     const size_t per = (size_t)g.lds.stride * sizeof(V);
-    size_t pb = std::min<size_t>(1024 / q.threads, cap / per);
-    *per_block = (int)std::max<size_t>(1, std::min(pb, count_len));
+    int k = 0;
+    while (k < kGroupClasses - 1 &&
+           (q.threads > kGroupClassThreads[k] || per > class_lds_cap(cap, k))) ++k;
+    const size_t pb = std::min<size_t>(kGroupClassThreads[k] / q.threads, class_lds_cap(cap, k) / per);
+    r.lds_cls = k;
+    r.per_block = (int)std::max<size_t>(1, std::min(pb, count_len));
   };
-  auto fill_fs = [&](FsArgs<V>& a, const LdsGeom& q, size_t len, const dev::PassDesc& pd, const void* tw) {
+  auto fill_fs = [&](FsArgs<V>& a, const LdsGeom& q, size_t len, const dev::PassDesc& pd, const void* tw,
+                     int* cls) {
     a.n1 = (int)s->n1;
     a.n2 = (int)s->n2;
-    a.log2c = ilog2_exact(q.per_block);
+    grouped_fs_class(q, len, sizeof(V), cap, cls, &a.log2c);
     a.threads = q.threads;
     a.ps = q.pad_shift;
     a.stride = dev::lds_len((int)len, q.pad_shift);
@@ -809,20 +845,21 @@ static GLen<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, int*
       g.lds.ps = 30;
       g.lds.stride = 1;
       g.lds.pd.npass = 0;
-      *per_block = (int)std::max<size_t>(1, std::min<size_t>(1024, count_len));
+      r.lds_cls = 0;
+      r.per_block = (int)std::max<size_t>(1, std::min<size_t>(kGroupClassThreads[0], count_len));
       break;
     case kLds: g.kind = kGkLds; fill_lds(s->n); break;
     case kRaderLds: g.kind = kGkRader; fill_lds(s->m); break;
     case kBlueLds: g.kind = kGkBlue; fill_lds(s->m); break;
     default:
       g.kind = s->kind == kFourStep ? kGkFsPlain : (s->kind == kRaderFs ? kGkFsRader : kGkFsBlue);
-      fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol);
-      fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow);
+      fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol, &r.col_cls);
+      fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow, &r.row_cls);
       g.col_ppt = s->gcol.ppt;
       g.row_ppt = s->grow.ppt;
       break;
   }
-  return g;
+  return r;
 }
 
 template <typename V>
@@ -833,14 +870,14 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
   std::vector<GLen<V>> lens;
   std::vector<long long> off, woff;
   struct T { GTask t; double cost; };
-  std::vector<T> phase[3];
+  std::vector<T> phase[3][kGroupClasses];
   long long wsize = 0;
   int li = 0;
   for (const auto& kv : by_len) {
     const fyfft_plan* s = p->subs[li];
     const std::vector<size_t>& idx = kv.second;
-    int per_block = 1;
-    lens.push_back(make_glen<V>(s, idx.size(), cap, &per_block));
+    const GLenPlan<V> lp = make_glen<V>(s, idx.size(), cap);
+    lens.push_back(lp.g);
     const GLen<V>& g = lens.back();
     const int first = (int)off.size();
     for (size_t i : idx) {
@@ -853,68 +890,78 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
     if (g.kind <= kGkBlue) {
       const size_t m = (size_t)g.lds.m;
       const double conv = g.kind == kGkLds ? 1.0 : 2.0;
-      for (size_t i = 0; i < idx.size(); i += per_block) {
-        const int c = (int)std::min<size_t>(per_block, idx.size() - i);
-        phase[0].push_back({GTask{li, 0, first + (int)i, c}, conv * c * m * lg});
-        p->phase_lds[0] = std::max(p->phase_lds[0], (size_t)per_block * g.lds.stride * elem);
+      const int k = lp.lds_cls, pb = lp.per_block;
+      for (size_t i = 0; i < idx.size(); i += pb) {
+        const int c = (int)std::min<size_t>(pb, idx.size() - i);
+        phase[0][k].push_back({GTask{li, 0, first + (int)i, c}, conv * c * m * lg});
       }
+      p->phase_lds[0][k] = std::max(p->phase_lds[0][k], (size_t)pb * g.lds.stride * elem);
     } else {
       const int C = 1 << g.col.log2c, R = 1 << g.row.log2c;
       const double conv = g.kind == kGkFsPlain ? 1.0 : 2.0;
+      const int kc = lp.col_cls, kr = lp.row_cls;
       for (size_t i = 0; i < idx.size(); ++i) {
         for (int c0 = 0; c0 < g.col.n2; c0 += C) {
-          phase[0].push_back({GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg});
-          if (g.kind != kGkFsPlain) phase[2].push_back({GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg});
+          phase[0][kc].push_back({GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg});
+          if (g.kind != kGkFsPlain)
+            phase[2][kc].push_back({GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg});
         }
         for (int r0 = 0; r0 < g.col.n1; r0 += R)
-          phase[1].push_back({GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg});
+          phase[1][kr].push_back({GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg});
       }
-      p->phase_lds[0] = std::max(p->phase_lds[0], s->gcol.lds_bytes);
-      p->phase_lds[1] = std::max(p->phase_lds[1], s->grow.lds_bytes);
-      if (g.kind != kGkFsPlain) p->phase_lds[2] = std::max(p->phase_lds[2], s->gcol.lds_bytes);
+      const size_t col_lds = (size_t)C * g.col.stride * elem, row_lds = (size_t)R * g.row.stride * elem;
+      p->phase_lds[0][kc] = std::max(p->phase_lds[0][kc], col_lds);
+      p->phase_lds[1][kr] = std::max(p->phase_lds[1][kr], row_lds);
+      if (g.kind != kGkFsPlain) p->phase_lds[2][kc] = std::max(p->phase_lds[2][kc], col_lds);
     }
     ++li;
   }
-  // Bin-pack each phase onto the XCDs: all tasks of one transform (four-step
-  // groups) stay together, greedy by decreasing group cost onto the least
-  // loaded XCD; within an XCD, tasks by decreasing cost.
+  // Bin-pack each (phase, class) list onto the XCDs: all tasks of one
+  // transform (four-step groups) stay together, greedy by decreasing group
+  // cost onto the least loaded XCD; within an XCD, tasks by decreasing cost.
   std::vector<GTask> tasks;
-  for (int ph = 0; ph < 3; ++ph) {
-    std::map<std::pair<int, int>, std::vector<T>> groups;  // (len, transform or task) -> tasks
-    for (size_t i = 0; i < phase[ph].size(); ++i) {
-      const T& t = phase[ph][i];
-      const bool fs = lens[t.t.len].kind >= kGkFsPlain;
-      groups[{t.t.len, fs ? t.t.first : -1 - (int)i}].push_back(t);
+  size_t class_lds[kGroupClasses] = {};
+  for (int ph = 0; ph < 3; ++ph)
+    for (int k = 0; k < kGroupClasses; ++k) {
+      const std::vector<T>& list = phase[ph][k];
+      std::map<std::pair<int, int>, std::vector<T>> groups;  // (len, transform or task) -> tasks
+      for (size_t i = 0; i < list.size(); ++i) {
+        const T& t = list[i];
+        const bool fs = lens[t.t.len].kind >= kGkFsPlain;
+        groups[{t.t.len, fs ? t.t.first : -1 - (int)i}].push_back(t);
+      }
+      std::vector<std::pair<double, std::vector<T>*>> order;
+      for (auto& kv : groups) {
+        double c = 0;
+        for (const T& t : kv.second) c += t.cost;
+        order.push_back({c, &kv.second});
+      }
+      std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+      std::vector<T> per_x[dev::kXcds];
+      double load[dev::kXcds] = {0};
+      for (auto& g : order) {
+        const int x = (int)(std::min_element(load, load + dev::kXcds) - load);
+        load[x] += g.first;
+        per_x[x].insert(per_x[x].end(), g.second->begin(), g.second->end());
+      }
+      p->task_base[ph][k] = (int)tasks.size();
+      p->task_count[ph][k] = (int)list.size();
+      for (int x = 0; x < dev::kXcds; ++x) {
+        std::stable_sort(per_x[x].begin(), per_x[x].end(), [](const T& a, const T& b) { return a.cost > b.cost; });
+        p->task_xbase[ph][k][x] = (int)tasks.size() - p->task_base[ph][k];
+        for (const T& t : per_x[x]) tasks.push_back(t.t);
+      }
+      p->task_xbase[ph][k][dev::kXcds] = (int)tasks.size() - p->task_base[ph][k];
+      p->phase_lds[ph][k] = std::max<size_t>(p->phase_lds[ph][k], 16);
+      class_lds[k] = std::max(class_lds[k], p->phase_lds[ph][k]);
+      // resident workgroups per CU: 1024 / class threads (LDS budget per
+      // class keeps them within the CU); the grid is a multiple of the
+      // XCD count
+      const size_t per_cu = 1024 / (size_t)kGroupClassThreads[k];
+      size_t grid = std::min<size_t>(list.size(), (size_t)ncu * per_cu);
+      grid = std::max<size_t>(dev::kXcds, (grid + dev::kXcds - 1) / dev::kXcds * dev::kXcds);
+      p->phase_grid[ph][k] = (unsigned)grid;
     }
-    std::vector<std::pair<double, std::vector<T>*>> order;
-    for (auto& kv : groups) {
-      double c = 0;
-      for (const T& t : kv.second) c += t.cost;
-      order.push_back({c, &kv.second});
-    }
-    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-    std::vector<T> per_x[dev::kXcds];
-    double load[dev::kXcds] = {0};
-    for (auto& g : order) {
-      const int x = (int)(std::min_element(load, load + dev::kXcds) - load);
-      load[x] += g.first;
-      per_x[x].insert(per_x[x].end(), g.second->begin(), g.second->end());
-    }
-    p->task_base[ph] = (int)tasks.size();
-    p->task_count[ph] = (int)phase[ph].size();
-    for (int x = 0; x < dev::kXcds; ++x) {
-      std::stable_sort(per_x[x].begin(), per_x[x].end(), [](const T& a, const T& b) { return a.cost > b.cost; });
-      p->task_xbase[ph][x] = (int)tasks.size() - p->task_base[ph];
-      for (const T& t : per_x[x]) tasks.push_back(t.t);
-    }
-    p->task_xbase[ph][dev::kXcds] = (int)tasks.size() - p->task_base[ph];
-    const size_t lds = std::max<size_t>(p->phase_lds[ph], 16);
-    const unsigned per_cu = (unsigned)std::max<size_t>(1, std::min<size_t>(2, cap / lds));
-    // a multiple of the XCD count, at most the resident capacity
-    size_t grid = std::min<size_t>(phase[ph].size(), (size_t)ncu * per_cu);
-    grid = std::max<size_t>(dev::kXcds, (grid + dev::kXcds - 1) / dev::kXcds * dev::kXcds);
-    p->phase_grid[ph] = (unsigned)grid;
-  }
   fyfft_status st;
   if ((st = p->alloc(std::max<size_t>(1, tasks.size()) * sizeof(GTask), &p->d_tasks)) ||
       (st = p->alloc(lens.size() * sizeof(GLen<V>), &p->d_lens)) ||
@@ -927,10 +974,14 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
   HIP_TRY(hipMemcpy(p->d_lens, lens.data(), lens.size() * sizeof(GLen<V>), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_off, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_woff, woff.data(), woff.size() * sizeof(long long), hipMemcpyHostToDevice));
-  HIP_TRY(prepare_grouped(p->prec, std::max({p->phase_lds[0], p->phase_lds[1], p->phase_lds[2]})));
+  HIP_TRY(prepare_grouped(p->prec, class_lds));
+  auto counts = [&](int ph) {
+    int n = 0;
+    for (int k = 0; k < kGroupClasses; ++k) n += p->task_count[ph][k];
+    return std::to_string(n);
+  };
   p->desc = "grouped count=" + std::to_string(p->count) + " lengths=" + std::to_string(lens.size()) +
-            " total=" + std::to_string(p->total) + " tasks=" + std::to_string(p->task_count[0]) + "/" +
-            std::to_string(p->task_count[1]) + "/" + std::to_string(p->task_count[2]) +
+            " total=" + std::to_string(p->total) + " tasks=" + counts(0) + "/" + counts(1) + "/" + counts(2) +
             " workspace=" + std::to_string(wsize);
   return FYFFT_SUCCESS;
 }
@@ -949,12 +1000,14 @@ static fyfft_status launch_grouped_phases(const fyfft_plan* p, const void* in, v
   a.tables = direction != FYFFT_FORWARD;
   a.normalize = direction == FYFFT_INVERSE;
   const int dir = direction == FYFFT_FORWARD ? -1 : +1;
-  for (int ph = 0; ph < 3; ++ph) {
-    if (!p->task_count[ph]) continue;
-    a.tasks = (const GTask*)p->d_tasks + p->task_base[ph];
-    for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = p->task_xbase[ph][x];
-    HIP_TRY(launch_grouped(p->prec, dir, &a, p->phase_grid[ph], std::max<size_t>(p->phase_lds[ph], 16), s));
-  }
+  for (int ph = 0; ph < 3; ++ph)
+    for (int k = kGroupClasses - 1; k >= 0; --k) {
+      if (!p->task_count[ph][k]) continue;
+      a.tasks = (const GTask*)p->d_tasks + p->task_base[ph][k];
+      for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = p->task_xbase[ph][k][x];
+      HIP_TRY(launch_grouped(p->prec, dir, &a, p->phase_grid[ph][k], kGroupClassThreads[k],
+                             p->phase_lds[ph][k], s));
+    }
   return FYFFT_SUCCESS;
 }
 

@@ -13,9 +13,33 @@
 namespace fyfft {
 namespace dev {
 
+// Global <-> LDS transfers issue up to 16 loads per thread before the first
+// dependent store: element i = start + j * step for j < P, guarded by
+// i < total.  P must bound the per-thread element count.
+template <int P, typename LoadF, typename StoreF>
+__device__ __forceinline__ void staged(int start, int step, int total, LoadF ld, StoreF st) {
+  using T = decltype(ld(0));
+  constexpr int K = P < 16 ? P : 16;
+#pragma unroll
+  for (int j0 = 0; j0 < P; j0 += K) {
+    T buf[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int i = start + (j0 + j) * step;
+      if (i < total) buf[j] = ld(i);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int i = start + (j0 + j) * step;
+      if (i < total) st(i, buf[j]);
+    }
+  }
+}
+
 // FFT policies for the bodies: how the LDS-resident sub-transform is run.
 // GenericFft: runtime pass list (any smooth length), passes out of line.
 template <typename V, int PPT> struct GenericFft {
+  static constexpr int kPpt = PPT;
   template <int D>
   static __device__ __forceinline__ void run(V* lds, int n, int t, int tid, const PassDesc& pd,
                                              const V* tw, bool active, int ps) {
@@ -33,6 +57,7 @@ __device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, b
 // FixedFft: length, threads per transform and radices known at compile time,
 // passes inlined.  The plan must use exactly this geometry and pass list.
 template <typename V, int N, int T, int... Rs> struct FixedFft {
+  static constexpr int kPpt = N / T;
   template <int D>
   static __device__ __forceinline__ void run(V* lds, int, int, int tid, const PassDesc&,
                                              const V* tw, bool active, int ps) {
@@ -47,7 +72,8 @@ __device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x,
   V* lds = as_lds(lds_in);
   const int N = a.m, T = a.threads, ps = a.ps;
   if (active)
-    for (int i = tid; i < N; i += T) lds[i + (i >> ps)] = gload(x + i);
+    staged<PPT>(tid, T, N, [&](int i) { return gload(x + i); },
+                   [&](int i, V v) { lds[i + (i >> ps)] = v; });
   __syncthreads();
   lds_fft<V, PPT, DIR>(lds, N, T, tid, a.pd, a.tw, active, ps);
   if (active)
@@ -64,21 +90,24 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
   V x0 = V{0, 0};
   if (active) {
     x0 = x[0];
-    for (int j = tid; j < L; j += T) lds[j + (j >> ps)] = x[a.perm_in[j]];
+    staged<PPT>(tid, T, L, [&](int j) { return x[a.perm_in[j]]; },
+                   [&](int j, V v) { lds[j + (j >> ps)] = v; });
   }
   __syncthreads();
   lds_fft<V, PPT, -1>(lds, L, T, tid, a.pd, a.tw, active, ps);
   const V sum_rest = lds[0];
   __syncthreads();
   if (active)
-    for (int j = tid; j < L; j += T) {
-      const int li = j + (j >> ps);
-      lds[li] = cmul(lds[li], a.kern[j]);
-    }
+    staged<PPT>(tid, T, L, [&](int j) { return a.kern[j]; },
+                   [&](int j, V k) {
+                     const int li = j + (j >> ps);
+                     lds[li] = cmul(lds[li], k);
+                   });
   __syncthreads();
   lds_fft<V, PPT, +1>(lds, L, T, tid, a.pd, a.tw, active, ps);
   if (active) {
-    for (int i = tid; i < L; i += T) y[a.perm_out[i]] = cscale(cadd(lds[i + (i >> ps)], x0), a.scale);
+    staged<PPT>(tid, T, L, [&](int i) { return a.perm_out[i]; },
+                   [&](int i, int d) { y[d] = cscale(cadd(lds[i + (i >> ps)], x0), a.scale); });
     if (tid == 0) y[0] = cscale(cadd(x0, sum_rest), a.scale);
   }
 }
@@ -91,18 +120,21 @@ __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y,
   V* lds = as_lds(lds_in);
   const int T = a.threads, M = a.m, N = a.n, ps = a.ps;
   if (active)
-    for (int n = tid; n < M; n += T) lds[n + (n >> ps)] = n < N ? cmul(x[n], a.chirp[n]) : V{0, 0};
+    staged<PPT>(tid, T, M, [&](int n) { return n < N ? cmul(x[n], a.chirp[n]) : V{0, 0}; },
+                   [&](int n, V v) { lds[n + (n >> ps)] = v; });
   __syncthreads();
   lds_fft<V, PPT, -1>(lds, M, T, tid, a.pd, a.tw, active, ps);
   if (active)
-    for (int j = tid; j < M; j += T) {
-      const int li = j + (j >> ps);
-      lds[li] = cmul(lds[li], a.kern[j]);
-    }
+    staged<PPT>(tid, T, M, [&](int j) { return a.kern[j]; },
+                   [&](int j, V k) {
+                     const int li = j + (j >> ps);
+                     lds[li] = cmul(lds[li], k);
+                   });
   __syncthreads();
   lds_fft<V, PPT, +1>(lds, M, T, tid, a.pd, a.tw, active, ps);
   if (active)
-    for (int k = tid; k < N; k += T) y[k] = cscale(cmul(lds[k + (k >> ps)], a.chirp[k]), a.scale);
+    staged<PPT>(tid, T, N, [&](int k) { return a.chirp[k]; },
+                   [&](int k, V c) { y[k] = cscale(cmul(lds[k + (k >> ps)], c), a.scale); });
 }
 
 // ------------------------------------------------------------ four-step
@@ -114,48 +146,76 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
 
 // Column pass over columns [c0, c0 + C) of one transform (C = 1 << log2c).
 // x: the transform's input, tmp: its workspace, out: its output, x0: its
-// Rader x[0] slot.
+// Rader x[0] slot.  Element i of the workgroup's N1 x C tile is row
+// n1 = i >> log2c, column c = i & (C - 1), so a wavefront reads C-element
+// row segments.
 template <typename V, typename F, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* tmp, V* out, V* x0,
                                             int c0, V* lds_in) {
+  constexpr int P = F::kPpt;
   V* lds_all = as_lds(lds_in);
-  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
+  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c, lc = a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
-  const int total = N1 << a.log2c;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int n1 = i >> a.log2c, c = i & (C - 1), n2 = c0 + c;
-    if (n2 >= N2) continue;
-    const int n = n1 * N2 + n2;
-    V v;
-    if constexpr (MODE == kFsPlainA) v = gload(x + n);
-    else if constexpr (MODE == kFsRaderA) v = x[a.perm[n]];
-    else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
-    else v = tmp[n];
-    lds_all[c * stride + n1 + (n1 >> ps)] = v;
-  }
+  const int total = N1 << lc;
+  const int ncols = min(C, N2 - c0);
+  staged<P>(threadIdx.x, blockDim.x, total,
+               [&](int i) {
+                 const int n1 = i >> lc, c = i & (C - 1);
+                 const int n = n1 * N2 + c0 + (c < ncols ? c : 0);
+                 V v;
+                 if constexpr (MODE == kFsPlainA) v = gload(x + n);
+                 else if constexpr (MODE == kFsRaderA) v = x[a.perm[n]];
+                 else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
+                 else v = tmp[n];
+                 return v;
+               },
+               [&](int i, V v) {
+                 const int n1 = i >> lc, c = i & (C - 1);
+                 if (c < ncols) lds_all[c * stride + n1 + (n1 >> ps)] = v;
+               });
   if constexpr (MODE == kFsRaderA) {
     if (c0 == 0 && threadIdx.x == 0) *x0 = x[0];
   }
   __syncthreads();
-  const bool active = slot < C && c0 + slot < N2;
+  const bool active = slot < C && slot < ncols;
   F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, a.tw, active, ps);
   V xz = V{0, 0};
   if constexpr (MODE == kFsRaderC) xz = *x0;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int k1 = i >> a.log2c, c = i & (C - 1), n2 = c0 + c;
-    if (n2 >= N2) continue;
-    const V v = lds_all[c * stride + k1 + (k1 >> ps)];
-    if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
-      tmp[(long long)k1 * N2 + n2] = cmul(v, big_twiddle<DIR>(a, (long long)n2 * k1));
-    } else if constexpr (MODE == kFsRaderC) {
-      const int n = k1 * N2 + n2;  // k1 runs over n1 of the convolution output here
-      out[a.perm[n]] = cscale(cadd(v, xz), a.scale);
-    } else {  // kFsBlueC
-      const int n = k1 * N2 + n2;
-      if (n < a.n_orig) out[n] = cscale(cmul(v, a.chirp[n]), a.scale);
-    }
+  if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
+    staged<P>(threadIdx.x, blockDim.x, total,
+                 [&](int i) {
+                   const int k1 = i >> lc, c = i & (C - 1), n2 = c0 + c;
+                   return big_twiddle<DIR>(a, (long long)(c < ncols ? n2 : c0) * k1);
+                 },
+                 [&](int i, V w) {
+                   const int k1 = i >> lc, c = i & (C - 1);
+                   if (c < ncols) tmp[(long long)k1 * N2 + c0 + c] = cmul(lds_all[c * stride + k1 + (k1 >> ps)], w);
+                 });
+  } else if constexpr (MODE == kFsRaderC) {
+    // k1 runs over n1 of the convolution output here
+    staged<P>(threadIdx.x, blockDim.x, total,
+                 [&](int i) {
+                   const int k1 = i >> lc, c = i & (C - 1);
+                   return a.perm[k1 * N2 + c0 + (c < ncols ? c : 0)];
+                 },
+                 [&](int i, int d) {
+                   const int k1 = i >> lc, c = i & (C - 1);
+                   if (c < ncols) out[d] = cscale(cadd(lds_all[c * stride + k1 + (k1 >> ps)], xz), a.scale);
+                 });
+  } else {  // kFsBlueC
+    staged<P>(threadIdx.x, blockDim.x, total,
+                 [&](int i) {
+                   const int k1 = i >> lc, c = i & (C - 1);
+                   const int n = k1 * N2 + c0 + (c < ncols ? c : 0);
+                   return n < a.n_orig ? a.chirp[n] : V{0, 0};
+                 },
+                 [&](int i, V ch) {
+                   const int k1 = i >> lc, c = i & (C - 1);
+                   const int n = k1 * N2 + c0 + c;
+                   if (c < ncols && n < a.n_orig) out[n] = cscale(cmul(lds_all[c * stride + k1 + (k1 >> ps)], ch), a.scale);
+                 });
   }
 }
 
@@ -163,6 +223,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
 template <typename V, typename F, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, const V* x0,
                                             int r0, V* lds_in) {
+  constexpr int P = F::kPpt;
   V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
@@ -171,10 +232,11 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
   const int rows = min(C, N1 - r0);
   const int total = rows * N2;
   const V* src = tmp + (long long)r0 * N2;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int c = i / N2, n2 = i - c * N2;
-    lds_all[c * stride + n2 + (n2 >> ps)] = src[i];
-  }
+  staged<P>(threadIdx.x, blockDim.x, total, [&](int i) { return src[i]; },
+               [&](int i, V v) {
+                 const int c = i / N2, n2 = i - c * N2;
+                 lds_all[c * stride + n2 + (n2 >> ps)] = v;
+               });
   __syncthreads();
   const bool active = slot < rows;
   V* lds = lds_all + (slot < C ? slot : 0) * stride;
@@ -196,19 +258,24 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
     __syncthreads();
     if (active) {
       const V* kr = a.kern + (long long)k1 * N2;
-      for (int k2 = tid; k2 < N2; k2 += a.threads) {
-        const int li = k2 + (k2 >> ps);
-        lds[li] = cmul(lds[li], kr[k2]);
-      }
+      staged<P>(tid, a.threads, N2, [&](int k2) { return kr[k2]; },
+                   [&](int k2, V k) {
+                     const int li = k2 + (k2 >> ps);
+                     lds[li] = cmul(lds[li], k);
+                   });
     }
     __syncthreads();
     F::template run<+1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     V* dst = tmp + (long long)r0 * N2;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-      const int c = i / N2, n2 = i - c * N2;
-      const V v = lds_all[c * stride + n2 + (n2 >> ps)];
-      dst[i] = cmul(v, big_twiddle<+1>(a, (long long)n2 * (r0 + c)));
-    }
+    staged<P>(threadIdx.x, blockDim.x, total,
+                 [&](int i) {
+                   const int c = i / N2, n2 = i - c * N2;
+                   return big_twiddle<+1>(a, (long long)n2 * (r0 + c));
+                 },
+                 [&](int i, V w) {
+                   const int c = i / N2, n2 = i - c * N2;
+                   dst[i] = cmul(lds_all[c * stride + n2 + (n2 >> ps)], w);
+                 });
   }
 }
 

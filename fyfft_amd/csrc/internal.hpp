@@ -182,9 +182,14 @@ template <typename V> struct GroupedArgs {
   int tables = 0;     // 0 forward, 1 inverse/adjoint tables
   int normalize = 0;  // scale by 1/n
 };
-hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, size_t lds, hipStream_t s);
-// Sets the dynamic-LDS limit of the grouped kernels (call at plan time).
-hipError_t prepare_grouped(int prec, size_t lds);
+// Grouped launches come in workgroup-size classes of 256, 512 and 1024
+// threads with LDS budgets of 1/4, 1/2 and all of a CU's LDS.
+constexpr int kGroupClasses = 3;
+constexpr int kGroupClassThreads[kGroupClasses] = {256, 512, 1024};
+hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, int block, size_t lds,
+                          hipStream_t s);
+// Sets the dynamic-LDS limits of the grouped kernels per class (call at plan time).
+hipError_t prepare_grouped(int prec, const size_t lds[kGroupClasses]);
 
 // --- kernels_mfma.hip
 // BF16/FP16-storage transforms with radix-16 stages on MFMA: N = R0 * 16^P

@@ -88,8 +88,11 @@ __device__ __forceinline__ void run_task(const GLen<V>& g, const GTask& t, const
   }
 }
 
-template <typename V, int DIR, int PLO, int PHI>
-__global__ __launch_bounds__(1024) void k_grouped(GroupedArgs<V> ga) {
+// B threads per workgroup; at least four waves per SIMD so that several
+// workgroups share a CU and one task's memory phase overlaps another's
+// arithmetic.
+template <typename V, int DIR, int PLO, int PHI, int B>
+__global__ __launch_bounds__(B, 4) void k_grouped(GroupedArgs<V> ga) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   TaskCtx<V> c{ga.off, ga.woff, ga.in, ga.out, ga.ws, ga.x0buf,
                     reinterpret_cast<V*>(smem_raw), ga.tables, ga.normalize};
@@ -105,31 +108,60 @@ __global__ __launch_bounds__(1024) void k_grouped(GroupedArgs<V> ga) {
   }
 }
 
+template <typename V, int PLO, int PHI, int B>
+static hipError_t set_grouped_lds(size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute((const void*)k_grouped<V, -1, PLO, PHI, B>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_grouped<V, +1, PLO, PHI, B>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return e;
+}
+
 template <typename V, int PLO, int PHI>
-static hipError_t go_grouped(int dir, const GroupedArgs<V>& a, unsigned grid, size_t lds, hipStream_t s) {
+static hipError_t go_grouped(int dir, const GroupedArgs<V>& a, unsigned grid, int block, size_t lds,
+                             hipStream_t s) {
   // the LDS attribute is set once at plan time (prepare_grouped), so this
   // path is free of non-stream calls and can be captured into a graph
-  auto k = dir < 0 ? k_grouped<V, -1, PLO, PHI> : k_grouped<V, +1, PLO, PHI>;
-  hipLaunchKernelGGL(k, dim3(grid), dim3(1024), lds, s, a);
+  auto pick = [&](auto k_fwd, auto k_inv) { return dir < 0 ? k_fwd : k_inv; };
+  switch (block) {
+    case 256:
+      hipLaunchKernelGGL(pick(k_grouped<V, -1, PLO, PHI, 256>, k_grouped<V, +1, PLO, PHI, 256>),
+                         dim3(grid), dim3(256), lds, s, a);
+      break;
+    case 512:
+      hipLaunchKernelGGL(pick(k_grouped<V, -1, PLO, PHI, 512>, k_grouped<V, +1, PLO, PHI, 512>),
+                         dim3(grid), dim3(512), lds, s, a);
+      break;
+    case 1024:
+      hipLaunchKernelGGL(pick(k_grouped<V, -1, PLO, PHI, 1024>, k_grouped<V, +1, PLO, PHI, 1024>),
+                         dim3(grid), dim3(1024), lds, s, a);
+      break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-hipError_t prepare_grouped(int prec, size_t lds) {
-  if (lds <= 65536) return hipSuccess;
+hipError_t prepare_grouped(int prec, const size_t lds[kGroupClasses]) {
   hipError_t e;
   if (prec == kF64) {
-    e = hipFuncSetAttribute((const void*)k_grouped<double2, -1, 8, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_grouped<double2, +1, 8, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    e = set_grouped_lds<double2, 8, 16, 256>(lds[0]);
+    if (e == hipSuccess) e = set_grouped_lds<double2, 8, 16, 512>(lds[1]);
+    if (e == hipSuccess) e = set_grouped_lds<double2, 8, 16, 1024>(lds[2]);
   } else {
-    e = hipFuncSetAttribute((const void*)k_grouped<float2, -1, 16, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e == hipSuccess) e = hipFuncSetAttribute((const void*)k_grouped<float2, +1, 16, 32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    e = set_grouped_lds<float2, 16, 32, 256>(lds[0]);
+    if (e == hipSuccess) e = set_grouped_lds<float2, 16, 32, 512>(lds[1]);
+    if (e == hipSuccess) e = set_grouped_lds<float2, 16, 32, 1024>(lds[2]);
   }
   return e;
 }
 
-hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, size_t lds, hipStream_t s) {
-  if (prec == kF64) return go_grouped<double2, 8, 16>(dir, *static_cast<const GroupedArgs<double2>*>(args), grid, lds, s);
-  return go_grouped<float2, 16, 32>(dir, *static_cast<const GroupedArgs<float2>*>(args), grid, lds, s);
+hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, int block, size_t lds,
+                          hipStream_t s) {
+  if (prec == kF64)
+    return go_grouped<double2, 8, 16>(dir, *static_cast<const GroupedArgs<double2>*>(args), grid, block, lds, s);
+  return go_grouped<float2, 16, 32>(dir, *static_cast<const GroupedArgs<float2>*>(args), grid, block, lds, s);
 }
 
 }  // namespace fyfft
