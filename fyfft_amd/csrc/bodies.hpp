@@ -65,19 +65,77 @@ template <typename V, int N, int T, int... Rs> struct FixedFft {
   }
 };
 
+// ------------------------------------------------------------ functors
+template <typename V> using gbl_ptr = __attribute__((address_space(1))) V*;
+
+template <typename V> struct GlobalSrc {  // x[i]
+  static constexpr bool kFromLds = false;
+  gbl_cptr<V> x;
+  __device__ V operator()(int i) const { return gload((const V*)x + i); }
+};
+template <typename V> struct GlobalSink {  // y[i] = v * scale
+  static constexpr bool kToLds = false;
+  gbl_ptr<V> y;
+  S<V> scale;
+  __device__ void operator()(int i, V v) const { gstore((V*)y + i, cscale(v, scale)); }
+};
+template <typename V> struct RaderSrc {  // a[j] = x[g^-j mod N]
+  static constexpr bool kFromLds = false;
+  gbl_cptr<V> x;
+  gbl_cptr<int> perm;
+  __device__ V operator()(int j) const { return ((const V*)x)[((const int*)perm)[j]]; }
+};
+template <typename V> struct RaderSink {  // y[g^i mod N] = (v + x0) * scale
+  static constexpr bool kToLds = false;
+  gbl_ptr<V> y;
+  gbl_cptr<int> perm;
+  V x0;
+  S<V> scale;
+  __device__ void operator()(int i, V v) const {
+    ((V*)y)[((const int*)perm)[i]] = cscale(cadd(v, x0), scale);
+  }
+};
+// Spectrum times the kernel spectrum, back into LDS.  With y0 set, bin 0
+// (the sum of the Rader sequence) also gives y[0] = (x0 + bin 0) * scale.
+template <typename V> struct KernSink {
+  static constexpr bool kToLds = true;
+  lds_ptr<V> p;
+  int ps;
+  gbl_cptr<V> kern;
+  gbl_ptr<V> y0;
+  V x0;
+  S<V> scale;
+  __device__ void operator()(int i, V v) const {
+    if (y0 != nullptr && i == 0) *(V*)y0 = cscale(cadd(x0, v), scale);
+    ((V*)p)[i + (i >> ps)] = cmul(v, ((const V*)kern)[i]);
+  }
+};
+template <typename V> struct BlueSrc {  // x[n] * chirp[n], zero past N
+  static constexpr bool kFromLds = false;
+  gbl_cptr<V> x;
+  gbl_cptr<V> chirp;
+  int n;
+  __device__ V operator()(int i) const {
+    return i < n ? cmul(((const V*)x)[i], ((const V*)chirp)[i]) : V{0, 0};
+  }
+};
+template <typename V> struct BlueSink {  // y[k] = v * chirp[k] * scale, k < N
+  static constexpr bool kToLds = false;
+  gbl_ptr<V> y;
+  gbl_cptr<V> chirp;
+  int n;
+  S<V> scale;
+  __device__ void operator()(int k, V v) const {
+    if (k < n) ((V*)y)[k] = cscale(cmul(v, ((const V*)chirp)[k]), scale);
+  }
+};
+
 // ------------------------------------------------------------ smooth
 template <typename V, int PPT, int DIR>
-__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds_in,
+__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds,
                                                 int tid, bool active) {
-  V* lds = as_lds(lds_in);
-  const int N = a.m, T = a.threads, ps = a.ps;
-  if (active)
-    staged<PPT>(tid, T, N, [&](int i) { return gload(x + i); },
-                   [&](int i, V v) { lds[i + (i >> ps)] = v; });
-  __syncthreads();
-  lds_fft<V, PPT, DIR>(lds, N, T, tid, a.pd, a.tw, active, ps);
-  if (active)
-    for (int i = tid; i < N; i += T) gstore(y + i, cscale(lds[i + (i >> ps)], a.scale));
+  lds_fft_io<V, PPT, DIR>(GlobalSrc<V>{(gbl_cptr<V>)x}, GlobalSink<V>{(gbl_ptr<V>)y, a.scale}, lds,
+                          a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
 // ------------------------------------------------------------ Rader
@@ -86,30 +144,14 @@ template <typename V, int PPT>
 __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                            bool active) {
   V* lds = as_lds(lds_in);
-  const int T = a.threads, L = a.m, ps = a.ps;
-  V x0 = V{0, 0};
-  if (active) {
-    x0 = x[0];
-    staged<PPT>(tid, T, L, [&](int j) { return x[a.perm_in[j]]; },
-                   [&](int j, V v) { lds[j + (j >> ps)] = v; });
-  }
-  __syncthreads();
-  lds_fft<V, PPT, -1>(lds, L, T, tid, a.pd, a.tw, active, ps);
-  const V sum_rest = lds[0];
-  __syncthreads();
-  if (active)
-    staged<PPT>(tid, T, L, [&](int j) { return a.kern[j]; },
-                   [&](int j, V k) {
-                     const int li = j + (j >> ps);
-                     lds[li] = cmul(lds[li], k);
-                   });
-  __syncthreads();
-  lds_fft<V, PPT, +1>(lds, L, T, tid, a.pd, a.tw, active, ps);
-  if (active) {
-    staged<PPT>(tid, T, L, [&](int i) { return a.perm_out[i]; },
-                   [&](int i, int d) { y[d] = cscale(cadd(lds[i + (i >> ps)], x0), a.scale); });
-    if (tid == 0) y[0] = cscale(cadd(x0, sum_rest), a.scale);
-  }
+  const V x0 = active ? x[0] : V{0, 0};
+  __syncthreads();  // every read of x[0] precedes the write of y[0] (in place)
+  lds_fft_io<V, PPT, -1>(RaderSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<int>)a.perm_in},
+                         KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
+                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  lds_fft_io<V, PPT, +1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
+                         RaderSink<V>{(gbl_ptr<V>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
+                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
 // ------------------------------------------------------------ Bluestein
@@ -118,23 +160,12 @@ template <typename V, int PPT>
 __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                           bool active) {
   V* lds = as_lds(lds_in);
-  const int T = a.threads, M = a.m, N = a.n, ps = a.ps;
-  if (active)
-    staged<PPT>(tid, T, M, [&](int n) { return n < N ? cmul(x[n], a.chirp[n]) : V{0, 0}; },
-                   [&](int n, V v) { lds[n + (n >> ps)] = v; });
-  __syncthreads();
-  lds_fft<V, PPT, -1>(lds, M, T, tid, a.pd, a.tw, active, ps);
-  if (active)
-    staged<PPT>(tid, T, M, [&](int j) { return a.kern[j]; },
-                   [&](int j, V k) {
-                     const int li = j + (j >> ps);
-                     lds[li] = cmul(lds[li], k);
-                   });
-  __syncthreads();
-  lds_fft<V, PPT, +1>(lds, M, T, tid, a.pd, a.tw, active, ps);
-  if (active)
-    staged<PPT>(tid, T, N, [&](int k) { return a.chirp[k]; },
-                   [&](int k, V c) { y[k] = cscale(cmul(lds[k + (k >> ps)], c), a.scale); });
+  lds_fft_io<V, PPT, -1>(BlueSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<V>)a.chirp, a.n},
+                         KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, nullptr, V{0, 0}, 1},
+                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  lds_fft_io<V, PPT, +1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
+                         BlueSink<V>{(gbl_ptr<V>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
+                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
 // ------------------------------------------------------------ four-step

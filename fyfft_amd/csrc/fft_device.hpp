@@ -396,6 +396,130 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
   }
 }
 
+// ---------------------------------------------------------------- I/O-fused passes
+// The first pass of an LDS-resident transform can read its inputs through a
+// source functor (HBM, a gather, a chirp pre-multiply ...) straight into
+// registers, and the last pass can hand its outputs to a sink functor (HBM,
+// a scatter, a kernel multiply back into LDS): that saves the LDS round trip
+// and the barrier at either end.  kFromLds / kToLds mark functors that read /
+// write the transform's LDS buffer: a pass whose source and sink both touch
+// it is in place and completes every read (barrier) before any write, and a
+// sink that writes LDS is followed by a barrier.
+template <typename V> struct LdsSrc {
+  static constexpr bool kFromLds = true;
+  lds_ptr<V> p;
+  int ps;
+  __device__ V operator()(int i) const { return ((const V*)p)[i + (i >> ps)]; }
+};
+template <typename V> struct LdsSink {
+  static constexpr bool kToLds = true;
+  lds_ptr<V> p;
+  int ps;
+  __device__ void operator()(int i, V v) const { ((V*)p)[i + (i >> ps)] = v; }
+};
+
+template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
+__device__ __noinline__ void io_pass_ool(Src src, Dst dst, int N, int Ls, int T, int tid,
+                                         gbl_cptr<V> tw_g, bool active) {
+  const V* tw = (const V*)tw_g;
+  const int nb = N / R;
+  V v[MAXB][R];
+  int q0 = 0, k0 = 0, tq = 0, tk = 0;
+  if (Ls > 1) {
+    q0 = tid / Ls;
+    k0 = tid - q0 * Ls;
+    tq = T / Ls;
+    tk = T - tq * Ls;
+  }
+  {
+    int k = k0;
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = tid + b * T;
+      if (active && j < nb) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = src(j + r * nb);
+        if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
+        Dft<R, DIR, V>::run(v[b]);
+      }
+      if (Ls > 1) {
+        k += tk;
+        if (k >= Ls) k -= Ls;
+      }
+    }
+  }
+  if constexpr (Src::kFromLds && Dst::kToLds) __syncthreads();
+  {
+    int q = q0, k = k0;
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const int j = tid + b * T;
+      if (active && j < nb) {
+        const int base = (Ls > 1) ? q * Ls * R + k : j * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst(base + r * Ls, v[b][r]);
+      }
+      if (Ls > 1) {
+        q += tq;
+        k += tk;
+        if (k >= Ls) { k -= Ls; ++q; }
+      }
+    }
+  }
+  if constexpr (Dst::kToLds) __syncthreads();
+}
+
+template <int R> struct RadixC { static constexpr int value = R; };
+template <typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
+  switch (R) {
+    case 2: f(RadixC<2>{}); break;
+    case 3: f(RadixC<3>{}); break;
+    case 4: f(RadixC<4>{}); break;
+    case 5: f(RadixC<5>{}); break;
+    case 7: f(RadixC<7>{}); break;
+    case 8: f(RadixC<8>{}); break;
+    default: f(RadixC<16>{}); break;
+  }
+}
+
+// Every pass of `pd` with the first reading `src` and the last writing
+// `dst`; `lds` (padded by ps) holds the transform between passes.  Every
+// thread of the workgroup must call it (barriers).
+template <typename V, int PPT, int DIR, typename Src, typename Dst>
+__device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, int T, int tid,
+                                           const PassDesc& pd, const V* __restrict__ tw_gen,
+                                           bool active, int ps) {
+  const lds_ptr<V> lds = (lds_ptr<V>)lds_gen;
+  const gbl_cptr<V> tw = (gbl_cptr<V>)tw_gen;
+  const LdsSrc<V> ls{lds, ps};
+  const LdsSink<V> lk{lds, ps};
+  const int P = pd.npass;
+  if (P <= 1) {
+    if (P == 1)
+      radix_switch(pd.radix[0], [&](auto rc) {
+        constexpr int R = decltype(rc)::value;
+        io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, 1, T, tid, tw, active);
+      });
+    else if (active)
+      for (int i = tid; i < N; i += T) lk(i, src(i));
+    if (P == 0) __syncthreads();
+    if (active)
+      for (int i = tid; i < N; i += T) dst(i, ls(i));
+    if constexpr (Dst::kToLds) __syncthreads();
+    return;
+  }
+  int Ls = 1;
+  for (int p = 0; p < P; ++p) {
+    radix_switch(pd.radix[p], [&](auto rc) {
+      constexpr int R = decltype(rc)::value;
+      if (p == 0) io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, Ls, T, tid, tw, active);
+      else if (p == P - 1) io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, dst, N, Ls, T, tid, tw, active);
+      else io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, lk, N, Ls, T, tid, tw, active);
+    });
+    Ls *= pd.radix[p];
+  }
+}
+
 // ---------------------------------------------------------------- roots
 // exp(sign * 2*pi*i * m / D) in FP64 with an exact integer reduction of m to
 // a quadrant, so the only rounding is in r/D and in sincospi (|angle| <=
