@@ -13,6 +13,7 @@ Other configurations (one JSON line each, same fields where they apply):
   --workload ragged    20000 ragged transforms in one grouped launch vs one launch per transform
   --workload spectral  spectral convolution [256, 64, 4096], 512 modes, forward + backward
   --workload lowprec   BF16 and FP16 storage, N=4096 x 65536, radix-16 stages on MFMA
+  --workload real      real-input R2C and real-output C2R, N=8192 x 65536 and N=2^20 x 256 (float)
 """
 from __future__ import annotations
 
@@ -41,7 +42,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=65536,
                     help="transforms in the CPU baseline sample (65536 x 4096 points: "
                          "about 20 s of CPU time over 16 threads)")
-    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral", "lowprec"],
+    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral", "lowprec", "real"],
                     default="c2c")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="3 FFT launches and 3 plain copies of the same size, for rocprofv3 --pmc")
@@ -143,6 +144,52 @@ def bench_transforms(a, cases, label):
     return line({"config": {"workload": label, "cases": rows}}, value=round(tot_flops / tot_ms / 1e6, 2),
                 steps=a.steps, warmup=a.warmup, ms_per_step=round(tot_ms, 4),
                 dtype="/".join(sorted({r["dtype"] for r in rows})),
+                roofline={"bound": "hbm", "achieved": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                          "traffic": None})
+
+
+def bench_real(a):
+    """R2C and C2R plans; algorithmic bytes per row: n reals + (n/2+1) complex."""
+    import numpy as np
+    import torch
+    import fyfft_amd as fy
+    import oracle
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    rows, tot_flops, tot_ms, tot_bytes = [], 0.0, 0.0, 0.0
+    for n, batch in ((8192, 65536), (1 << 20, 256)):
+        g = torch.Generator(device=dev).manual_seed(0)
+        x = torch.randn(batch, n, device=dev, generator=g)
+        X = torch.empty(batch, n // 2 + 1, dtype=torch.complex64, device=dev)
+        y = torch.empty_like(x)
+        plan = fy.RealPlan(n, batch, fy.F32)
+        h = stream.cuda_stream
+        byt = 4.0 * n * batch + 8.0 * (n // 2 + 1) * batch
+        for d, src, dst in ((fy.FORWARD, x, X), (fy.INVERSE, X, y)):
+            ms = time_steps(lambda: plan.execute_ptr(src.data_ptr(), dst.data_ptr(), d, h), a.steps, a.warmup,
+                            stream)
+            r = [0, batch - 1]
+            xs = x[r].cpu().numpy().astype(np.complex128)
+            ref = oracle.fft_rows(xs)[:, : n // 2 + 1]
+            err = (oracle.rel_l2(X[r].cpu().numpy(), ref) if d == fy.FORWARD
+                   else oracle.rel_l2(y[r].cpu().numpy(), x[r].cpu().numpy()))
+            flops = 0.5 * flops_of(n) * batch  # half of the complex transform's nominal count
+            rows.append({"n": n, "batch": batch, "dtype": "f32",
+                         "direction": "R2C" if d == fy.FORWARD else "C2R (vs input)",
+                         "plan": plan.describe(), "ms": round(ms, 4),
+                         "GFLOP/s": round(flops / ms / 1e6, 1),
+                         "hbm_frac": round(byt / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                         "rel_l2_vs_fp64": err})
+            tot_flops += flops
+            tot_ms += ms
+            tot_bytes += byt
+        plan.close()
+        del x, X, y
+        torch.cuda.empty_cache()
+    return line({"config": {"workload": "real", "cases": rows}}, value=round(tot_flops / tot_ms / 1e6, 2),
+                steps=a.steps, warmup=a.warmup, ms_per_step=round(tot_ms, 4), dtype="f32",
                 roofline={"bound": "hbm", "achieved": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
                           "frac": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
@@ -290,6 +337,8 @@ def main():
             out = bench_ragged(a)
         elif a.workload == "lowprec":
             out = bench_lowprec(a)
+        elif a.workload == "real":
+            out = bench_real(a)
         else:
             out = bench_spectral(a)
         print(json.dumps(out), flush=True)

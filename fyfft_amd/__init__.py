@@ -101,6 +101,7 @@ def _load() -> ctypes.CDLL:
         L.fyfft_plan_create_grouped.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(sz), sz, i, ctypes.c_uint]
         L.fyfft_plan_elements.argtypes = [vp, ctypes.POINTER(sz)]
         L.fyfft_plan_create_ex.argtypes = [ctypes.POINTER(vp), sz, sz, i, ctypes.POINTER(_Options)]
+        L.fyfft_plan_create_real.argtypes = [ctypes.POINTER(vp), sz, sz, i]
         L.fyfft_wisdom_export.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.fyfft_wisdom_import.argtypes = [ctypes.c_char_p]
         L.fyfft_wisdom_forget.argtypes = []
@@ -112,7 +113,8 @@ def _load() -> ctypes.CDLL:
                      "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
                      "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_plan_create_grouped",
-                     "fyfft_plan_elements", "fyfft_plan_create_ex", "fyfft_wisdom_export",
+                     "fyfft_plan_elements", "fyfft_plan_create_ex", "fyfft_plan_create_real",
+                     "fyfft_wisdom_export",
                      "fyfft_wisdom_import", "fyfft_wisdom_forget"):
             getattr(L, name).restype = i
         _lib = L
@@ -321,6 +323,102 @@ def irfft_modes(modes, L: int, scale0: float, scale: float, stream=None):
                                      k, float(scale0), float(scale),
                                      ctypes.c_void_p(_stream_handle(stream, modes.device))), "irfft_modes")
     return y
+
+
+class RealPlan(Plan):
+    """``batch`` rows of real length ``n``: ``execute(x, X)`` (FORWARD, R2C)
+    maps real rows [.., n] to complex [.., n//2 + 1]; ``execute(X, x,
+    INVERSE)`` (C2R) maps a Hermitian half spectrum back to real rows,
+    scaled by 1/n.  Input and output must not overlap."""
+
+    def __init__(self, n: int, batch: int = 1, precision: int = F32):
+        self.n, self.batch, self.precision = int(n), int(batch), int(precision)
+        self._h = ctypes.c_void_p()
+        _check(_load().fyfft_plan_create_real(ctypes.byref(self._h), self.n, self.batch, self.precision),
+               f"plan_create_real(N={n}, batch={batch}, precision={precision})")
+
+    def _check_side(self, t, name, real):
+        import torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError(f"{name} must be a device tensor")
+        want = ((torch.float32 if self.precision == F32 else torch.float64) if real
+                else (torch.complex64 if self.precision == F32 else torch.complex128))
+        if t.dtype != want:
+            raise TypeError(f"{name} dtype {t.dtype}, plan needs {want}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        need = self.batch * (self.n if real else self.n // 2 + 1)
+        if t.numel() != need:
+            raise ValueError(f"{name} has {t.numel()} elements, plan needs {need}")
+
+    def execute(self, x, out=None, direction: int = FORWARD, stream=None):
+        import torch
+        r2c = direction == FORWARD
+        self._check_side(x, "input", r2c)
+        if out is None:
+            dt = ((torch.complex64 if self.precision == F32 else torch.complex128) if r2c
+                  else (torch.float32 if self.precision == F32 else torch.float64))
+            shape = tuple(x.shape[:-1]) + ((self.n // 2 + 1) if r2c else self.n,)
+            out = torch.empty(shape, dtype=dt, device=x.device)
+        self._check_side(out, "output", not r2c)
+        self.execute_ptr(x.data_ptr(), out.data_ptr(), direction, _stream_handle(stream, x.device))
+        return out
+
+    def execute_host(self, x: np.ndarray, out: np.ndarray | None = None, direction: int = FORWARD):
+        r2c = direction == FORWARD
+        rdt, cdt = (np.float32, np.complex64) if self.precision == F32 else (np.float64, np.complex128)
+        x = np.ascontiguousarray(x, dtype=rdt if r2c else cdt)
+        if x.size != self.batch * (self.n if r2c else self.n // 2 + 1):
+            raise ValueError("host buffer size does not match the plan")
+        if out is None:
+            out = np.empty((self.batch, self.n // 2 + 1) if r2c else (self.batch, self.n),
+                           dtype=cdt if r2c else rdt)
+        _check(_load().fyfft_execute_host(self._h, ctypes.c_void_p(x.ctypes.data),
+                                          ctypes.c_void_p(out.ctypes.data), int(direction)),
+               "execute_host")
+        return out
+
+
+_real_cache: dict = {}
+
+
+def _cached_real_plan(n: int, batch: int, precision: int, device: int) -> RealPlan:
+    import torch
+    key = (n, batch, precision, device)
+    with _plan_cache_lock:
+        p = _real_cache.get(key)
+        if p is None:
+            with torch.cuda.device(device):
+                p = RealPlan(n, batch, precision)
+            if len(_real_cache) > 64:
+                _real_cache.clear()
+            _real_cache[key] = p
+        return p
+
+
+def rfft(x, stream=None):
+    """Real-input FFT over the last dimension of a float32/float64 device
+    tensor: complex [..., n//2 + 1], unnormalised (torch.fft.rfft semantics)."""
+    import torch
+    if x.dtype not in (torch.float32, torch.float64) or not x.is_cuda:
+        raise TypeError("rfft: real float32/float64 device tensor expected")
+    x = x.contiguous()
+    n = x.shape[-1]
+    p = _cached_real_plan(n, x.numel() // n, F32 if x.dtype == torch.float32 else F64, x.device.index or 0)
+    return p.execute(x, None, FORWARD, stream)
+
+
+def irfft(X, n: int, stream=None):
+    """Inverse of rfft: real [..., n] from the half spectrum [..., n//2 + 1],
+    scaled by 1/n (torch.fft.irfft semantics)."""
+    import torch
+    if X.dtype not in (torch.complex64, torch.complex128) or not X.is_cuda:
+        raise TypeError("irfft: complex64/complex128 device tensor expected")
+    X = X.contiguous()
+    if X.shape[-1] != n // 2 + 1:
+        raise ValueError(f"irfft: last dimension {X.shape[-1]} != n//2 + 1 = {n // 2 + 1}")
+    p = _cached_real_plan(n, X.numel() // X.shape[-1], _precision_of(X.dtype), X.device.index or 0)
+    return p.execute(X, None, INVERSE, stream)
 
 
 GROUPED_GRAPH = 1

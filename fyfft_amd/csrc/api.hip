@@ -21,7 +21,11 @@ namespace {
 constexpr uint32_t kMagic = 0xF1FF7A11u;
 thread_local std::string g_last_error;
 
-enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7, kMfma = 8 };
+enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7, kMfma = 8,
+            kReal = 9 };
+
+// Forms of a real (R2C / C2R) plan.
+enum RealForm { kRealFusedHalf = 0, kRealFusedFull = 1, kRealHalfPasses = 2, kRealFullPasses = 3 };
 
 // Workspace of multi-pass plans is capped; larger batches run in chunks.
 constexpr size_t kWorkspaceCap = (size_t)1 << 30;
@@ -68,6 +72,11 @@ struct fyfft_plan {
   void* chirp[2] = {nullptr, nullptr};
   int* perm_in = nullptr;
   int* perm_out = nullptr;
+  // real plans: complex sub-plan subs[0] of length n/2 (half forms) or n
+  int real_form = -1;
+  void* rtw = nullptr;   // exp(-2*pi*i*k/n), k <= n/2 (half forms)
+  void* rws = nullptr;   // rchunk rows of the sub-plan's length (multi-pass forms)
+  size_t rchunk = 0;
   // grouped plans
   unsigned gflags = 0;
   size_t count = 0, total = 0;               // transforms, sum of lengths
@@ -570,10 +579,10 @@ static void fill_pass(FsArgs<V>& a, const LdsGeom& g, size_t len, const dev::Pas
 
 template <typename V>
 static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int dir, bool inv_tables,
-                                  double scale, hipStream_t s) {
+                                  double scale, size_t rows, hipStream_t s) {
   const size_t n = p->n, L = (p->kind == kFourStep) ? n : p->m;
-  for (size_t b0 = 0; b0 < p->batch; b0 += p->chunk) {
-    const size_t cb = std::min(p->chunk, p->batch - b0);
+  for (size_t b0 = 0; b0 < rows; b0 += p->chunk) {
+    const size_t cb = std::min(p->chunk, rows - b0);
     FsArgs<V> a;
     a.in = in + b0 * n;
     a.out = out + b0 * n;
@@ -613,11 +622,11 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
 
 template <typename V>
 static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool inv_tables,
-                                  double scale, hipStream_t s) {
+                                  double scale, size_t rows, hipStream_t s) {
   ConvArgs<V> a;
   a.in = in;
   a.out = out;
-  a.batch = (long long)p->batch;
+  a.batch = (long long)rows;
   a.dist = (long long)p->n;
   a.n_orig = (int)p->n;
   a.m = (int)p->m;
@@ -1049,6 +1058,182 @@ static fyfft_status exec_grouped(const fyfft_plan* p, const void* in, void* out,
   return FYFFT_SUCCESS;
 }
 
+// Execution over the first `rows` (<= batch) transforms of a plan.
+static fyfft_status exec_real(const fyfft_plan* p, const void* in, void* out, int direction, size_t rows,
+                              hipStream_t s);
+
+static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out, int direction, size_t rows,
+                                 hipStream_t s) {
+  if (!in || !out) return FYFFT_INVALID_ARGUMENT;
+  if (direction != FYFFT_FORWARD && direction != FYFFT_INVERSE && direction != FYFFT_ADJOINT)
+    return FYFFT_INVALID_ARGUMENT;
+  const double scale = direction == FYFFT_INVERSE ? 1.0 / (double)p->n : 1.0;
+  const int dir = direction == FYFFT_FORWARD ? kForward : (direction == FYFFT_INVERSE ? kInverse : kAdjoint);
+  const bool inv_tables = direction != FYFFT_FORWARD;
+  switch (p->kind) {
+    case kTrivial:
+      if (in != out)
+        HIP_TRY(hipMemcpyAsync(out, in, rows * elem_bytes(p->prec), hipMemcpyDeviceToDevice, s));
+      return FYFFT_SUCCESS;
+    case kLds:
+      HIP_TRY(launch_lds(p->prec, dir, p->lds, in, out, rows, p->n, scale, s));
+      return FYFFT_SUCCESS;
+    case kFourStep:
+    case kRaderFs:
+    case kBlueFs: {
+      const int d = dir == kForward ? -1 : +1;
+      if (p->prec == kF64)
+        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, rows, s);
+      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, rows, s);
+    }
+    case kRaderLds:
+    case kBlueLds:
+      if (p->prec == kF64)
+        return exec_conv_lds<double2>(p, (const double2*)in, (double2*)out, inv_tables, scale, rows, s);
+      return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, rows, s);
+    case kGrouped:
+      return exec_grouped(p, in, out, direction, s);
+    case kMfma:
+      HIP_TRY(launch_mfma16(p->prec == kBF16, dir == kForward ? -1 : +1, p->n, in, out, rows,
+                            p->lds.tw, scale, s));
+      return FYFFT_SUCCESS;
+    case kReal:
+      return exec_real(p, in, out, direction, rows, s);
+  }
+  return FYFFT_INVALID_PLAN;
+}
+
+// ------------------------------------------------------------ real plans
+template <typename V>
+static fyfft_status exec_real_fused(const fyfft_plan* p, const void* in, void* out, bool r2c, size_t rows,
+                                    hipStream_t s) {
+  const fyfft_plan* sp = p->subs[0];
+  const bool half = p->real_form == kRealFusedHalf;
+  const size_t len = half ? p->n / 2 : p->n;
+  RealArgs<V> a;
+  a.in = in;
+  a.out = out;
+  a.rows = (long long)rows;
+  a.n = (int)p->n;
+  const LdsGeom& g = sp->lds.geom;
+  a.threads = g.threads;
+  a.per_block = g.per_block;
+  a.ps = g.pad_shift;
+  a.stride = dev::lds_len((int)len, g.pad_shift);
+  a.pd = sp->lds.pd;
+  a.tw = (const V*)sp->lds.tw;
+  a.rtw = (const V*)p->rtw;
+  a.scale = (dev::S<V>)(r2c ? 1.0 : 1.0 / (double)len);
+  const int mode = half ? (r2c ? kRealR2cHalf : kRealC2rHalf) : (r2c ? kRealR2cFull : kRealC2rFull);
+  HIP_TRY(launch_real_lds(p->prec, mode, &a, g, s));
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status exec_real(const fyfft_plan* p, const void* in, void* out, int direction, size_t rows,
+                              hipStream_t s) {
+  if (direction == FYFFT_ADJOINT) {
+    g_last_error = "real plans run FYFFT_FORWARD (R2C) and FYFFT_INVERSE (C2R) only";
+    return FYFFT_UNSUPPORTED;
+  }
+  const bool r2c = direction == FYFFT_FORWARD;
+  if (p->real_form == kRealFusedHalf || p->real_form == kRealFusedFull)
+    return p->prec == kF64 ? exec_real_fused<double2>(p, in, out, r2c, rows, s)
+                           : exec_real_fused<float2>(p, in, out, r2c, rows, s);
+  const fyfft_plan* sp = p->subs[0];
+  const size_t n = p->n, h = n / 2, hk = n / 2 + 1;
+  const size_t cplx = elem_bytes(p->prec), real = cplx / 2;
+  const bool half = p->real_form == kRealHalfPasses;
+  for (size_t b0 = 0; b0 < rows; b0 += p->rchunk) {
+    const long long cb = (long long)std::min(p->rchunk, rows - b0);
+    if (r2c) {
+      const char* x = (const char*)in + b0 * n * real;
+      char* X = (char*)out + b0 * hk * cplx;
+      if (half) {
+        fyfft_status st = execute_rows(sp, x, p->rws, FYFFT_FORWARD, (size_t)cb, s);
+        if (st) return st;
+        HIP_TRY(launch_real_pass(p->prec, kRealOpPost, p->rws, X, cb, (int)n, p->rtw, 1.0, s));
+      } else {
+        HIP_TRY(launch_real_pass(p->prec, kRealOpExpand, x, p->rws, cb, (int)n, nullptr, 1.0, s));
+        fyfft_status st = execute_rows(sp, p->rws, p->rws, FYFFT_FORWARD, (size_t)cb, s);
+        if (st) return st;
+        HIP_TRY(launch_real_pass(p->prec, kRealOpTruncate, p->rws, X, cb, (int)n, nullptr, 1.0, s));
+      }
+    } else {
+      const char* X = (const char*)in + b0 * hk * cplx;
+      char* y = (char*)out + b0 * n * real;
+      if (half) {
+        HIP_TRY(launch_real_pass(p->prec, kRealOpPre, X, p->rws, cb, (int)n, p->rtw, 1.0, s));
+        fyfft_status st = execute_rows(sp, p->rws, y, FYFFT_INVERSE, (size_t)cb, s);
+        if (st) return st;
+      } else {
+        HIP_TRY(launch_real_pass(p->prec, kRealOpHermExpand, X, p->rws, cb, (int)n, nullptr, 1.0, s));
+        fyfft_status st = execute_rows(sp, p->rws, p->rws, FYFFT_INVERSE, (size_t)cb, s);
+        if (st) return st;
+        HIP_TRY(launch_real_pass(p->prec, kRealOpRealPart, p->rws, y, cb, (int)n, nullptr, 1.0, s));
+      }
+    }
+    (void)h;
+  }
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status create_real_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision) {
+  if (!out) return FYFFT_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (batch == 0) return FYFFT_INVALID_ARGUMENT;
+  if (precision == FYFFT_BF16 || precision == FYFFT_F16) return FYFFT_UNSUPPORTED;
+  if (precision != FYFFT_F32 && precision != FYFFT_F64) return FYFFT_INVALID_ARGUMENT;
+  if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  fyfft_plan* p = new (std::nothrow) fyfft_plan();
+  if (!p) return FYFFT_ALLOC_FAILED;
+  p->kind = kReal;
+  p->n = n;
+  p->batch = batch;
+  p->prec = precision == FYFFT_F64 ? kF64 : kF32;
+  p->device = dev;
+  p->strategy = strategy_for(n);
+  const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(dev), h = n / 2;
+  LdsGeom g;
+  if (n % 2 == 0 && h >= 2 && is_smooth7(h) && lds_geometry(h, elem, cap, &g)) p->real_form = kRealFusedHalf;
+  else if (n % 2 == 0 && h >= 2 && is_smooth7(h) && fs_possible(h, elem, cap)) p->real_form = kRealHalfPasses;
+  else if (n >= 2 && is_smooth7(n) && lds_geometry(n, elem, cap, &g)) p->real_form = kRealFusedFull;
+  else p->real_form = kRealFullPasses;
+  const bool half = p->real_form == kRealFusedHalf || p->real_form == kRealHalfPasses;
+  const bool fused = p->real_form == kRealFusedHalf || p->real_form == kRealFusedFull;
+  const size_t len = half ? h : n;
+  p->rchunk = fused ? batch : std::min(batch, std::max<size_t>(1, kWorkspaceCap / (len * elem)));
+  fyfft_plan* sp = nullptr;
+  fyfft_status st = create_plan_impl(&sp, len, fused ? 1 : p->rchunk, precision,
+                                     fused ? FYFFT_FORM_PLAIN : FYFFT_FORM_AUTO);
+  if (!st) {
+    p->subs.push_back(sp);
+    if (fused && sp->kind != kLds) st = FYFFT_UNSUPPORTED;
+  }
+  if (!st && half) {
+    st = p->alloc((h + 1) * elem, &p->rtw);
+    if (!st) {
+      const hipError_t e = launch_gen_roots(p->prec, p->rtw, (long long)(h + 1), 1, (long long)n, nullptr);
+      if (e != hipSuccess) st = from_hip(e, "real twiddles");
+    }
+  }
+  if (!st && !fused) st = p->alloc(p->rchunk * len * elem, &p->rws);
+  if (!st) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) st = from_hip(e, "real plan tables");
+  }
+  if (st) {
+    delete p;
+    return st;
+  }
+  static const char* kForms[] = {"fused half-length LDS", "fused full-length LDS",
+                                 "half-length transform + pre/post pass", "full-length transform + expand pass"};
+  p->desc = "real N=" + std::to_string(n) + " form=" + kForms[p->real_form] + " [" + sp->desc + "]";
+  *out = p;
+  return FYFFT_SUCCESS;
+}
+
 extern "C" {
 
 const char* fyfft_status_string(fyfft_status s) {
@@ -1097,6 +1282,10 @@ fyfft_status fyfft_plan_create_ex(fyfft_plan** out, size_t n, size_t batch, fyff
     form = options->form;
   }
   return create_plan_impl(out, n, batch, precision, form);
+}
+
+fyfft_status fyfft_plan_create_real(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision) {
+  return create_real_impl(out, n, batch, precision);
 }
 
 fyfft_status fyfft_wisdom_export(char* buf, size_t len, size_t* needed) {
@@ -1181,42 +1370,7 @@ fyfft_status fyfft_plan_destroy(fyfft_plan* p) {
 
 fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int direction, void* stream) {
   if (!valid(p)) return FYFFT_INVALID_PLAN;
-  if (!in || !out) return FYFFT_INVALID_ARGUMENT;
-  if (direction != FYFFT_FORWARD && direction != FYFFT_INVERSE && direction != FYFFT_ADJOINT)
-    return FYFFT_INVALID_ARGUMENT;
-  hipStream_t s = (hipStream_t)stream;
-  const double scale = direction == FYFFT_INVERSE ? 1.0 / (double)p->n : 1.0;
-  const int dir = direction == FYFFT_FORWARD ? kForward : (direction == FYFFT_INVERSE ? kInverse : kAdjoint);
-  const bool inv_tables = direction != FYFFT_FORWARD;
-  switch (p->kind) {
-    case kTrivial:
-      if (in != out)
-        HIP_TRY(hipMemcpyAsync(out, in, p->batch * elem_bytes(p->prec), hipMemcpyDeviceToDevice, s));
-      return FYFFT_SUCCESS;
-    case kLds:
-      HIP_TRY(launch_lds(p->prec, dir, p->lds, in, out, p->batch, p->n, scale, s));
-      return FYFFT_SUCCESS;
-    case kFourStep:
-    case kRaderFs:
-    case kBlueFs: {
-      const int d = dir == kForward ? -1 : +1;
-      if (p->prec == kF64)
-        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, s);
-      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, s);
-    }
-    case kRaderLds:
-    case kBlueLds:
-      if (p->prec == kF64)
-        return exec_conv_lds<double2>(p, (const double2*)in, (double2*)out, inv_tables, scale, s);
-      return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, s);
-    case kGrouped:
-      return exec_grouped(p, in, out, direction, s);
-    case kMfma:
-      HIP_TRY(launch_mfma16(p->prec == kBF16, dir == kForward ? -1 : +1, p->n, in, out, p->batch,
-                            p->lds.tw, scale, s));
-      return FYFFT_SUCCESS;
-  }
-  return FYFFT_INVALID_PLAN;
+  return execute_rows(p, in, out, direction, p->batch, (hipStream_t)stream);
 }
 
 fyfft_status fyfft_rfft_modes(const fyfft_plan* p, const void* x, void* modes, size_t k,
@@ -1233,7 +1387,18 @@ fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, 
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   if (!in || !out) return FYFFT_INVALID_ARGUMENT;
   std::lock_guard<std::mutex> lock(p->host_mu);
-  const size_t bytes = (p->kind == kGrouped ? p->total : p->n * p->batch) * elem_bytes(p->prec);
+  const size_t elem = elem_bytes(p->prec);
+  size_t in_bytes, out_bytes;
+  if (p->kind == kReal) {
+    const size_t real = p->batch * p->n * (elem / 2), half = p->batch * (p->n / 2 + 1) * elem;
+    in_bytes = direction == FYFFT_FORWARD ? real : half;
+    out_bytes = direction == FYFFT_FORWARD ? half : real;
+  } else {
+    in_bytes = out_bytes = (p->kind == kGrouped ? p->total : p->n * p->batch) * elem;
+  }
+  // in place for complex plans; real plans stage input and output apart
+  const size_t out_off = p->kind == kReal ? (in_bytes + 255) / 256 * 256 : 0;
+  const size_t bytes = std::max(in_bytes, out_off + out_bytes);
   if (p->d_io_bytes < bytes) {
     if (p->d_io) (void)hipFree(p->d_io);
     p->d_io = nullptr;
@@ -1241,10 +1406,11 @@ fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, 
     HIP_TRY(hipMalloc(&p->d_io, bytes));
     p->d_io_bytes = bytes;
   }
-  HIP_TRY(hipMemcpy(p->d_io, in, bytes, hipMemcpyHostToDevice));
-  fyfft_status st = fyfft_execute(p, p->d_io, p->d_io, direction, nullptr);
+  void* dout = (char*)p->d_io + out_off;
+  HIP_TRY(hipMemcpy(p->d_io, in, in_bytes, hipMemcpyHostToDevice));
+  fyfft_status st = fyfft_execute(p, p->d_io, dout, direction, nullptr);
   if (st) return st;
-  HIP_TRY(hipMemcpy(out, p->d_io, bytes, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
   return FYFFT_SUCCESS;
 }
 

@@ -143,6 +143,33 @@ template <typename V> struct SpecArgs {
 };
 hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGeom& g, hipStream_t s);
 
+// --- kernels_real.hip
+// Fused LDS-resident real transforms (one row per workgroup slot):
+//   kRealR2cHalf / kRealC2rHalf: even n through one complex FFT of n/2,
+//   kRealR2cFull / kRealC2rFull: any n through one complex FFT of n.
+// R2C rows: n reals in, n/2+1 complex out; C2R the reverse.
+enum RealMode { kRealR2cHalf = 0, kRealC2rHalf = 1, kRealR2cFull = 2, kRealC2rFull = 3 };
+template <typename V> struct RealArgs {
+  const void* in = nullptr;
+  void* out = nullptr;
+  long long rows = 0;
+  int n = 0;  // real length
+  int threads = 0, per_block = 1, ps = 4, stride = 0;
+  dev::PassDesc pd{};
+  const V* tw = nullptr;   // Stockham table of the complex FFT (n/2 or n)
+  const V* rtw = nullptr;  // exp(-2*pi*i*k/n), k <= n/2 (half forms)
+  dev::S<V> scale = 1;
+};
+hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, hipStream_t s);
+// Elementwise passes of the multi-pass real forms over `rows` rows of real
+// length n: Post (half spectrum Z -> X[0..n/2], times scale), Pre (X -> Z),
+// Expand (real -> complex), Truncate (full spectrum -> bins 0..n/2, times
+// scale), HermExpand (bins 0..n/2 -> full spectrum), RealPart (times scale).
+enum RealOp { kRealOpPost = 0, kRealOpPre = 1, kRealOpExpand = 2, kRealOpTruncate = 3,
+              kRealOpHermExpand = 4, kRealOpRealPart = 5 };
+hipError_t launch_real_pass(int prec, int op, const void* src, void* dst, long long rows, int n,
+                            const void* w, double scale, hipStream_t s);
+
 // --- kernels_grouped.hip
 enum GroupedKind { kGkLds = 0, kGkRader = 1, kGkBlue = 2, kGkFsPlain = 3, kGkFsRader = 4, kGkFsBlue = 5 };
 

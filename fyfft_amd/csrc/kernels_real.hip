@@ -1,0 +1,256 @@
+// Real-input (R2C) and real-output (C2R) transforms.
+//
+// Even n = 2h.  The real row is read as h complex values z[j] = x[2j] +
+// i*x[2j+1] (the same bytes), one complex FFT of length h gives Z, and
+//   E[k] = (Z[k] + conj(Z[h-k])) / 2          (spectrum of x[2j])
+//   O[k] = (Z[k] - conj(Z[h-k])) / (2i)       (spectrum of x[2j+1])
+//   X[k] = E[k] + w^k O[k],  w = exp(-2*pi*i/n),  k = 0..h  (Z[h] = Z[0]).
+// The inverse runs the same algebra backwards from a Hermitian half
+// spectrum X[0..h]:
+//   E[k] = (X[k] + conj(X[h-k])) / 2,  O[k] = (X[k] - conj(X[h-k])) conj(w^k) / 2,
+//   z = IFFT_h(E + i*O) / h,  x[2j] = Re z[j],  x[2j+1] = Im z[j],
+// which is x = IDFT_n(X) / n with the upper half of X implied by symmetry.
+// HBM traffic is 4n bytes in and 8(h+1) out for FP32 (half the C2C bytes).
+//
+// Odd n: one complex FFT of length n over the real row (imaginary parts
+// zero), keeping bins 0..(n-1)/2; the inverse rebuilds the full spectrum
+// from the half (X[n-k] = conj(X[k])) and keeps the real part.
+//
+// LDS-resident kernels fuse everything into one pass over HBM; the
+// elementwise kernels below serve the multi-pass forms for lengths whose
+// transform does not fit one workgroup.
+#include "bodies.hpp"
+
+namespace fyfft {
+using namespace dev;
+
+namespace {
+
+template <typename V> __device__ __forceinline__ V conj_(V a) { return V{a.x, -a.y}; }
+template <typename V> __device__ __forceinline__ V half_(V a) { return V{a.x * (S<V>)0.5, a.y * (S<V>)0.5}; }
+
+// X[k] from the half-length spectrum (zk = Z[k mod h], zm = Z[(h-k) mod h])
+template <typename V> __device__ __forceinline__ V r2c_combine(V zk, V zm, V w) {
+  const V e = half_(cadd(zk, conj_(zm)));
+  const V d = half_(csub(zk, conj_(zm)));
+  const V o = V{d.y, -d.x};  // d / i
+  return cadd(e, cmul(w, o));
+}
+
+// Z[k] = E[k] + i*O[k] from the Hermitian half spectrum (xk = X[k], xm = X[h-k])
+template <typename V> __device__ __forceinline__ V c2r_combine(V xk, V xm, V w) {
+  const V e = half_(cadd(xk, conj_(xm)));
+  const V o = cmulc(half_(csub(xk, conj_(xm))), w);  // * conj(w^k)
+  return V{e.x - o.y, e.y + o.x};
+}
+
+// ---- source and sink functors of the fused kernels
+template <typename V> struct HermHalfSrc {  // Z[k] of an even-length C2R
+  static constexpr bool kFromLds = false;
+  gbl_cptr<V> x;  // X[0..h]
+  gbl_cptr<V> w;  // w^k, k <= h
+  int h;
+  __device__ V operator()(int k) const {
+    const V* X = (const V*)x;
+    V xk = X[k], xm = X[h - k];
+    if (k == 0) xk.y = xm.y = 0;  // bins 0 and n/2 of a real signal are real
+    return c2r_combine(xk, xm, ((const V*)w)[k]);
+  }
+};
+template <typename V> struct RealSrc {  // (x[i], 0)
+  static constexpr bool kFromLds = false;
+  const __attribute__((address_space(1))) S<V>* x;
+  __device__ V operator()(int i) const { return V{((const S<V>*)x)[i], 0}; }
+};
+template <typename V> struct HalfSink {  // keep bins 0..kmax
+  static constexpr bool kToLds = false;
+  gbl_ptr<V> y;
+  int kmax;
+  __device__ void operator()(int k, V v) const {
+    if (k <= kmax) ((V*)y)[k] = v;
+  }
+};
+template <typename V> struct HermFullSrc {  // full spectrum from bins 0..n/2
+  static constexpr bool kFromLds = false;
+  gbl_cptr<V> x;
+  int n;
+  __device__ V operator()(int k) const {
+    const V* X = (const V*)x;
+    return 2 * k <= n ? X[k] : conj_(X[n - k]);
+  }
+};
+template <typename V> struct RealSink {  // y[i] = Re(v) * scale
+  static constexpr bool kToLds = false;
+  __attribute__((address_space(1))) S<V>* y;
+  S<V> scale;
+  __device__ void operator()(int i, V v) const { ((S<V>*)y)[i] = v.x * scale; }
+};
+
+}  // namespace
+
+// One workgroup slot per row; MODE selects the form.
+template <typename V, int PPT, int MODE>
+__global__ __launch_bounds__(1024) void k_real_lds(RealArgs<V> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int T = a.threads;
+  const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
+  const long long t = (long long)blockIdx.x * a.per_block + slot;
+  const bool active = t < a.rows;
+  const long long row = active ? t : 0;
+  V* lds = reinterpret_cast<V*>(smem_raw) + slot * a.stride;
+  const lds_ptr<V> ldsp = (lds_ptr<V>)as_lds(lds);
+  const int n = a.n, h = n / 2, ps = a.ps;
+  if constexpr (MODE == kRealR2cHalf) {
+    const V* z = (const V*)a.in + row * h;
+    lds_fft_io<V, PPT, -1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{ldsp, ps}, lds, h, T, tid, a.pd,
+                           a.tw, active, ps);
+    if (active) {
+      V* X = (V*)a.out + row * (h + 1);
+      const V* L = (const V*)ldsp;
+      for (int k = tid; k <= h; k += T) {
+        const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
+        gstore(X + k, cscale(r2c_combine(L[k0 + (k0 >> ps)], L[km + (km >> ps)], a.rtw[k]), a.scale));
+      }
+    }
+  } else if constexpr (MODE == kRealC2rHalf) {
+    const V* X = (const V*)a.in + row * (h + 1);
+    V* z = (V*)a.out + row * h;
+    lds_fft_io<V, PPT, +1>(HermHalfSrc<V>{(gbl_cptr<V>)X, (gbl_cptr<V>)a.rtw, h},
+                           GlobalSink<V>{(gbl_ptr<V>)z, a.scale}, lds, h, T, tid, a.pd, a.tw, active, ps);
+  } else if constexpr (MODE == kRealR2cFull) {
+    const S<V>* x = (const S<V>*)a.in + row * n;
+    V* X = (V*)a.out + row * (n / 2 + 1);
+    lds_fft_io<V, PPT, -1>(RealSrc<V>{(const __attribute__((address_space(1))) S<V>*)x},
+                           HalfSink<V>{(gbl_ptr<V>)X, n / 2}, lds, n, T, tid, a.pd, a.tw, active, ps);
+  } else {  // kRealC2rFull
+    const V* X = (const V*)a.in + row * (n / 2 + 1);
+    S<V>* y = (S<V>*)a.out + row * n;
+    lds_fft_io<V, PPT, +1>(HermFullSrc<V>{(gbl_cptr<V>)X, n},
+                           RealSink<V>{(__attribute__((address_space(1))) S<V>*)y, a.scale}, lds, n, T,
+                           tid, a.pd, a.tw, active, ps);
+  }
+}
+
+// ---- elementwise helpers of the multi-pass forms (one thread per output)
+template <typename V>
+__global__ void k_real_post(const V* z, V* X, long long rows, int h, const V* w, S<V> scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * (h + 1)) return;
+  const long long r = i / (h + 1);
+  const int k = (int)(i - r * (h + 1));
+  const V* Z = z + r * h;
+  const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
+  X[i] = cscale(r2c_combine(Z[k0], Z[km], w[k]), scale);
+}
+template <typename V>
+__global__ void k_real_pre(const V* X, V* z, long long rows, int h, const V* w) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * h) return;
+  const long long r = i / h;
+  const int k = (int)(i - r * h);
+  const V* Xr = X + r * (h + 1);
+  V xk = Xr[k], xm = Xr[h - k];
+  if (k == 0) xk.y = xm.y = 0;  // bins 0 and n/2 of a real signal are real
+  z[i] = c2r_combine(xk, xm, w[k]);
+}
+template <typename V>
+__global__ void k_real_expand(const S<V>* x, V* W, long long count) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) W[i] = V{x[i], 0};
+}
+template <typename V>
+__global__ void k_real_truncate(const V* W, V* X, long long rows, int n, S<V> scale) {
+  const int hk = n / 2 + 1;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * hk) return;
+  const long long r = i / hk;
+  const int k = (int)(i - r * hk);
+  X[i] = cscale(W[r * n + k], scale);
+}
+template <typename V>
+__global__ void k_herm_expand(const V* X, V* W, long long rows, int n) {
+  const int hk = n / 2 + 1;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * n) return;
+  const long long r = i / n;
+  const int k = (int)(i - r * n);
+  const V* Xr = X + r * hk;
+  W[i] = 2 * k <= n ? Xr[k] : conj_(Xr[n - k]);
+}
+template <typename V>
+__global__ void k_real_part(const V* W, S<V>* y, long long count, S<V> scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) y[i] = W[i].x * scale;
+}
+
+template <typename V, int PPT>
+static hipError_t go_real(int mode, const RealArgs<V>& a, const LdsGeom& g, hipStream_t s) {
+  void (*k)(RealArgs<V>) = nullptr;
+  switch (mode) {
+    case kRealR2cHalf: k = k_real_lds<V, PPT, kRealR2cHalf>; break;
+    case kRealC2rHalf: k = k_real_lds<V, PPT, kRealC2rHalf>; break;
+    case kRealR2cFull: k = k_real_lds<V, PPT, kRealR2cFull>; break;
+    case kRealC2rFull: k = k_real_lds<V, PPT, kRealC2rFull>; break;
+    default: return hipErrorInvalidValue;
+  }
+  if (g.lds_bytes > 65536)
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds_bytes);
+  const unsigned grid = (unsigned)((a.rows + g.per_block - 1) / g.per_block);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(g.threads * g.per_block), g.lds_bytes, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, hipStream_t s) {
+  if (prec == kF64) {
+    const auto& a = *static_cast<const RealArgs<double2>*>(args);
+    if (a.rows == 0) return hipSuccess;
+    return g.ppt <= 8 ? go_real<double2, 8>(mode, a, g, s) : go_real<double2, 16>(mode, a, g, s);
+  }
+  const auto& a = *static_cast<const RealArgs<float2>*>(args);
+  if (a.rows == 0) return hipSuccess;
+  return g.ppt <= 16 ? go_real<float2, 16>(mode, a, g, s) : go_real<float2, 32>(mode, a, g, s);
+}
+
+static inline dim3 grid_of(long long count) { return dim3((unsigned)((count + 255) / 256)); }
+
+template <typename V>
+static hipError_t go_real_pass(int op, const void* src, void* dst, long long rows, int n, const void* w,
+                               double scale, hipStream_t s) {
+  const int h = n / 2;
+  switch (op) {
+    case kRealOpPost:
+      hipLaunchKernelGGL(k_real_post<V>, grid_of(rows * (h + 1)), dim3(256), 0, s, (const V*)src, (V*)dst,
+                         rows, h, (const V*)w, (S<V>)scale);
+      break;
+    case kRealOpPre:
+      hipLaunchKernelGGL(k_real_pre<V>, grid_of(rows * h), dim3(256), 0, s, (const V*)src, (V*)dst, rows, h,
+                         (const V*)w);
+      break;
+    case kRealOpExpand:
+      hipLaunchKernelGGL(k_real_expand<V>, grid_of(rows * n), dim3(256), 0, s, (const S<V>*)src, (V*)dst,
+                         rows * n);
+      break;
+    case kRealOpTruncate:
+      hipLaunchKernelGGL(k_real_truncate<V>, grid_of(rows * (n / 2 + 1)), dim3(256), 0, s, (const V*)src,
+                         (V*)dst, rows, n, (S<V>)scale);
+      break;
+    case kRealOpHermExpand:
+      hipLaunchKernelGGL(k_herm_expand<V>, grid_of(rows * n), dim3(256), 0, s, (const V*)src, (V*)dst, rows, n);
+      break;
+    case kRealOpRealPart:
+      hipLaunchKernelGGL(k_real_part<V>, grid_of(rows * n), dim3(256), 0, s, (const V*)src, (S<V>*)dst,
+                         rows * n, (S<V>)scale);
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_real_pass(int prec, int op, const void* src, void* dst, long long rows, int n,
+                            const void* w, double scale, hipStream_t s) {
+  if (rows == 0) return hipSuccess;
+  if (prec == kF64) return go_real_pass<double2>(op, src, dst, rows, n, w, scale, s);
+  return go_real_pass<float2>(op, src, dst, rows, n, w, scale, s);
+}
+
+}  // namespace fyfft

@@ -114,6 +114,26 @@ typedef struct fyfft_plan fyfft_plan;
 fyfft_status fyfft_plan_create(fyfft_plan** plan, size_t n, size_t batch,
                                fyfft_precision precision);
 
+/* Real-input / real-output plans (no counterpart in the reference, which
+ * has complex<float> transforms only; callers there pass real data as a
+ * complex row with zero imaginary parts).  `batch` rows of real length n:
+ *   fyfft_execute(plan, x, X, FYFFT_FORWARD, stream): R2C, x is batch x n
+ *     reals, X is batch x (n/2 + 1) complex, X[k] = sum_j x[j] e^{-2 pi i jk/n};
+ *   fyfft_execute(plan, X, x, FYFFT_INVERSE, stream): C2R, the inverse of a
+ *     Hermitian spectrum given by its bins 0..n/2, scaled by 1/n (the
+ *     imaginary parts of bin 0, and of bin n/2 for even n, are ignored);
+ *   FYFFT_ADJOINT returns FYFFT_UNSUPPORTED.
+ * Element type: float/double for the real side, fyfft's complex layout for
+ * the spectrum.  In and out must not overlap.  Even n runs through one
+ * complex transform of length n/2 (half the HBM bytes of a C2C transform),
+ * odd n through one of length n; lengths whose transform fits one
+ * workgroup's LDS run as one fused kernel, others as the planned complex
+ * transform plus elementwise passes.  fyfft_plan_elements reports batch*n;
+ * fyfft_execute_host copies batch*n reals one way and batch*(n/2+1)
+ * complex values the other. */
+fyfft_status fyfft_plan_create_real(fyfft_plan** plan, size_t n, size_t batch,
+                                    fyfft_precision precision);
+
 /* Grouped ragged plans (no counterpart in the reference, which plans one
  * length at a time): `count` transforms of lengths lengths[0..count-1],
  * packed back to back (transform i starts at element sum_{j<i} lengths[j]),

@@ -108,3 +108,13 @@ def test_options_reserved_must_be_zero(fy):
     assert fy.lib().fyfft_plan_create_ex(ctypes.byref(h), 64, 1, 0, ctypes.byref(opts)) == 1
     opts = fy._Options(7)
     assert fy.lib().fyfft_plan_create_ex(ctypes.byref(h), 64, 1, 0, ctypes.byref(opts)) == 1
+
+
+def test_real_plan_argument_checks_without_gpu(fy):
+    lib = fy.lib()
+    h = ctypes.c_void_p()
+    assert lib.fyfft_plan_create_real(ctypes.byref(h), 0, 1, 0) == 2     # invalid length
+    assert lib.fyfft_plan_create_real(ctypes.byref(h), 64, 0, 0) == 1    # batch 0
+    assert lib.fyfft_plan_create_real(ctypes.byref(h), 64, 1, 2) == 3    # bf16 real: unsupported
+    assert lib.fyfft_plan_create_real(ctypes.byref(h), 64, 1, 9) == 1    # unknown precision
+    assert lib.fyfft_plan_create_real(None, 64, 1, 0) == 1
