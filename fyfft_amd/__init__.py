@@ -102,6 +102,8 @@ def _load() -> ctypes.CDLL:
         L.fyfft_plan_elements.argtypes = [vp, ctypes.POINTER(sz)]
         L.fyfft_plan_create_ex.argtypes = [ctypes.POINTER(vp), sz, sz, i, ctypes.POINTER(_Options)]
         L.fyfft_plan_create_real.argtypes = [ctypes.POINTER(vp), sz, sz, i]
+        L.fyfft_plan_create_many.argtypes = [ctypes.POINTER(vp), sz, sz, ctypes.POINTER(_Layout), i]
+        L.fyfft_plan_create_nd.argtypes = [ctypes.POINTER(vp), i, ctypes.POINTER(sz), sz, i]
         L.fyfft_wisdom_export.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.fyfft_wisdom_import.argtypes = [ctypes.c_char_p]
         L.fyfft_wisdom_forget.argtypes = []
@@ -114,6 +116,7 @@ def _load() -> ctypes.CDLL:
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
                      "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_plan_create_grouped",
                      "fyfft_plan_elements", "fyfft_plan_create_ex", "fyfft_plan_create_real",
+                     "fyfft_plan_create_many", "fyfft_plan_create_nd",
                      "fyfft_wisdom_export",
                      "fyfft_wisdom_import", "fyfft_wisdom_forget"):
             getattr(L, name).restype = i
@@ -323,6 +326,131 @@ def irfft_modes(modes, L: int, scale0: float, scale: float, stream=None):
                                      k, float(scale0), float(scale),
                                      ctypes.c_void_p(_stream_handle(stream, modes.device))), "irfft_modes")
     return y
+
+
+class _Layout(ctypes.Structure):
+    _fields_ = [("istride", ctypes.c_size_t), ("idist", ctypes.c_size_t),
+                ("ostride", ctypes.c_size_t), ("odist", ctypes.c_size_t)]
+
+
+class _StridedPlan(Plan):
+    """Common execution of strided plans: the tensors are flat device
+    buffers covering the input / output layouts."""
+
+    in_elems = out_elems = 0
+
+    def _check_buf(self, t, name, need):
+        import torch
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError(f"{name} must be a device tensor")
+        if _precision_of(t.dtype) != self.precision:
+            raise TypeError(f"{name} dtype {t.dtype} does not match the plan precision")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        if t.numel() < need:
+            raise ValueError(f"{name} has {t.numel()} elements, the layout spans {need}")
+
+    def execute(self, x, out=None, direction: int = FORWARD, stream=None):
+        import torch
+        self._check_buf(x, "input", self.in_elems)
+        if out is None:
+            out = torch.zeros(max(self.out_elems, 1), dtype=x.dtype, device=x.device)
+            if self.out_elems == x.numel():
+                out = out.view(x.shape)
+        self._check_buf(out, "output", self.out_elems)
+        self.execute_ptr(x.data_ptr(), out.data_ptr(), direction, _stream_handle(stream, x.device))
+        return out
+
+    def execute_host(self, x: np.ndarray, out: np.ndarray | None = None, direction: int = FORWARD):
+        dt = np.complex64 if self.precision == F32 else np.complex128
+        x = np.ascontiguousarray(x, dtype=dt)
+        if x.size != self.in_elems:
+            raise ValueError(f"host input has {x.size} elements, the layout spans {self.in_elems}")
+        out = np.zeros(self.out_elems, dt) if out is None else out
+        if out.dtype != dt or out.size != self.out_elems or not out.flags.c_contiguous:
+            raise ValueError("host output must be a contiguous buffer spanning the output layout")
+        _check(_load().fyfft_execute_host(self._h, ctypes.c_void_p(x.ctypes.data),
+                                          ctypes.c_void_p(out.ctypes.data), int(direction)),
+               "execute_host")
+        return out
+
+
+class ManyPlan(_StridedPlan):
+    """``howmany`` transforms of length ``n``; element j of transform b at
+    x[b*idist + j*istride] in the input and y[b*odist + j*ostride] in the
+    output (element units, flat buffers)."""
+
+    def __init__(self, n: int, howmany: int, istride: int = 1, idist: int | None = None,
+                 ostride: int | None = None, odist: int | None = None, precision: int = F32):
+        idist = n * istride if idist is None else idist
+        ostride = istride if ostride is None else ostride
+        odist = idist if odist is None else odist
+        self.n, self.batch, self.precision = int(n), int(howmany), int(precision)
+        self.layout = (int(istride), int(idist), int(ostride), int(odist))
+        self._h = ctypes.c_void_p()
+        lay = _Layout(*self.layout)
+        _check(_load().fyfft_plan_create_many(ctypes.byref(self._h), self.n, self.batch, ctypes.byref(lay),
+                                              self.precision),
+               f"plan_create_many(N={n}, howmany={howmany}, layout={self.layout})")
+        self.in_elems = (howmany - 1) * idist + (n - 1) * istride + 1
+        self.out_elems = (howmany - 1) * odist + (n - 1) * ostride + 1
+
+
+class NdPlan(_StridedPlan):
+    """``batch`` contiguous row-major arrays of shape ``dims`` (rank 1..3);
+    the transform runs over every axis (inverse scaled by 1/prod(dims))."""
+
+    def __init__(self, dims, batch: int = 1, precision: int = F32):
+        self.dims = tuple(int(d) for d in dims)
+        self.batch, self.precision = int(batch), int(precision)
+        self.n = int(np.prod(self.dims))
+        arr = (ctypes.c_size_t * len(self.dims))(*self.dims)
+        self._h = ctypes.c_void_p()
+        _check(_load().fyfft_plan_create_nd(ctypes.byref(self._h), len(self.dims), arr, self.batch,
+                                            self.precision),
+               f"plan_create_nd(dims={self.dims}, batch={batch})")
+        self.in_elems = self.out_elems = self.n * self.batch
+
+
+_nd_cache: dict = {}
+
+
+def fftn(x, ndim: int | None = None, inverse: bool = False, stream=None):
+    """C2C transform over the last ``ndim`` (1..3, default all up to 3) axes
+    of a complex64/complex128 device tensor."""
+    import torch
+    if x.dtype not in (torch.complex64, torch.complex128) or not x.is_cuda:
+        raise TypeError("fftn: complex64/complex128 device tensor expected")
+    ndim = min(x.dim(), 3) if ndim is None else int(ndim)
+    if not 1 <= ndim <= min(3, x.dim()):
+        raise ValueError("fftn: 1..3 transform axes")
+    x = x.contiguous()
+    dims = tuple(x.shape[x.dim() - ndim:])
+    batch = x.numel() // int(np.prod(dims)) if x.numel() else 0
+    key = (dims, batch, _precision_of(x.dtype), x.device.index or 0)
+    with _plan_cache_lock:
+        p = _nd_cache.get(key)
+        if p is None:
+            with torch.cuda.device(x.device):
+                p = NdPlan(dims, batch, _precision_of(x.dtype))
+            if len(_nd_cache) > 32:
+                _nd_cache.clear()
+            _nd_cache[key] = p
+    out = torch.empty_like(x)
+    p.execute_ptr(x.data_ptr(), out.data_ptr(), INVERSE if inverse else FORWARD, _stream_handle(stream, x.device))
+    return out
+
+
+def ifftn(x, ndim: int | None = None, stream=None):
+    return fftn(x, ndim, True, stream)
+
+
+def fft2(x, stream=None):
+    return fftn(x, 2, False, stream)
+
+
+def ifft2(x, stream=None):
+    return fftn(x, 2, True, stream)
 
 
 class RealPlan(Plan):

@@ -22,7 +22,25 @@ constexpr uint32_t kMagic = 0xF1FF7A11u;
 thread_local std::string g_last_error;
 
 enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7, kMfma = 8,
-            kReal = 9 };
+            kReal = 9, kStrided = 10 };
+
+// One strided pass: transforms of length n at (plane p, column c), element j
+// at p*pdist + c*cdist + j*stride, separately for input and output.
+struct Layout2 {
+  long long stride = 1, cdist = 0, pdist = 0;
+};
+enum PassMethod { kPmRows = 0, kPmColumns = 1, kPmGather = 2 };
+struct StridedPass {
+  size_t n = 0, ncols = 1, planes = 1;
+  Layout2 in, out;
+  int method = kPmRows;
+  fyfft_plan* sub = nullptr;  // contiguous plan of length n (rows, gather); owned by subs
+  LdsGeom g;                  // columns
+  dev::PassDesc pd{};
+  void* tw = nullptr;
+  int fixed_n = 0;
+  size_t chunk = 0;           // gather: transforms per workspace chunk
+};
 
 // Forms of a real (R2C / C2R) plan.
 enum RealForm { kRealFusedHalf = 0, kRealFusedFull = 1, kRealHalfPasses = 2, kRealFullPasses = 3 };
@@ -74,6 +92,11 @@ struct fyfft_plan {
   int* perm_out = nullptr;
   // real plans: complex sub-plan subs[0] of length n/2 (half forms) or n
   int real_form = -1;
+  // strided plans (many / multi-dimensional): passes run in order, the first
+  // from in to out, the others in place on out
+  std::vector<StridedPass> passes;
+  void* lws = nullptr;   // gather workspace
+  size_t in_ext = 0, out_ext = 0;  // elements spanned by the input / output layout
   void* rtw = nullptr;   // exp(-2*pi*i*k/n), k <= n/2 (half forms)
   void* rws = nullptr;   // rchunk rows of the sub-plan's length (multi-pass forms)
   size_t rchunk = 0;
@@ -579,16 +602,16 @@ static void fill_pass(FsArgs<V>& a, const LdsGeom& g, size_t len, const dev::Pas
 
 template <typename V>
 static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int dir, bool inv_tables,
-                                  double scale, size_t rows, hipStream_t s) {
+                                  double scale, size_t rows, size_t dist, hipStream_t s) {
   const size_t n = p->n, L = (p->kind == kFourStep) ? n : p->m;
   for (size_t b0 = 0; b0 < rows; b0 += p->chunk) {
     const size_t cb = std::min(p->chunk, rows - b0);
     FsArgs<V> a;
-    a.in = in + b0 * n;
-    a.out = out + b0 * n;
+    a.in = in + b0 * dist;
+    a.out = out + b0 * dist;
     a.tmp = (V*)p->tmp;
-    a.in_dist = (long long)n;
-    a.out_dist = (long long)n;
+    a.in_dist = (long long)dist;
+    a.out_dist = (long long)dist;
     a.tmp_dist = (long long)L;
     a.n1 = (int)p->n1;
     a.n2 = (int)p->n2;
@@ -622,12 +645,12 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
 
 template <typename V>
 static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool inv_tables,
-                                  double scale, size_t rows, hipStream_t s) {
+                                  double scale, size_t rows, size_t dist, hipStream_t s) {
   ConvArgs<V> a;
   a.in = in;
   a.out = out;
   a.batch = (long long)rows;
-  a.dist = (long long)p->n;
+  a.dist = (long long)dist;
   a.n_orig = (int)p->n;
   a.m = (int)p->m;
   a.threads = p->lds.geom.threads;
@@ -1062,8 +1085,16 @@ static fyfft_status exec_grouped(const fyfft_plan* p, const void* in, void* out,
 static fyfft_status exec_real(const fyfft_plan* p, const void* in, void* out, int direction, size_t rows,
                               hipStream_t s);
 
+static fyfft_status exec_strided(const fyfft_plan* p, const void* in, void* out, int direction, hipStream_t s);
+
+// dist: elements between consecutive transforms (0: the length n).  Only the
+// plain batched kinds accept dist != n.
 static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out, int direction, size_t rows,
-                                 hipStream_t s) {
+                                 hipStream_t s, size_t dist = 0) {
+  if (dist == 0) dist = p->n;
+  if (dist != p->n && p->kind != kTrivial && p->kind != kLds && p->kind != kFourStep && p->kind != kRaderFs &&
+      p->kind != kBlueFs && p->kind != kRaderLds && p->kind != kBlueLds)
+    return FYFFT_UNSUPPORTED;
   if (!in || !out) return FYFFT_INVALID_ARGUMENT;
   if (direction != FYFFT_FORWARD && direction != FYFFT_INVERSE && direction != FYFFT_ADJOINT)
     return FYFFT_INVALID_ARGUMENT;
@@ -1072,25 +1103,27 @@ static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out,
   const bool inv_tables = direction != FYFFT_FORWARD;
   switch (p->kind) {
     case kTrivial:
-      if (in != out)
-        HIP_TRY(hipMemcpyAsync(out, in, rows * elem_bytes(p->prec), hipMemcpyDeviceToDevice, s));
+      if (in != out) {
+        const size_t e = elem_bytes(p->prec);
+        HIP_TRY(hipMemcpy2DAsync(out, dist * e, in, dist * e, e, rows, hipMemcpyDeviceToDevice, s));
+      }
       return FYFFT_SUCCESS;
     case kLds:
-      HIP_TRY(launch_lds(p->prec, dir, p->lds, in, out, rows, p->n, scale, s));
+      HIP_TRY(launch_lds(p->prec, dir, p->lds, in, out, rows, dist, scale, s));
       return FYFFT_SUCCESS;
     case kFourStep:
     case kRaderFs:
     case kBlueFs: {
       const int d = dir == kForward ? -1 : +1;
       if (p->prec == kF64)
-        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, rows, s);
-      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, rows, s);
+        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, rows, dist, s);
+      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, rows, dist, s);
     }
     case kRaderLds:
     case kBlueLds:
       if (p->prec == kF64)
-        return exec_conv_lds<double2>(p, (const double2*)in, (double2*)out, inv_tables, scale, rows, s);
-      return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, rows, s);
+        return exec_conv_lds<double2>(p, (const double2*)in, (double2*)out, inv_tables, scale, rows, dist, s);
+      return exec_conv_lds<float2>(p, (const float2*)in, (float2*)out, inv_tables, scale, rows, dist, s);
     case kGrouped:
       return exec_grouped(p, in, out, direction, s);
     case kMfma:
@@ -1099,6 +1132,8 @@ static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out,
       return FYFFT_SUCCESS;
     case kReal:
       return exec_real(p, in, out, direction, rows, s);
+    case kStrided:
+      return exec_strided(p, in, out, direction, s);
   }
   return FYFFT_INVALID_PLAN;
 }
@@ -1234,6 +1269,236 @@ static fyfft_status create_real_impl(fyfft_plan** out, size_t n, size_t batch, f
   return FYFFT_SUCCESS;
 }
 
+// ------------------------------------------------------------ strided plans
+// Plans one pass: rows when each transform is contiguous and the transforms
+// sit at one distance; columns when transforms are adjacent (cdist 1) and
+// fit one workgroup's LDS; otherwise gather into a contiguous workspace,
+// transform there and scatter back.
+static fyfft_status plan_strided_pass(fyfft_plan* p, StridedPass& sp, fyfft_precision precision,
+                                      size_t* ws_elems) {
+  const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
+  const size_t count = sp.ncols * sp.planes;
+  const bool rows = sp.in.stride == 1 && sp.out.stride == 1 && sp.in.cdist == sp.out.cdist &&
+                    sp.in.cdist >= (long long)sp.n &&
+                    (sp.planes == 1 || (sp.in.pdist == (long long)sp.ncols * sp.in.cdist &&
+                                        sp.out.pdist == (long long)sp.ncols * sp.out.cdist));
+  const bool cols = sp.in.cdist == 1 && sp.out.cdist == 1 && sp.n >= 2 && is_smooth7(sp.n) &&
+                    fs_geometry(sp.n, elem, cap, &sp.g);
+  fyfft_plan* sub = nullptr;
+  fyfft_status st;
+  if (rows) {
+    sp.method = kPmRows;
+    st = create_plan_impl(&sub, sp.n, count, precision, FYFFT_FORM_AUTO);
+    if (st) return st;
+    p->subs.push_back(sub);
+    sp.sub = sub;
+    return FYFFT_SUCCESS;
+  }
+  if (cols) {
+    sp.method = kPmColumns;
+    st = make_pass_table(p, sp.n, &sp.pd, &sp.tw);
+    if (st) return st;
+    sp.fixed_n = find_fs_fixed(p->prec, sp.pd, (int)sp.n, sp.g.threads);
+    return FYFFT_SUCCESS;
+  }
+  sp.method = kPmGather;
+  sp.chunk = std::min(count, std::max<size_t>(1, kWorkspaceCap / (sp.n * elem)));
+  st = create_plan_impl(&sub, sp.n, sp.chunk, precision, FYFFT_FORM_AUTO);
+  if (st) return st;
+  p->subs.push_back(sub);
+  sp.sub = sub;
+  *ws_elems = std::max(*ws_elems, sp.chunk * sp.n);
+  return FYFFT_SUCCESS;
+}
+
+template <typename V>
+static fyfft_status exec_columns(const fyfft_plan* p, const StridedPass& sp, const void* in, void* out,
+                                 int direction, hipStream_t s) {
+  ColArgs<V> a;
+  a.in = (const V*)in;
+  a.out = (V*)out;
+  a.istride = sp.in.stride;
+  a.ostride = sp.out.stride;
+  a.ipdist = sp.in.pdist;
+  a.opdist = sp.out.pdist;
+  a.n = (int)sp.n;
+  a.ncols = (int)sp.ncols;
+  a.log2c = ilog2_exact(sp.g.per_block);
+  a.threads = sp.g.threads;
+  a.ps = sp.g.pad_shift;
+  a.stride = dev::lds_len((int)sp.n, sp.g.pad_shift);
+  a.pd = sp.pd;
+  a.tw = (const V*)sp.tw;
+  a.scale = (dev::S<V>)(direction == FYFFT_INVERSE ? 1.0 / (double)sp.n : 1.0);
+  HIP_TRY(launch_columns(p->prec, direction == FYFFT_FORWARD ? -1 : +1, &a, sp.planes, sp.g, sp.fixed_n, s));
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status exec_strided(const fyfft_plan* p, const void* in, void* out, int direction, hipStream_t s) {
+  for (size_t i = 0; i < p->passes.size(); ++i) {
+    const StridedPass& sp = p->passes[i];
+    const void* src = i == 0 ? in : out;
+    fyfft_status st = FYFFT_SUCCESS;
+    if (sp.method == kPmRows) {
+      st = execute_rows(sp.sub, src, out, direction, sp.ncols * sp.planes, s, (size_t)sp.in.cdist);
+    } else if (sp.method == kPmColumns) {
+      st = p->prec == kF64 ? exec_columns<double2>(p, sp, src, out, direction, s)
+                           : exec_columns<float2>(p, sp, src, out, direction, s);
+    } else {
+      const size_t count = sp.ncols * sp.planes;
+      for (size_t t0 = 0; t0 < count && !st; t0 += sp.chunk) {
+        const long long cnt = (long long)std::min(sp.chunk, count - t0);
+        HIP_TRY(launch_gather(p->prec, false, src, p->lws, (long long)sp.n, sp.in.stride, sp.in.cdist,
+                              (long long)sp.ncols, sp.in.pdist, (long long)t0, cnt, s));
+        st = execute_rows(sp.sub, p->lws, p->lws, direction, (size_t)cnt, s);
+        if (!st)
+          HIP_TRY(launch_gather(p->prec, true, out, p->lws, (long long)sp.n, sp.out.stride, sp.out.cdist,
+                                (long long)sp.ncols, sp.out.pdist, (long long)t0, cnt, s));
+      }
+    }
+    if (st) return st;
+  }
+  return FYFFT_SUCCESS;
+}
+
+static const char* method_name(int m) {
+  return m == kPmRows ? "rows" : (m == kPmColumns ? "columns" : "gather");
+}
+
+static fyfft_status finish_strided(fyfft_plan* p, size_t ws_elems, const std::string& head) {
+  fyfft_status st = FYFFT_SUCCESS;
+  if (ws_elems) st = p->alloc(ws_elems * elem_bytes(p->prec), &p->lws);
+  if (!st) {
+    const hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) st = from_hip(e, "strided plan tables");
+  }
+  if (st) return st;
+  p->desc = head;
+  for (const StridedPass& sp : p->passes)
+    p->desc += std::string(" | n=") + std::to_string(sp.n) + " " + method_name(sp.method) +
+               (sp.sub ? " [" + sp.sub->desc + "]" : " [C=" + std::to_string(sp.g.per_block) + " " +
+                                                          radix_string(sp.pd) + "]");
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status new_strided_plan(fyfft_plan** out, fyfft_precision precision, fyfft_plan** pp) {
+  if (!out) return FYFFT_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (precision == FYFFT_BF16 || precision == FYFFT_F16) return FYFFT_UNSUPPORTED;
+  if (precision != FYFFT_F32 && precision != FYFFT_F64) return FYFFT_INVALID_ARGUMENT;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  fyfft_plan* p = new (std::nothrow) fyfft_plan();
+  if (!p) return FYFFT_ALLOC_FAILED;
+  p->kind = kStrided;
+  p->prec = precision == FYFFT_F64 ? kF64 : kF32;
+  p->device = dev;
+  *pp = p;
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status create_many_impl(fyfft_plan** out, size_t n, size_t howmany, const fyfft_layout* L,
+                                     fyfft_precision precision) {
+  if (!out || !L || howmany == 0) return FYFFT_INVALID_ARGUMENT;
+  if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+  if (L->istride == 0 || L->ostride == 0 || (howmany > 1 && (L->idist == 0 || L->odist == 0)))
+    return FYFFT_INVALID_ARGUMENT;
+  fyfft_plan* p = nullptr;
+  fyfft_status st = new_strided_plan(out, precision, &p);
+  if (st) return st;
+  p->n = n;
+  p->batch = howmany;
+  p->strategy = strategy_for(n);
+  StridedPass sp;
+  sp.n = n;
+  sp.ncols = howmany;
+  sp.planes = 1;
+  sp.in.stride = (long long)L->istride;
+  sp.in.cdist = (long long)(howmany > 1 ? L->idist : n * L->istride);
+  sp.out.stride = (long long)L->ostride;
+  sp.out.cdist = (long long)(howmany > 1 ? L->odist : n * L->ostride);
+  sp.in.pdist = sp.in.cdist * (long long)howmany;
+  sp.out.pdist = sp.out.cdist * (long long)howmany;
+  p->in_ext = (size_t)((long long)(howmany - 1) * sp.in.cdist + (long long)(n - 1) * sp.in.stride + 1);
+  p->out_ext = (size_t)((long long)(howmany - 1) * sp.out.cdist + (long long)(n - 1) * sp.out.stride + 1);
+  size_t ws = 0;
+  p->passes.push_back(sp);
+  st = plan_strided_pass(p, p->passes.back(), precision, &ws);
+  if (!st)
+    st = finish_strided(p, ws, "many N=" + std::to_string(n) + " howmany=" + std::to_string(howmany) +
+                                   " in(" + std::to_string(L->istride) + "," + std::to_string(L->idist) +
+                                   ") out(" + std::to_string(L->ostride) + "," + std::to_string(L->odist) + ")");
+  if (st) {
+    delete p;
+    return st;
+  }
+  *out = p;
+  return FYFFT_SUCCESS;
+}
+
+static fyfft_status create_nd_impl(fyfft_plan** out, int rank, const size_t* dims, size_t batch,
+                                   fyfft_precision precision) {
+  if (!out || !dims || rank < 1 || rank > 3 || batch == 0) return FYFFT_INVALID_ARGUMENT;
+  size_t vol = 1;
+  for (int d = 0; d < rank; ++d) {
+    if (dims[d] < 1 || dims[d] > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+    vol *= dims[d];
+  }
+  if (vol > ((size_t)1 << 40) / batch) return FYFFT_INVALID_LENGTH;
+  fyfft_plan* p = nullptr;
+  fyfft_status st = new_strided_plan(out, precision, &p);
+  if (st) return st;
+  p->n = vol;
+  p->batch = batch;
+  p->strategy = strategy_for(dims[rank - 1]);
+  p->in_ext = p->out_ext = vol * batch;
+  size_t ws = 0;
+  std::string head = "nd dims=";
+  for (int d = 0; d < rank; ++d) head += (d ? "x" : "") + std::to_string(dims[d]);
+  head += " batch=" + std::to_string(batch);
+  // innermost axis first (contiguous rows), then the strided axes
+  for (int d = rank - 1; d >= 0 && !st; --d) {
+    if (dims[d] == 1) continue;
+    size_t inner = 1, outer = batch;
+    for (int e = d + 1; e < rank; ++e) inner *= dims[e];
+    for (int e = 0; e < d; ++e) outer *= dims[e];
+    StridedPass sp;
+    sp.n = dims[d];
+    if (inner == 1) {  // contiguous rows
+      sp.ncols = 1;
+      sp.planes = outer;
+      sp.in.stride = 1;
+      sp.in.cdist = (long long)dims[d];
+      sp.in.pdist = (long long)dims[d];
+    } else {
+      sp.ncols = inner;
+      sp.planes = outer;
+      sp.in.stride = (long long)inner;
+      sp.in.cdist = 1;
+      sp.in.pdist = (long long)(inner * dims[d]);
+    }
+    sp.out = sp.in;
+    p->passes.push_back(sp);
+    st = plan_strided_pass(p, p->passes.back(), precision, &ws);
+  }
+  if (!st && p->passes.empty()) {  // every dimension 1: a copy
+    StridedPass sp;
+    sp.n = 1;
+    sp.planes = batch;
+    sp.in.cdist = sp.in.pdist = 1;
+    sp.out = sp.in;
+    p->passes.push_back(sp);
+    st = plan_strided_pass(p, p->passes.back(), precision, &ws);
+  }
+  if (!st) st = finish_strided(p, ws, head);
+  if (st) {
+    delete p;
+    return st;
+  }
+  *out = p;
+  return FYFFT_SUCCESS;
+}
+
 extern "C" {
 
 const char* fyfft_status_string(fyfft_status s) {
@@ -1286,6 +1551,16 @@ fyfft_status fyfft_plan_create_ex(fyfft_plan** out, size_t n, size_t batch, fyff
 
 fyfft_status fyfft_plan_create_real(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision) {
   return create_real_impl(out, n, batch, precision);
+}
+
+fyfft_status fyfft_plan_create_many(fyfft_plan** out, size_t n, size_t howmany, const fyfft_layout* layout,
+                                    fyfft_precision precision) {
+  return create_many_impl(out, n, howmany, layout, precision);
+}
+
+fyfft_status fyfft_plan_create_nd(fyfft_plan** out, int rank, const size_t* dims, size_t batch,
+                                  fyfft_precision precision) {
+  return create_nd_impl(out, rank, dims, batch, precision);
 }
 
 fyfft_status fyfft_wisdom_export(char* buf, size_t len, size_t* needed) {
@@ -1357,7 +1632,7 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
 fyfft_status fyfft_plan_elements(const fyfft_plan* p, size_t* elements) {
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   if (!elements) return FYFFT_INVALID_ARGUMENT;
-  *elements = p->kind == kGrouped ? p->total : p->n * p->batch;
+  *elements = p->kind == kGrouped ? p->total : (p->kind == kStrided ? p->in_ext : p->n * p->batch);
   return FYFFT_SUCCESS;
 }
 
@@ -1393,11 +1668,16 @@ fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, 
     const size_t real = p->batch * p->n * (elem / 2), half = p->batch * (p->n / 2 + 1) * elem;
     in_bytes = direction == FYFFT_FORWARD ? real : half;
     out_bytes = direction == FYFFT_FORWARD ? half : real;
+  } else if (p->kind == kStrided) {
+    in_bytes = p->in_ext * elem;
+    out_bytes = p->out_ext * elem;
   } else {
     in_bytes = out_bytes = (p->kind == kGrouped ? p->total : p->n * p->batch) * elem;
   }
-  // in place for complex plans; real plans stage input and output apart
-  const size_t out_off = p->kind == kReal ? (in_bytes + 255) / 256 * 256 : 0;
+  // in place for batched complex plans; real and strided plans stage input
+  // and output apart (the output starts as zeros where a layout has gaps)
+  const bool apart = p->kind == kReal || p->kind == kStrided;
+  const size_t out_off = apart ? (in_bytes + 255) / 256 * 256 : 0;
   const size_t bytes = std::max(in_bytes, out_off + out_bytes);
   if (p->d_io_bytes < bytes) {
     if (p->d_io) (void)hipFree(p->d_io);
@@ -1408,6 +1688,7 @@ fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, 
   }
   void* dout = (char*)p->d_io + out_off;
   HIP_TRY(hipMemcpy(p->d_io, in, in_bytes, hipMemcpyHostToDevice));
+  if (apart) HIP_TRY(hipMemcpy(dout, out, out_bytes, hipMemcpyHostToDevice));
   fyfft_status st = fyfft_execute(p, p->d_io, dout, direction, nullptr);
   if (st) return st;
   HIP_TRY(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));

@@ -143,6 +143,34 @@ template <typename V> struct SpecArgs {
 };
 hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGeom& g, hipStream_t s);
 
+// --- kernels_layout.hip
+// Strided transforms.  Element j of transform (plane p, column c) sits at
+// p*pdist + c + j*stride (columns adjacent in memory, as the non-contiguous
+// axes of a row-major multi-dimensional array): a workgroup transforms
+// 1 << log2c adjacent columns of one plane in LDS, reading and writing rows
+// of adjacent elements.  In place when in == out with equal layouts.
+template <typename V> struct ColArgs {
+  const V* in = nullptr;
+  V* out = nullptr;
+  long long istride = 0, ostride = 0, ipdist = 0, opdist = 0;
+  int n = 0;           // transform length
+  int ncols = 0;       // columns per plane
+  int log2c = 0;       // columns per workgroup = 1 << log2c
+  int threads = 0;     // threads per column transform
+  int ps = 4, stride = 0;
+  dev::PassDesc pd{};
+  const V* tw = nullptr;
+  dev::S<V> scale = 1;
+};
+hipError_t launch_columns(int prec, int dir, const void* args, size_t planes, const LdsGeom& g, int fixed_n,
+                          hipStream_t s);
+// Gather transforms t0 .. t0+count-1 (t = p*ncols + c) of a general strided
+// layout into contiguous rows of ws (scatter: the reverse).  Element j of
+// transform (p, c) at p*pdist + c*cdist + j*stride.
+hipError_t launch_gather(int prec, bool scatter, const void* strided, void* ws, long long n,
+                         long long stride, long long cdist, long long ncols, long long pdist,
+                         long long t0, long long count, hipStream_t s);
+
 // --- kernels_real.hip
 // Fused LDS-resident real transforms (one row per workgroup slot):
 //   kRealR2cHalf / kRealC2rHalf: even n through one complex FFT of n/2,

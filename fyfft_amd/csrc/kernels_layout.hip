@@ -1,0 +1,130 @@
+// Strided layouts: the column kernel of multi-dimensional and interleaved
+// batched transforms, and the gather/scatter pair that serves every other
+// layout through a contiguous workspace.
+#include "bodies.hpp"
+
+namespace fyfft {
+using namespace dev;
+
+// Columns [c0, c0 + C) of one plane: C*n elements read as n rows of C
+// adjacent values, C column transforms in LDS, written back the same way.
+template <typename V, typename F, int DIR>
+__device__ __forceinline__ void body_columns(const ColArgs<V>& a, const V* x, V* y, int c0, V* lds_in) {
+  constexpr int P = F::kPpt;
+  V* lds_all = as_lds(lds_in);
+  const int N = a.n, lc = a.log2c, C = 1 << lc;
+  const int slot = threadIdx.x / a.threads;
+  const int tid = threadIdx.x - slot * a.threads;
+  const int ps = a.ps, stride = a.stride;
+  const int total = N << lc;
+  const int ncols = min(C, a.ncols - c0);
+  staged<P>(threadIdx.x, blockDim.x, total,
+            [&](int i) {
+              const int j = i >> lc, c = i & (C - 1);
+              return gload(x + (long long)j * a.istride + c0 + (c < ncols ? c : 0));
+            },
+            [&](int i, V v) {
+              const int j = i >> lc, c = i & (C - 1);
+              if (c < ncols) lds_all[c * stride + j + (j >> ps)] = v;
+            });
+  __syncthreads();
+  const bool active = slot < ncols;
+  F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N, a.threads, tid, a.pd, a.tw, active, ps);
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int k = i >> lc, c = i & (C - 1);
+    if (c < ncols)
+      gstore(y + (long long)k * a.ostride + c0 + c, cscale(lds_all[c * stride + k + (k >> ps)], a.scale));
+  }
+}
+
+template <typename V, typename F, int DIR>
+__global__ __launch_bounds__(1024) void k_columns(ColArgs<V> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int groups = (a.ncols + (1 << a.log2c) - 1) >> a.log2c;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const long long p = lb / groups;
+  const int c0 = (int)(lb - p * groups) << a.log2c;
+  body_columns<V, F, DIR>(a, a.in + p * a.ipdist, a.out + p * a.opdist, c0, reinterpret_cast<V*>(smem_raw));
+}
+
+template <typename V, typename F>
+static hipError_t go_columns(int dir, const ColArgs<V>& a, size_t planes, size_t lds, int threads,
+                             hipStream_t s) {
+  auto k = dir < 0 ? k_columns<V, F, -1> : k_columns<V, F, +1>;
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const size_t groups = ((size_t)a.ncols + (1 << a.log2c) - 1) >> a.log2c;
+  hipLaunchKernelGGL(k, dim3((unsigned)(groups * planes)), dim3(threads), lds, s, a);
+  return hipGetLastError();
+}
+
+template <typename V, int PPT>
+static hipError_t columns_len(int fixed_n, int dir, const ColArgs<V>& a, size_t planes, size_t lds, int threads,
+                              hipStream_t s) {
+  constexpr int P = sizeof(V) == 8 ? 16 : 8;  // points per thread of the fixed forms
+  switch (fixed_n) {
+    case 64: return go_columns<V, FixedFft<V, 64, 64 / P, 8, 8>>(dir, a, planes, lds, threads, s);
+    case 128: return go_columns<V, FixedFft<V, 128, 128 / P, 16, 8>>(dir, a, planes, lds, threads, s);
+    case 256: return go_columns<V, FixedFft<V, 256, 256 / P, 16, 16>>(dir, a, planes, lds, threads, s);
+    case 512: return go_columns<V, FixedFft<V, 512, 512 / P, 8, 8, 8>>(dir, a, planes, lds, threads, s);
+    case 1024: return go_columns<V, FixedFft<V, 1024, 1024 / P, 16, 8, 8>>(dir, a, planes, lds, threads, s);
+    case 2048: return go_columns<V, FixedFft<V, 2048, 2048 / P, 16, 16, 8>>(dir, a, planes, lds, threads, s);
+    case 4096:
+      if constexpr (sizeof(V) == 8)
+        return go_columns<V, FixedFft<V, 4096, 4096 / P, 16, 16, 16>>(dir, a, planes, lds, threads, s);
+      break;
+  }
+  return go_columns<V, GenericFft<V, PPT>>(dir, a, planes, lds, threads, s);
+}
+
+hipError_t launch_columns(int prec, int dir, const void* args, size_t planes, const LdsGeom& g, int fixed_n,
+                          hipStream_t s) {
+  if (planes == 0) return hipSuccess;
+  const int threads = g.threads * g.per_block;
+  if (prec == kF64) {
+    const auto& a = *static_cast<const ColArgs<double2>*>(args);
+    return g.ppt <= 8 ? columns_len<double2, 8>(fixed_n, dir, a, planes, g.lds_bytes, threads, s)
+                      : columns_len<double2, 16>(fixed_n, dir, a, planes, g.lds_bytes, threads, s);
+  }
+  const auto& a = *static_cast<const ColArgs<float2>*>(args);
+  return g.ppt <= 16 ? columns_len<float2, 16>(fixed_n, dir, a, planes, g.lds_bytes, threads, s)
+                     : columns_len<float2, 32>(fixed_n, dir, a, planes, g.lds_bytes, threads, s);
+}
+
+// ------------------------------------------------------------ gather / scatter
+template <typename V, bool SCATTER>
+__global__ void k_gather(const V* src, V* dst, long long n, long long stride, long long cdist, long long ncols,
+                         long long pdist, long long t0, long long count) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count * n) return;
+  const long long r = e / n, j = e - r * n;  // row r of the workspace, element j
+  const long long t = t0 + r;
+  const long long p = t / ncols, c = t - p * ncols;
+  const long long g = p * pdist + c * cdist + j * stride;
+  if (SCATTER) dst[g] = src[e];
+  else dst[e] = src[g];
+}
+
+hipError_t launch_gather(int prec, bool scatter, const void* strided, void* ws, long long n, long long stride,
+                         long long cdist, long long ncols, long long pdist, long long t0, long long count,
+                         hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const dim3 grid((unsigned)((count * n + 255) / 256)), block(256);
+  if (prec == kF64) {
+    if (scatter)
+      hipLaunchKernelGGL((k_gather<double2, true>), grid, block, 0, s, (const double2*)ws, (double2*)strided,
+                         n, stride, cdist, ncols, pdist, t0, count);
+    else
+      hipLaunchKernelGGL((k_gather<double2, false>), grid, block, 0, s, (const double2*)strided, (double2*)ws,
+                         n, stride, cdist, ncols, pdist, t0, count);
+  } else {
+    if (scatter)
+      hipLaunchKernelGGL((k_gather<float2, true>), grid, block, 0, s, (const float2*)ws, (float2*)strided,
+                         n, stride, cdist, ncols, pdist, t0, count);
+    else
+      hipLaunchKernelGGL((k_gather<float2, false>), grid, block, 0, s, (const float2*)strided, (float2*)ws,
+                         n, stride, cdist, ncols, pdist, t0, count);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fyfft

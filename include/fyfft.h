@@ -114,6 +114,32 @@ typedef struct fyfft_plan fyfft_plan;
 fyfft_status fyfft_plan_create(fyfft_plan** plan, size_t n, size_t batch,
                                fyfft_precision precision);
 
+/* Strided batches (no counterpart in the reference, which transforms one
+ * contiguous array per call): `howmany` transforms of length n; element j
+ * of transform b is read from in[b*idist + j*istride] and written to
+ * out[b*odist + j*ostride] (element units).  Contiguous rows at one
+ * distance run the batched kernels directly; interleaved transforms
+ * (idist = odist = 1) run a column kernel that moves rows of adjacent
+ * elements through LDS; every other layout goes through a contiguous
+ * workspace (gather, transform, scatter).  In place requires identical
+ * input and output layouts.  fyfft_plan_elements reports the elements the
+ * input layout spans; fyfft_execute_host copies the spans of both layouts
+ * (output elements outside the layout keep the values of `out`). */
+typedef struct fyfft_layout {
+  size_t istride, idist, ostride, odist;
+} fyfft_layout;
+fyfft_status fyfft_plan_create_many(fyfft_plan** plan, size_t n, size_t howmany,
+                                    const fyfft_layout* layout, fyfft_precision precision);
+
+/* Multi-dimensional C2C (no counterpart in the reference): `batch`
+ * contiguous row-major arrays of rank 1..3 with dims[0] the slowest axis.
+ * Executed as one batched 1-D pass per axis, innermost first: the innermost
+ * axis as contiguous rows, the others by the column kernel when the axis
+ * length fits one workgroup's LDS (gather/transform/scatter otherwise).
+ * FYFFT_INVERSE scales by 1/prod(dims).  In place allowed. */
+fyfft_status fyfft_plan_create_nd(fyfft_plan** plan, int rank, const size_t* dims, size_t batch,
+                                  fyfft_precision precision);
+
 /* Real-input / real-output plans (no counterpart in the reference, which
  * has complex<float> transforms only; callers there pass real data as a
  * complex row with zero imaginary parts).  `batch` rows of real length n:

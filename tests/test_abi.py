@@ -118,3 +118,21 @@ def test_real_plan_argument_checks_without_gpu(fy):
     assert lib.fyfft_plan_create_real(ctypes.byref(h), 64, 1, 2) == 3    # bf16 real: unsupported
     assert lib.fyfft_plan_create_real(ctypes.byref(h), 64, 1, 9) == 1    # unknown precision
     assert lib.fyfft_plan_create_real(None, 64, 1, 0) == 1
+
+
+def test_strided_plan_argument_checks_without_gpu(fy):
+    lib = fy.lib()
+    h = ctypes.c_void_p()
+    lay = fy._Layout(1, 64, 1, 64)
+    assert lib.fyfft_plan_create_many(ctypes.byref(h), 64, 0, ctypes.byref(lay), 0) == 1   # howmany 0
+    assert lib.fyfft_plan_create_many(ctypes.byref(h), 0, 4, ctypes.byref(lay), 0) == 2    # length 0
+    assert lib.fyfft_plan_create_many(ctypes.byref(h), 64, 4, None, 0) == 1                # no layout
+    zero = fy._Layout(0, 64, 1, 64)
+    assert lib.fyfft_plan_create_many(ctypes.byref(h), 64, 4, ctypes.byref(zero), 0) == 1  # stride 0
+    assert lib.fyfft_plan_create_many(ctypes.byref(h), 64, 4, ctypes.byref(lay), 2) == 3   # bf16
+    dims = (ctypes.c_size_t * 3)(4, 4, 4)
+    assert lib.fyfft_plan_create_nd(ctypes.byref(h), 0, dims, 1, 0) == 1                   # rank 0
+    assert lib.fyfft_plan_create_nd(ctypes.byref(h), 4, dims, 1, 0) == 1                   # rank 4
+    assert lib.fyfft_plan_create_nd(ctypes.byref(h), 3, dims, 0, 0) == 1                   # batch 0
+    bad = (ctypes.c_size_t * 2)(4, 0)
+    assert lib.fyfft_plan_create_nd(ctypes.byref(h), 2, bad, 1, 0) == 2                   # dim 0
