@@ -14,6 +14,7 @@ Other configurations (one JSON line each, same fields where they apply):
   --workload spectral  spectral convolution [256, 64, 4096], 512 modes, forward + backward
   --workload lowprec   BF16 and FP16 storage, N=4096 x 65536, radix-16 stages on MFMA
   --workload real      real-input R2C and real-output C2R, N=8192 x 65536 and N=2^20 x 256 (float)
+  --workload nd        2-D 8192 x 8192 and 3-D 512^3 complex<float>, and a strided (column) batch
 """
 from __future__ import annotations
 
@@ -42,7 +43,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=65536,
                     help="transforms in the CPU baseline sample (65536 x 4096 points: "
                          "about 20 s of CPU time over 16 threads)")
-    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral", "lowprec", "real"],
+    ap.add_argument("--workload", choices=["c2c", "mixed", "prime", "ragged", "spectral", "lowprec", "real", "nd"],
                     default="c2c")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="3 FFT launches and 3 plain copies of the same size, for rocprofv3 --pmc")
@@ -196,6 +197,62 @@ def bench_real(a):
                           "traffic": None})
 
 
+def bench_nd(a):
+    """Multi-dimensional and strided plans; algorithmic bytes 16 per complex
+    element (read once, written once), as for the 1-D plans."""
+    import numpy as np
+    import torch
+    import fyfft_amd as fy
+    import oracle
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    rows, tot_flops, tot_ms, tot_bytes = [], 0.0, 0.0, 0.0
+    cases = [("2-D", (8192, 8192), 1), ("3-D", (512, 512, 512), 1), ("columns", (4096,), 65536)]
+    for label, dims, batch in cases:
+        g = torch.Generator(device=dev).manual_seed(0)
+        total = int(np.prod(dims)) * batch
+        x = torch.view_as_complex(torch.randn(total, 2, device=dev, generator=g))
+        y = torch.empty_like(x)
+        if label == "columns":
+            # 65536 interleaved transforms of length 4096: element j of transform b at b + j*65536
+            plan = fy.ManyPlan(dims[0], batch, batch, 1, batch, 1, fy.F32)
+        else:
+            plan = fy.NdPlan(dims, batch, fy.F32)
+        h = stream.cuda_stream
+        ms = time_steps(lambda: plan.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h), a.steps, a.warmup,
+                        stream)
+        if label == "columns":
+            cols = [0, batch - 1]
+            xs = x.view(dims[0], batch)[:, cols].T.cpu().numpy().astype(np.complex128)
+            err = oracle.rel_l2(y.view(dims[0], batch)[:, cols].T.cpu().numpy(), oracle.fft_rows(xs))
+        else:
+            # size-independent check: the inverse plan recovers the input
+            z = torch.empty_like(x)
+            plan.execute_ptr(y.data_ptr(), z.data_ptr(), fy.INVERSE, h)
+            torch.cuda.synchronize()
+            err = float(torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x))
+            del z
+        flops = 5.0 * total * math.log2(int(np.prod(dims)))
+        byt = 16.0 * total
+        rows.append({"case": label, "dims": list(dims), "batch": batch, "plan": plan.describe()[:300],
+                     "ms": round(ms, 4), "GFLOP/s": round(flops / ms / 1e6, 1),
+                     "hbm_frac": round(byt / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                     "check": "rel-L2 vs FP64 oracle (2 transforms)" if label == "columns"
+                              else "rel-L2 of inverse(forward(x)) vs x", "rel_l2": err})
+        tot_flops += flops
+        tot_ms += ms
+        tot_bytes += byt
+        plan.close()
+        del x, y
+        torch.cuda.empty_cache()
+    return line({"config": {"workload": "nd", "cases": rows}}, value=round(tot_flops / tot_ms / 1e6, 2),
+                steps=a.steps, warmup=a.warmup, ms_per_step=round(tot_ms, 4), dtype="f32",
+                roofline={"bound": "hbm", "achieved": round(tot_bytes / (tot_ms * 1e-3) / 1e9, 1),
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                          "traffic": None})
+
+
 def bench_ragged(a):
     import numpy as np
     import torch
@@ -339,6 +396,8 @@ def main():
             out = bench_lowprec(a)
         elif a.workload == "real":
             out = bench_real(a)
+        elif a.workload == "nd":
+            out = bench_nd(a)
         else:
             out = bench_spectral(a)
         print(json.dumps(out), flush=True)

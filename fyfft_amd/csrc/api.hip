@@ -1282,8 +1282,11 @@ static fyfft_status plan_strided_pass(fyfft_plan* p, StridedPass& sp, fyfft_prec
                     sp.in.cdist >= (long long)sp.n &&
                     (sp.planes == 1 || (sp.in.pdist == (long long)sp.ncols * sp.in.cdist &&
                                         sp.out.pdist == (long long)sp.ncols * sp.out.cdist));
+  // the column kernel needs >= 64-byte row segments (8 FP32 / 4 FP64
+  // columns per workgroup); longer columns go through the tiled transpose
+  // into contiguous rows and back
   const bool cols = sp.in.cdist == 1 && sp.out.cdist == 1 && sp.n >= 2 && is_smooth7(sp.n) &&
-                    fs_geometry(sp.n, elem, cap, &sp.g);
+                    fs_geometry(sp.n, elem, cap, &sp.g) && (size_t)sp.g.per_block * elem >= 64;
   fyfft_plan* sub = nullptr;
   fyfft_status st;
   if (rows) {

@@ -104,27 +104,77 @@ __global__ void k_gather(const V* src, V* dst, long long n, long long stride, lo
   else dst[e] = src[g];
 }
 
+// Adjacent transforms (cdist 1): 32 transforms x 32 elements per tile
+// through LDS, so both the strided side (32 adjacent columns of one row)
+// and the contiguous side (32 consecutive elements of one transform) move
+// in 256-byte (FP32) segments.
+template <typename V, bool SCATTER>
+__global__ __launch_bounds__(256) void k_gather_tiled(const V* src, V* dst, long long n, long long stride,
+                                                      long long ncols, long long pdist, long long t0,
+                                                      long long count, long long jtiles) {
+  __shared__ V tile[32][33];
+  const long long jb = (blockIdx.x % jtiles) * 32, rb = (blockIdx.x / jtiles) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  if (!SCATTER) {
+    for (int jj = ty; jj < 32; jj += 8) {
+      const long long j = jb + jj, r = rb + tx;
+      if (j < n && r < count) {
+        const long long t = t0 + r, p = t / ncols, c = t - p * ncols;
+        tile[jj][tx] = src[p * pdist + c + j * stride];
+      }
+    }
+    __syncthreads();
+    for (int rr = ty; rr < 32; rr += 8) {
+      const long long r = rb + rr, j = jb + tx;
+      if (r < count && j < n) dst[r * n + j] = tile[tx][rr];
+    }
+  } else {
+    for (int rr = ty; rr < 32; rr += 8) {
+      const long long r = rb + rr, j = jb + tx;
+      if (r < count && j < n) tile[tx][rr] = src[r * n + j];
+    }
+    __syncthreads();
+    for (int jj = ty; jj < 32; jj += 8) {
+      const long long j = jb + jj, r = rb + tx;
+      if (j < n && r < count) {
+        const long long t = t0 + r, p = t / ncols, c = t - p * ncols;
+        dst[p * pdist + c + j * stride] = tile[jj][tx];
+      }
+    }
+  }
+}
+
+template <typename V>
+static hipError_t go_gather(bool scatter, const void* strided, void* ws, long long n, long long stride,
+                            long long cdist, long long ncols, long long pdist, long long t0, long long count,
+                            hipStream_t s) {
+  if (cdist == 1) {
+    const long long jt = (n + 31) / 32, rt = (count + 31) / 32;
+    const dim3 grid((unsigned)(jt * rt)), block(256);
+    if (scatter)
+      hipLaunchKernelGGL((k_gather_tiled<V, true>), grid, block, 0, s, (const V*)ws, (V*)strided, n, stride,
+                         ncols, pdist, t0, count, jt);
+    else
+      hipLaunchKernelGGL((k_gather_tiled<V, false>), grid, block, 0, s, (const V*)strided, (V*)ws, n, stride,
+                         ncols, pdist, t0, count, jt);
+    return hipGetLastError();
+  }
+  const dim3 grid((unsigned)((count * n + 255) / 256)), block(256);
+  if (scatter)
+    hipLaunchKernelGGL((k_gather<V, true>), grid, block, 0, s, (const V*)ws, (V*)strided, n, stride, cdist,
+                       ncols, pdist, t0, count);
+  else
+    hipLaunchKernelGGL((k_gather<V, false>), grid, block, 0, s, (const V*)strided, (V*)ws, n, stride, cdist,
+                       ncols, pdist, t0, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather(int prec, bool scatter, const void* strided, void* ws, long long n, long long stride,
                          long long cdist, long long ncols, long long pdist, long long t0, long long count,
                          hipStream_t s) {
   if (count == 0) return hipSuccess;
-  const dim3 grid((unsigned)((count * n + 255) / 256)), block(256);
-  if (prec == kF64) {
-    if (scatter)
-      hipLaunchKernelGGL((k_gather<double2, true>), grid, block, 0, s, (const double2*)ws, (double2*)strided,
-                         n, stride, cdist, ncols, pdist, t0, count);
-    else
-      hipLaunchKernelGGL((k_gather<double2, false>), grid, block, 0, s, (const double2*)strided, (double2*)ws,
-                         n, stride, cdist, ncols, pdist, t0, count);
-  } else {
-    if (scatter)
-      hipLaunchKernelGGL((k_gather<float2, true>), grid, block, 0, s, (const float2*)ws, (float2*)strided,
-                         n, stride, cdist, ncols, pdist, t0, count);
-    else
-      hipLaunchKernelGGL((k_gather<float2, false>), grid, block, 0, s, (const float2*)strided, (float2*)ws,
-                         n, stride, cdist, ncols, pdist, t0, count);
-  }
-  return hipGetLastError();
+  if (prec == kF64) return go_gather<double2>(scatter, strided, ws, n, stride, cdist, ncols, pdist, t0, count, s);
+  return go_gather<float2>(scatter, strided, ws, n, stride, cdist, ncols, pdist, t0, count, s);
 }
 
 }  // namespace fyfft

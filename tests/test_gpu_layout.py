@@ -96,7 +96,8 @@ MANY_CASES = [
     (100, 7, 7, 1, 7, 1, "columns"),
     (256, 40, 40, 1, 48, 1, "columns"),
     (50, 4, 3, 200, 2, 150, "gather"),
-    (6720, 2, 2, 1, 2, 1, "columns"),
+    (6720, 2, 2, 1, 2, 1, "gather"),
+    (600, 64, 64, 1, 64, 1, "columns"),
 ]
 
 
