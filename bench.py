@@ -12,7 +12,7 @@ Other configurations (one JSON line each, same fields where they apply):
   --workload prime     Rader N=65537 x 2048, Bluestein N=10007 x 1024 and N=1000003 x 16
   --workload ragged    20000 ragged transforms in one grouped launch vs one launch per transform
   --workload spectral  spectral convolution [256, 64, 4096], 512 modes, forward + backward
-  --workload lowprec   BF16 and FP16 storage, N=4096 x 65536, radix-16 stages on MFMA
+  --workload lowprec   BF16 and FP16 storage, N=4096 x 65536 (FP32-arithmetic and MFMA forms)
   --workload real      real-input R2C and real-output C2R, N=8192 x 65536 and N=2^20 x 256 (float)
   --workload nd        2-D 8192 x 8192 and 3-D 512^3 complex<float>, and a strided (column) batch
 """
@@ -337,11 +337,14 @@ def bench_lowprec(a):
     stream = torch.cuda.current_stream(dev)
     n, batch = a.n, a.batch
     rows, tot_ms, tot_bytes, tot_flops = [], 0.0, 0.0, 0.0
-    for name, tdt, prec in (("bf16", torch.bfloat16, fy.BF16), ("fp16", torch.float16, fy.F16)):
+    for name, tdt, prec, form in (("bf16", torch.bfloat16, fy.BF16, fy.FORM_AUTO),
+                                  ("fp16", torch.float16, fy.F16, fy.FORM_AUTO),
+                                  ("bf16", torch.bfloat16, fy.BF16, fy.FORM_MFMA),
+                                  ("fp16", torch.float16, fy.F16, fy.FORM_MFMA)):
         g = torch.Generator(device=dev).manual_seed(0)
         x = torch.randn(batch, n, 2, device=dev, generator=g).to(tdt)
         y = torch.empty_like(x)
-        with fy.Plan(n, batch, prec) as p:
+        with fy.Plan(n, batch, prec, form) as p:
             h = stream.cuda_stream
             ms = time_steps(lambda: p.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h),
                             a.steps, a.warmup, stream)
@@ -353,7 +356,8 @@ def bench_lowprec(a):
         err = float((y[r].cpu().double() - ext).norm() / ext.norm())
         floor = float((ext.to(tdt).double() - ext).norm() / ext.norm())
         byt = 2.0 * 4 * n * batch
-        rows.append({"dtype": name, "plan": desc, "ms": round(ms, 4),
+        rows.append({"dtype": name, "form": "mfma" if form == fy.FORM_MFMA else "auto", "plan": desc,
+                     "ms": round(ms, 4),
                      "GFLOP/s": round(flops_of(n) * batch / ms / 1e6, 1),
                      "hbm_frac": round(byt / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                      "rel_l2_vs_fp64": err, "storage_rounding_floor": floor})
@@ -361,8 +365,9 @@ def bench_lowprec(a):
         tot_bytes += byt
         tot_flops += flops_of(n) * batch
         del x, y
-    return line({"config": {"workload": f"low-precision storage, forward C2C N={n} x {batch}, "
-                                        "radix-16 stages on MFMA with split hi/lo fp16 operands",
+    return line({"config": {"workload": f"low-precision storage, forward C2C N={n} x {batch}: FP32 "
+                                        "arithmetic with 16-bit storage (auto form) and radix-16 stages "
+                                        "on MFMA with split hi/lo fp16 operands (mfma form)",
                             "cases": rows}},
                 value=round(tot_flops / tot_ms / 1e6, 2), steps=a.steps, warmup=a.warmup,
                 ms_per_step=round(tot_ms, 4), dtype="bf16/fp16",

@@ -31,7 +31,7 @@ MAX_LENGTH = 1 << 24          # this library's limit (FYFFT_MAX_LENGTH)
 
 FORWARD, INVERSE, ADJOINT = 0, 1, 2
 F32, F64, BF16, F16 = 0, 1, 2, 3
-FORM_AUTO, FORM_PLAIN, FORM_TUNED = 0, 1, 2
+FORM_AUTO, FORM_PLAIN, FORM_TUNED, FORM_MFMA = 0, 1, 2, 3
 
 
 class _Options(ctypes.Structure):

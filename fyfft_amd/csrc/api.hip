@@ -22,7 +22,7 @@ constexpr uint32_t kMagic = 0xF1FF7A11u;
 thread_local std::string g_last_error;
 
 enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, kBlueLds = 5, kBlueFs = 6, kGrouped = 7, kMfma = 8,
-            kReal = 9, kStrided = 10 };
+            kReal = 9, kStrided = 10, kLowp = 11 };
 
 // One strided pass: transforms of length n at (plane p, column c), element j
 // at p*pdist + c*cdist + j*stride, separately for input and output.
@@ -715,11 +715,16 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
       precision != FYFFT_F16)
     return FYFFT_INVALID_ARGUMENT;
   if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
-  if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED)
+  if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED && form != FYFFT_FORM_MFMA)
     return FYFFT_INVALID_ARGUMENT;
   const bool low = precision == FYFFT_BF16 || precision == FYFFT_F16;
+  if (form == FYFFT_FORM_MFMA && !low) return FYFFT_INVALID_ARGUMENT;
   int r0 = 1;
-  if (low && !mfma16_supported(n, &r0)) return FYFFT_UNSUPPORTED;
+  dev::PassDesc lowp_pd{};
+  // 16-bit storage: FP32 arithmetic in the compile-time kernels unless the
+  // MFMA form is asked for (or the length has no compile-time kernel)
+  const bool lowp = low && form != FYFFT_FORM_MFMA && lowp_fixed_supported(n, &lowp_pd);
+  if (low && !lowp && !mfma16_supported(n, &r0)) return FYFFT_UNSUPPORTED;
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   fyfft_plan* p = new (std::nothrow) fyfft_plan();
@@ -730,7 +735,20 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
   p->device = dev;
   p->strategy = strategy_for(n);
   fyfft_status st = FYFFT_SUCCESS;
-  if (low) {
+  if (lowp) {
+    p->kind = kLowp;
+    p->lds.pd = lowp_pd;
+    void* tw = nullptr;
+    st = p->alloc(n * sizeof(float2), &tw);
+    if (!st) {
+      const hipError_t e = launch_gen_stockham_twiddles(kF32, tw, (int)n, lowp_pd, nullptr);
+      if (e != hipSuccess) st = from_hip(e, "twiddles");
+    }
+    p->lds.tw = tw;
+    p->lds.geom.n = (int)n;
+    p->desc = std::string("lowp-fixed ") + (precision == FYFFT_BF16 ? "bf16" : "fp16") + " N=" +
+              std::to_string(n) + " radices=" + radix_string(lowp_pd) + " FP32 arithmetic, 16-bit storage";
+  } else if (low) {
     // FP32 Stockham table for the pass list [R0,] 16, ..., 16
     p->kind = kMfma;
     dev::PassDesc& pd = p->lds.pd;
@@ -1129,6 +1147,10 @@ static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out,
     case kMfma:
       HIP_TRY(launch_mfma16(p->prec == kBF16, dir == kForward ? -1 : +1, p->n, in, out, rows,
                             p->lds.tw, scale, s));
+      return FYFFT_SUCCESS;
+    case kLowp:
+      HIP_TRY(launch_lowp_fixed(p->prec == kBF16, dir == kForward ? -1 : +1, p->n, in, out, rows,
+                                p->lds.tw, scale, s));
       return FYFFT_SUCCESS;
     case kReal:
       return exec_real(p, in, out, direction, rows, s);

@@ -75,6 +75,20 @@ __device__ __forceinline__ void gstore(double2* p, double2 v) {
   __builtin_nontemporal_store(f64x2_t{v.x, v.y}, reinterpret_cast<f64x2_t*>(p));
 }
 
+// 16-bit storage element: two 16-bit floats (re, im) in 4 bytes (H = __bf16
+// or _Float16); loads widen to FP32, stores round to nearest even.
+template <typename H> struct cplx16 { H re, im; };
+
+template <typename H> __device__ __forceinline__ float2 load16(const cplx16<H>* p) {
+  const unsigned u = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(p));
+  const cplx16<H> c = __builtin_bit_cast(cplx16<H>, u);
+  return float2{(float)c.re, (float)c.im};
+}
+template <typename H> __device__ __forceinline__ void store16(cplx16<H>* p, float2 v) {
+  const cplx16<H> c{(H)v.x, (H)v.y};
+  __builtin_nontemporal_store(__builtin_bit_cast(unsigned, c), reinterpret_cast<unsigned*>(p));
+}
+
 // cos/sin(2*pi*j/R) for the odd prime radices, exact decimal expansions.
 template <int R> __device__ __forceinline__ constexpr double cos_r(int j) {
   j %= R;
@@ -419,9 +433,8 @@ template <typename V> struct LdsSink {
 };
 
 template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
-__device__ __noinline__ void io_pass_ool(Src src, Dst dst, int N, int Ls, int T, int tid,
-                                         gbl_cptr<V> tw_g, bool active) {
-  const V* tw = (const V*)tw_g;
+__device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, int tid,
+                                        const V* __restrict__ tw, bool active) {
   const int nb = N / R;
   V v[MAXB][R];
   int q0 = 0, k0 = 0, tq = 0, tk = 0;
@@ -467,6 +480,30 @@ __device__ __noinline__ void io_pass_ool(Src src, Dst dst, int N, int Ls, int T,
     }
   }
   if constexpr (Dst::kToLds) __syncthreads();
+}
+
+// Out-of-line form for the runtime pass loops.
+template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
+__device__ __noinline__ void io_pass_ool(Src src, Dst dst, int N, int Ls, int T, int tid,
+                                         gbl_cptr<V> tw_g, bool active) {
+  io_pass<V, R, MAXB, DIR>(src, dst, N, Ls, T, tid, (const V*)tw_g, active);
+}
+
+// Compile-time pass list (N, T and the radices fixed, Ls starting at 1) with
+// the first pass reading `src` and the last writing `dst`; `lds` holds the
+// transform between passes (pad shift ps).
+template <typename V, int DIR, int N, int T, int Ls, typename Src, typename Dst, int R, int... Rest>
+__device__ __forceinline__ void fixed_io_passes(Src src, Dst dst, V* lds, int ps, int tid,
+                                                const V* __restrict__ tw, bool active) {
+  constexpr bool first = Ls == 1, last = sizeof...(Rest) == 0;
+  constexpr int MAXB = cdiv(N / R, T);
+  const LdsSrc<V> ls{(lds_ptr<V>)lds, ps};
+  const LdsSink<V> lk{(lds_ptr<V>)lds, ps};
+  if constexpr (first && last) io_pass<V, R, MAXB, DIR>(src, dst, N, Ls, T, tid, tw, active);
+  else if constexpr (first) io_pass<V, R, MAXB, DIR>(src, lk, N, Ls, T, tid, tw, active);
+  else if constexpr (last) io_pass<V, R, MAXB, DIR>(ls, dst, N, Ls, T, tid, tw, active);
+  else io_pass<V, R, MAXB, DIR>(ls, lk, N, Ls, T, tid, tw, active);
+  if constexpr (!last) fixed_io_passes<V, DIR, N, T, Ls * R, Src, Dst, Rest...>(src, dst, lds, ps, tid, tw, active);
 }
 
 template <int R> struct RadixC { static constexpr int value = R; };

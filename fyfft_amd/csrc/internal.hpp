@@ -246,6 +246,14 @@ hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, in
 // Sets the dynamic-LDS limits of the grouped kernels per class (call at plan time).
 hipError_t prepare_grouped(int prec, const size_t lds[kGroupClasses]);
 
+// --- kernels_lowp.hip
+// BF16/FP16 storage with FP32 arithmetic for the lengths of the FP32
+// compile-time kernels (N = 256 .. 16384 powers of two); pd receives the
+// pass list the FP32 twiddle table must be built for.
+bool lowp_fixed_supported(size_t n, dev::PassDesc* pd);
+hipError_t launch_lowp_fixed(bool bf16, int dir, size_t n, const void* in, void* out, size_t batch,
+                             const void* tw, double scale, hipStream_t s);
+
 // --- kernels_mfma.hip
 // BF16/FP16-storage transforms with radix-16 stages on MFMA: N = R0 * 16^P
 // for the supported powers of two (256 .. 16384); *r0 receives R0.

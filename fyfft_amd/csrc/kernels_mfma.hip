@@ -99,19 +99,6 @@ __device__ __forceinline__ void mfma_dft16(const Mfma16A& A, const float2 x[4], 
   for (int i = 0; i < 4; ++i) y[i] = float2{yr[i] * inv, yi[i] * inv};
 }
 
-// Storage element: two 16-bit floats (re, im) in 4 bytes.
-template <typename H> struct cplx16 { H re, im; };
-
-template <typename H> __device__ __forceinline__ float2 load16(const cplx16<H>* p) {
-  const unsigned u = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(p));
-  const cplx16<H> c = __builtin_bit_cast(cplx16<H>, u);
-  return float2{(float)c.re, (float)c.im};
-}
-template <typename H> __device__ __forceinline__ void store16(cplx16<H>* p, float2 v) {
-  const cplx16<H> c{(H)v.x, (H)v.y};
-  __builtin_nontemporal_store(__builtin_bit_cast(unsigned, c), reinterpret_cast<unsigned*>(p));
-}
-
 // One radix-16 Stockham pass on MFMA.  W waves x G groups x 16 butterflies
 // cover the N/16 butterflies; src/dst are LDS (FP32 complex, index i at
 // i + (i >> ps)) or, for the first/last pass, the 16-bit HBM buffers.

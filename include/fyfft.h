@@ -89,8 +89,16 @@ typedef enum fyfft_strategy {
  *           per thread, specialised vs generic) on the device at plan time
  *           and keep the fastest; the choice is remembered as wisdom per
  *           (N, precision).  Applies to lengths held in one workgroup's LDS;
- *           other lengths plan as AUTO. */
-typedef enum fyfft_form { FYFFT_FORM_AUTO = 0, FYFFT_FORM_PLAIN = 1, FYFFT_FORM_TUNED = 2 } fyfft_form;
+ *           other lengths plan as AUTO;
+ *   MFMA  - BF16/F16 storage only: radix-16 stages on MFMA with split hi/lo
+ *           fp16 operands (for BF16/F16 the other forms run FP32 arithmetic
+ *           in the compile-time kernels, rounding once at the store). */
+typedef enum fyfft_form {
+  FYFFT_FORM_AUTO = 0,
+  FYFFT_FORM_PLAIN = 1,
+  FYFFT_FORM_TUNED = 2,
+  FYFFT_FORM_MFMA = 3 /* BF16/F16 only: radix-16 stages on MFMA */
+} fyfft_form;
 
 typedef struct fyfft_options {
   int form;         /* fyfft_form */
@@ -99,11 +107,14 @@ typedef struct fyfft_options {
 
 typedef struct fyfft_plan fyfft_plan;
 
-/* BF16 / FP16 plans store data as 16-bit complex pairs and run the
- * radix-16 stages on MFMA with split hi/lo operands and FP32 accumulation
- * (internal error at FP32 level; the stored result is rounded once).  They
- * are available for N = 256 .. 16384 powers of two; other lengths return
- * FYFFT_UNSUPPORTED.  The reference has only complex<float>. */
+/* BF16 / FP16 plans store data as 16-bit complex pairs (4 bytes per
+ * element).  By default they run the FP32 compile-time kernels with
+ * widening loads and one rounding at the final store; FYFFT_FORM_MFMA runs
+ * the radix-16 stages on MFMA with split hi/lo operands and FP32
+ * accumulation instead.  Either way the internal error is at FP32 level and
+ * the stored result is rounded once.  Available for N = 256 .. 16384 powers
+ * of two; other lengths return FYFFT_UNSUPPORTED.  The reference has only
+ * complex<float>. */
 
 /* Replaces FFT1D_C2C_invoker::FFT1D_C2C_invoker(std::size_t)
  * (foye_FFT.hpp:101-126), extended with a batch of `batch` contiguous
