@@ -97,6 +97,11 @@ struct fyfft_plan {
   std::vector<StridedPass> passes;
   void* lws = nullptr;   // gather workspace
   size_t in_ext = 0, out_ext = 0;  // elements spanned by the input / output layout
+  // half-length tables of the spectral transforms (built on first use)
+  mutable std::mutex half_mu;
+  mutable int half_state = 0;       // 0 not built, 1 ready, -1 not available
+  mutable LdsLaunch half;
+  mutable void* half_rtw = nullptr;
   void* rtw = nullptr;   // exp(-2*pi*i*k/n), k <= n/2 (half forms)
   void* rws = nullptr;   // rchunk rows of the sub-plan's length (multi-pass forms)
   size_t rchunk = 0;
@@ -696,14 +701,66 @@ static fyfft_status exec_spectral(const fyfft_plan* p, bool inverse, const void*
   return FYFFT_SUCCESS;
 }
 
+// Tables of the half-length spectral forms: the pass list, Stockham table
+// and geometry of length n/2 (compile-time kernel geometry where one
+// exists) and exp(-2 pi i k / n) for k <= n/2.  Built once per plan.
+static bool spectral_half_tables(const fyfft_plan* cp) {
+  std::lock_guard<std::mutex> lock(cp->half_mu);
+  if (cp->half_state) return cp->half_state > 0;
+  fyfft_plan* p = const_cast<fyfft_plan*>(cp);
+  const size_t n = p->n, h = n / 2;
+  cp->half_state = -1;
+  if (n % 2 || h < 2 || !is_smooth7(h)) return false;
+  if (plan_lds(p, h, &p->half, true)) return false;
+  if (p->alloc((h + 1) * elem_bytes(p->prec), &p->half_rtw)) return false;
+  if (launch_gen_roots(p->prec, p->half_rtw, (long long)(h + 1), 1, (long long)n, nullptr) != hipSuccess) return false;
+  if (hipDeviceSynchronize() != hipSuccess) return false;
+  cp->half_state = 1;
+  return true;
+}
+
+template <typename V>
+static fyfft_status exec_spectral_half(const fyfft_plan* p, bool inverse, const void* src, void* dst,
+                                       size_t k, double scale0, double scale, hipStream_t s) {
+  SpecArgs<V> a;
+  if (inverse) {
+    a.cplx_in = (const V*)src;
+    a.real_out = (dev::S<V>*)dst;
+  } else {
+    a.real_in = (const dev::S<V>*)src;
+    a.cplx_out = (V*)dst;
+  }
+  const LdsGeom& g = p->half.geom;
+  const size_t h = p->n / 2;
+  a.rows = (long long)p->batch;
+  a.n = (int)p->n;
+  a.k = (int)k;
+  a.threads = g.threads;
+  a.per_block = g.per_block;
+  a.ps = g.pad_shift;
+  a.stride = dev::lds_len((int)h, g.pad_shift);
+  a.pd = p->half.pd;
+  a.tw = (const V*)p->half.tw;
+  a.rtw = (const V*)p->half_rtw;
+  a.a0 = (dev::S<V>)scale0;
+  a.a = (dev::S<V>)scale;
+  HIP_TRY(launch_spectral_half(p->prec, inverse, &a, g, p->half.fixed >= 0 ? (int)h : 0, s));
+  return FYFFT_SUCCESS;
+}
+
 static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src, void* dst,
                              size_t k, double scale0, double scale, void* stream) {
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   if (!src || !dst || k < 1 || k > p->n) return FYFFT_INVALID_ARGUMENT;
+  if (p->kind != kLds && p->kind != kFourStep) return FYFFT_UNSUPPORTED;
+  const hipStream_t s = (hipStream_t)stream;
+  if (k <= p->n / 2 + 1 && spectral_half_tables(p))
+    return p->prec == kF64 ? exec_spectral_half<double2>(p, inverse, src, dst, k, scale0, scale, s)
+                           : exec_spectral_half<float2>(p, inverse, src, dst, k, scale0, scale, s);
   if (p->kind != kLds) return FYFFT_UNSUPPORTED;
   if (p->prec == kF64)
-    return exec_spectral<double2>(p, inverse, src, dst, k, scale0, scale, (hipStream_t)stream);
-  return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, (hipStream_t)stream);
+    return exec_spectral<double2>(p, inverse, src, dst, k, scale0, scale, s);
+  return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, s);
 }
 
 static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,

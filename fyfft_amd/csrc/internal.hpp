@@ -140,8 +140,14 @@ template <typename V> struct SpecArgs {
   dev::PassDesc pd{};
   const V* tw = nullptr;
   dev::S<V> a0 = 1, a = 1;             // scale of mode 0 and of modes 1..k-1
+  const V* rtw = nullptr;              // half-length forms: exp(-2 pi i k / n), k <= n/2
 };
 hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGeom& g, hipStream_t s);
+// Half-length forms (even n, k <= n/2 + 1): pd/tw/g describe the length n/2;
+// fixed_h = n/2 selects compile-time passes matching the FP32/FP64 fixed
+// kernels' geometry, 0 the runtime passes.
+hipError_t launch_spectral_half(int prec, bool inverse, const void* args, const LdsGeom& g, int fixed_h,
+                                hipStream_t s);
 
 // --- kernels_layout.hip
 // Strided transforms.  Element j of transform (plane p, column c) sits at

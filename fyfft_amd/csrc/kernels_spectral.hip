@@ -9,7 +9,18 @@
 // With a0 = 1/L, a = 2/L the second is the real inverse of a Hermitian
 // spectrum whose upper half mirrors the kept modes (irfft of the first K
 // bins); with a0 = a = 1 it is the real part of the adjoint transform.
-#include "internal.hpp"
+//
+// Even L with K <= L/2 + 1 runs the half-length forms: the real row is read
+// as h = L/2 complex values, one FFT of length h (compile-time passes where
+// the plan has them) and the two-for-one combination
+//   X[k] = (Z[k] + conj Z[h-k])/2 + w^k (Z[k] - conj Z[h-k])/(2i)
+// give the kept modes.  The inverse writes
+//   Re(sum_{k<K} c_k Y[k] e^{+2 pi i nk/L}) = sum_k H[k] e^{+2 pi i nk/L}
+// with the Hermitian H[0] = Re(c_0 Y[0]), H[k] = H*[L-k] = c_k Y[k]/2
+// (0 < k < K; H[h] = Re(c_h Y[h]) if kept), runs the same combination
+// backwards into one inverse FFT of length h and stores the h complex
+// results as the 2h reals of the row.
+#include "bodies.hpp"
 
 namespace fyfft {
 using namespace dev;
@@ -55,6 +66,123 @@ __global__ __launch_bounds__(1024) void k_irfft_modes(SpecArgs<V> a) {
     S<V>* y = a.real_out + row * L;
     for (int n = tid; n < L; n += T) __builtin_nontemporal_store(lds[n + (n >> ps)].x, y + n);
   }
+}
+
+// ------------------------------------------------------------ half-length forms
+template <typename V> __device__ __forceinline__ V conj_s(V a) { return V{a.x, -a.y}; }
+
+// Source of the inverse: Z[k] = E[k] + i O[k] from the Hermitian H built
+// on the fly from the kept modes.
+template <typename V> struct ModesHermSrc {
+  static constexpr bool kFromLds = false;
+  gbl_cptr<V> m;  // kept modes of the row
+  gbl_cptr<V> w;  // exp(-2 pi i k / L), k <= h
+  int K, h;
+  S<V> a0, a;
+  __device__ V herm(int j) const {
+    const V* M = (const V*)m;
+    if (j >= K) return V{0, 0};
+    if (j == 0) return V{a0 * M[0].x, 0};
+    if (j == h) return V{a * M[j].x, 0};
+    return cscale(M[j], a * (S<V>)0.5);
+  }
+  __device__ V operator()(int k) const {
+    const V hk = herm(k), hm = conj_s(herm(h - k));
+    const V e = cscale(cadd(hk, hm), (S<V>)0.5);
+    const V o = cmulc(cscale(csub(hk, hm), (S<V>)0.5), ((const V*)w)[k]);
+    return V{e.x - o.y, e.y + o.x};
+  }
+};
+
+template <typename V, int PPT> struct GenericIo {
+  template <int D, typename Src, typename Dst>
+  static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int n, int T, int tid,
+                                             const PassDesc& pd, const V* tw, bool active, int ps) {
+    lds_fft_io<V, PPT, D>(src, dst, lds, n, T, tid, pd, tw, active, ps);
+  }
+};
+template <typename V, int N, int T, int PS, int... Rs> struct FixedIo {
+  template <int D, typename Src, typename Dst>
+  static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int, int, int tid, const PassDesc&,
+                                             const V* tw, bool active, int) {
+    fixed_io_passes<V, D, N, T, 1, Src, Dst, Rs...>(src, dst, as_lds(lds), PS, tid, tw, active);
+  }
+};
+
+template <typename V, typename F>
+__global__ __launch_bounds__(1024) void k_rfft_modes_half(SpecArgs<V> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int T = a.threads, h = a.n / 2, K = a.k, ps = a.ps;
+  const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
+  const long long row = (long long)blockIdx.x * a.per_block + slot;
+  const bool active = row < a.rows;
+  V* lds = as_lds(reinterpret_cast<V*>(smem_raw) + slot * a.stride);
+  const V* z = reinterpret_cast<const V*>(a.real_in) + (active ? row : 0) * h;
+  F::template run<-1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, h, T, tid, a.pd,
+                      a.tw, active, ps);
+  if (active) {
+    V* out = a.cplx_out + row * K;
+    for (int k = tid; k < K; k += T) {
+      const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
+      const V zk = lds[k0 + (k0 >> ps)], zm = conj_s(lds[km + (km >> ps)]);
+      const V e = cscale(cadd(zk, zm), (S<V>)0.5);
+      const V d = cscale(csub(zk, zm), (S<V>)0.5);
+      const V X = cadd(e, cmul(a.rtw[k], V{d.y, -d.x}));
+      gstore(out + k, cscale(X, k == 0 ? a.a0 : a.a));
+    }
+  }
+}
+
+template <typename V, typename F>
+__global__ __launch_bounds__(1024) void k_irfft_modes_half(SpecArgs<V> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int T = a.threads, h = a.n / 2, ps = a.ps;
+  const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
+  const long long row = (long long)blockIdx.x * a.per_block + slot;
+  const bool active = row < a.rows;
+  V* lds = reinterpret_cast<V*>(smem_raw) + slot * a.stride;
+  const long long r = active ? row : 0;
+  V* y = reinterpret_cast<V*>(a.real_out) + r * h;
+  F::template run<+1>(ModesHermSrc<V>{(gbl_cptr<V>)(a.cplx_in + r * a.k), (gbl_cptr<V>)a.rtw, a.k, h, a.a0, a.a},
+                      GlobalSink<V>{(gbl_ptr<V>)y, (S<V>)2}, lds, h, T, tid, a.pd, a.tw, active, ps);
+}
+
+template <typename V, typename F>
+static hipError_t go_half(bool inverse, const SpecArgs<V>& a, const LdsGeom& g, hipStream_t s) {
+  auto k = inverse ? k_irfft_modes_half<V, F> : k_rfft_modes_half<V, F>;
+  const size_t lds = (size_t)a.stride * g.per_block * sizeof(V);
+  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const unsigned grid = (unsigned)((a.rows + g.per_block - 1) / g.per_block);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(g.threads * g.per_block), lds, s, a);
+  return hipGetLastError();
+}
+
+// fixed_h: the half length when the plan's half-length table has a
+// compile-time kernel (its geometry is g), 0 for the runtime passes.
+hipError_t launch_spectral_half(int prec, bool inverse, const void* args, const LdsGeom& g, int fixed_h,
+                                hipStream_t s) {
+  if (prec == kF64) {
+    const auto& a = *static_cast<const SpecArgs<double2>*>(args);
+    if (a.rows == 0) return hipSuccess;
+    switch (fixed_h) {
+      case 1024: return go_half<double2, FixedIo<double2, 1024, 64, 30, 16, 8, 8>>(inverse, a, g, s);
+      case 4096: return go_half<double2, FixedIo<double2, 4096, 256, 30, 16, 16, 16>>(inverse, a, g, s);
+    }
+    return g.ppt <= 8 ? go_half<double2, GenericIo<double2, 8>>(inverse, a, g, s)
+                      : go_half<double2, GenericIo<double2, 16>>(inverse, a, g, s);
+  }
+  const auto& a = *static_cast<const SpecArgs<float2>*>(args);
+  if (a.rows == 0) return hipSuccess;
+  switch (fixed_h) {
+    case 256: return go_half<float2, FixedIo<float2, 256, 16, 30, 16, 16>>(inverse, a, g, s);
+    case 512: return go_half<float2, FixedIo<float2, 512, 32, 30, 8, 8, 8>>(inverse, a, g, s);
+    case 1024: return go_half<float2, FixedIo<float2, 1024, 64, 30, 16, 8, 8>>(inverse, a, g, s);
+    case 2048: return go_half<float2, FixedIo<float2, 2048, 128, 30, 16, 16, 8>>(inverse, a, g, s);
+    case 4096: return go_half<float2, FixedIo<float2, 4096, 256, 30, 16, 16, 16>>(inverse, a, g, s);
+    case 8192: return go_half<float2, FixedIo<float2, 8192, 512, 30, 16, 8, 8, 8>>(inverse, a, g, s);
+  }
+  return g.ppt <= 16 ? go_half<float2, GenericIo<float2, 16>>(inverse, a, g, s)
+                     : go_half<float2, GenericIo<float2, 32>>(inverse, a, g, s);
 }
 
 template <typename V, int PPT>

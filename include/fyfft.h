@@ -238,9 +238,13 @@ fyfft_status fyfft_length_radices(size_t n, int* radices, int max, int* count);
 /* Spectral-convolution transforms (no counterpart in the reference, which
  * has neither real-input transforms nor differentiation; they serve the
  * fused spectral convolution and its hand-written adjoint).  `plan` is an
- * F32 or F64 plan of length L whose batch is the number of rows; L must fit
- * one workgroup's LDS (FYFFT_UNSUPPORTED otherwise).  c_0 = scale0 and
- * c_j = scale for j >= 1; 1 <= k <= L.  Asynchronous on `stream`.
+ * F32 or F64 plan of length L whose batch is the number of rows.  Even L
+ * with k <= L/2 + 1 runs one complex transform of length L/2 per row (the
+ * real row read as L/2 complex pairs); L/2 must fit one workgroup's LDS.
+ * Otherwise L itself must fit (FYFFT_UNSUPPORTED if neither holds).  The
+ * first call builds the plan's half-length tables (synchronous, allocates).
+ * c_0 = scale0 and c_j = scale for j >= 1; 1 <= k <= L.  Asynchronous on
+ * `stream`.
  *
  * fyfft_rfft_modes:  x is rows x L real (float/double), modes is rows x k
  *   complex:  modes[r][j] = c_j * sum_n x[r][n] exp(-2*pi*i*n*j/L).

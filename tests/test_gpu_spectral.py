@@ -10,7 +10,11 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("L,k", [(256, 32), (4096, 512), (6720, 100), (1024, 1024)])
+# even L with k <= L/2 + 1 take the half-length forms (compile-time passes
+# for L/2 = 256 .. 8192, runtime passes for 6720/2); (1024, 1024) and odd L
+# take the full-length form; k = L/2 + 1 keeps the Nyquist bin
+@pytest.mark.parametrize("L,k", [(256, 32), (4096, 512), (6720, 100), (1024, 1024), (256, 129),
+                                 (4096, 2049), (243, 50), (2048, 1), (16384, 700)])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_modes_vs_numpy(fy, cuda, L, k, prec):
     import torch
