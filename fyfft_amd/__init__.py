@@ -98,6 +98,8 @@ def _load() -> ctypes.CDLL:
         dbl = ctypes.c_double
         L.fyfft_rfft_modes.argtypes = [vp, vp, vp, sz, dbl, dbl, vp]
         L.fyfft_irfft_modes.argtypes = [vp, vp, vp, sz, dbl, dbl, vp]
+        L.fyfft_rfft_modes_ex.argtypes = [vp, vp, vp, sz, dbl, dbl, i, vp]
+        L.fyfft_irfft_modes_ex.argtypes = [vp, vp, vp, sz, dbl, dbl, i, vp]
         L.fyfft_plan_create_grouped.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(sz), sz, i, ctypes.c_uint]
         L.fyfft_plan_elements.argtypes = [vp, ctypes.POINTER(sz)]
         L.fyfft_plan_create_ex.argtypes = [ctypes.POINTER(vp), sz, sz, i, ctypes.POINTER(_Options)]
@@ -114,7 +116,8 @@ def _load() -> ctypes.CDLL:
         for name in ("fyfft_plan_create", "fyfft_plan_destroy", "fyfft_execute",
                      "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
-                     "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_plan_create_grouped",
+                     "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_rfft_modes_ex", "fyfft_irfft_modes_ex",
+                     "fyfft_plan_create_grouped",
                      "fyfft_plan_elements", "fyfft_plan_create_ex", "fyfft_plan_create_real",
                      "fyfft_plan_create_many", "fyfft_plan_create_nd",
                      "fyfft_wisdom_export",
@@ -289,10 +292,12 @@ def _stream_handle(stream, device):
     return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
 
 
-def rfft_modes(x, k: int, scale0: float = 1.0, scale: float = 1.0, stream=None):
+def rfft_modes(x, k: int, scale0: float = 1.0, scale: float = 1.0, stream=None, mode_major: bool = False):
     """First k modes of the C2C FFT of real rows x [..., L] (float32/float64
     device tensor): out[..., j] = c_j * sum_n x[..., n] e^{-2 pi i n j / L},
-    c_0 = scale0, c_j = scale.  Returns complex [..., k]."""
+    c_0 = scale0, c_j = scale.  Returns complex [..., k], or [k, ...] with
+    mode_major (one contiguous [rows] slab per mode, the layout of per-mode
+    GEMMs)."""
     import torch
     if x.dtype not in (torch.float32, torch.float64) or not x.is_cuda:
         raise TypeError("rfft_modes: real float32/float64 device tensor expected")
@@ -301,30 +306,31 @@ def rfft_modes(x, k: int, scale0: float = 1.0, scale: float = 1.0, stream=None):
     rows = x.numel() // L
     prec = F32 if x.dtype == torch.float32 else F64
     p = _cached_plan(L, rows, prec, x.device.index or 0)
-    out = torch.empty(*x.shape[:-1], k, dtype=torch.complex64 if prec == F32 else torch.complex128,
-                      device=x.device)
-    _check(_load().fyfft_rfft_modes(p._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                                    k, float(scale0), float(scale),
-                                    ctypes.c_void_p(_stream_handle(stream, x.device))), "rfft_modes")
+    shape = (k, *x.shape[:-1]) if mode_major else (*x.shape[:-1], k)
+    out = torch.empty(shape, dtype=torch.complex64 if prec == F32 else torch.complex128, device=x.device)
+    _check(_load().fyfft_rfft_modes_ex(p._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                       k, float(scale0), float(scale), 1 if mode_major else 0,
+                                       ctypes.c_void_p(_stream_handle(stream, x.device))), "rfft_modes")
     return out
 
 
-def irfft_modes(modes, L: int, scale0: float, scale: float, stream=None):
+def irfft_modes(modes, L: int, scale0: float, scale: float, stream=None, mode_major: bool = False):
     """y[..., n] = Re(sum_{j<k} c_j modes[..., j] e^{+2 pi i n j / L}) for
-    complex modes [..., k]; c_0 = scale0, c_j = scale.  Returns real [..., L]."""
+    complex modes [..., k] (or [k, ...] with mode_major); c_0 = scale0,
+    c_j = scale.  Returns real [..., L]."""
     import torch
     if modes.dtype not in (torch.complex64, torch.complex128) or not modes.is_cuda:
         raise TypeError("irfft_modes: complex64/complex128 device tensor expected")
     modes = modes.contiguous()
-    k = modes.shape[-1]
+    k = modes.shape[0] if mode_major else modes.shape[-1]
+    lead = modes.shape[1:] if mode_major else modes.shape[:-1]
     rows = modes.numel() // k
     prec = _precision_of(modes.dtype)
     p = _cached_plan(L, rows, prec, modes.device.index or 0)
-    y = torch.empty(*modes.shape[:-1], L, dtype=torch.float32 if prec == F32 else torch.float64,
-                    device=modes.device)
-    _check(_load().fyfft_irfft_modes(p._h, ctypes.c_void_p(modes.data_ptr()), ctypes.c_void_p(y.data_ptr()),
-                                     k, float(scale0), float(scale),
-                                     ctypes.c_void_p(_stream_handle(stream, modes.device))), "irfft_modes")
+    y = torch.empty(*lead, L, dtype=torch.float32 if prec == F32 else torch.float64, device=modes.device)
+    _check(_load().fyfft_irfft_modes_ex(p._h, ctypes.c_void_p(modes.data_ptr()), ctypes.c_void_p(y.data_ptr()),
+                                        k, float(scale0), float(scale), 1 if mode_major else 0,
+                                        ctypes.c_void_p(_stream_handle(stream, modes.device))), "irfft_modes")
     return y
 
 

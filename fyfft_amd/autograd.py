@@ -63,31 +63,35 @@ class SpectralConv1dFunction(torch.autograd.Function):
     """y = Re(IFFT_L(Z)), Z[0] = Y[0], Z[k] = 2 Y[k] (0 < k < K), Z = 0 above,
     Y[b,o,k] = sum_i X[b,i,k] W[i,o,k], X = FFT_L(x)[..., :K]."""
 
+    # The modes live mode-major ([K, B, C]) between the transforms, so each
+    # mode's channel mixing is one GEMM of a batched matmul with no layout
+    # copies of the [B, C, K] mode tensors.
+
     @staticmethod
     def forward(ctx, x, w):
         L = x.shape[-1]
         K = w.shape[-1]
         if 2 * K > L:
             raise ValueError("SpectralConv1d: modes must satisfy 2*K <= L")
-        xm = rfft_modes(x, K)                       # [B, Ci, K]
-        ym = mix_modes(xm, w)                       # [B, Co, K]
-        y = irfft_modes(ym, L, 1.0 / L, 2.0 / L)    # [B, Co, L]
-        ctx.save_for_backward(xm, w)
+        xm = rfft_modes(x, K, mode_major=True)               # [K, B, Ci]
+        wt = w.permute(2, 0, 1).contiguous()                  # [K, Ci, Co]
+        ym = torch.matmul(xm, wt)                             # [K, B, Co]
+        y = irfft_modes(ym, L, 1.0 / L, 2.0 / L, mode_major=True)   # [B, Co, L]
+        ctx.save_for_backward(xm, wt)
         ctx.L = L
         return y
 
     @staticmethod
     def backward(ctx, g):
-        xm, w = ctx.saved_tensors
-        L, K = ctx.L, w.shape[-1]
+        xm, wt = ctx.saved_tensors
+        L, K = ctx.L, wt.shape[0]
         # dL/dY = S * (1/L) * FFT(g) restricted to the kept modes (S = 1, 2, 2, ...)
-        gy = rfft_modes(g.contiguous(), K, 1.0 / L, 2.0 / L)      # [B, Co, K]
-        gx_modes = mix_modes(gy, w.conj().transpose(0, 1))          # [B, Ci, K]
-        gw = torch.matmul(xm.conj().permute(2, 1, 0).contiguous(),  # [K, Ci, B]
-                          gy.permute(2, 0, 1).contiguous())         # [K, B, Co]
-        gw = gw.permute(1, 2, 0).contiguous()                       # [Ci, Co, K]
+        gy = rfft_modes(g.contiguous(), K, 1.0 / L, 2.0 / L, mode_major=True)   # [K, B, Co]
+        gxm = torch.matmul(gy, wt.conj().transpose(1, 2).contiguous())         # [K, B, Ci]
+        gw = torch.matmul(xm.conj().transpose(1, 2), gy)                        # [K, Ci, Co]
+        gw = gw.permute(1, 2, 0).contiguous()                                   # [Ci, Co, K]
         # dL/dx = Re(F^H pad(gX)): the real part of the adjoint over K modes
-        gx = irfft_modes(gx_modes, L, 1.0, 1.0)
+        gx = irfft_modes(gxm, L, 1.0, 1.0, mode_major=True)
         return gx, gw
 
 

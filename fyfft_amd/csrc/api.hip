@@ -676,8 +676,10 @@ static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool
 
 template <typename V>
 static fyfft_status exec_spectral(const fyfft_plan* p, bool inverse, const void* src, void* dst,
-                                  size_t k, double scale0, double scale, hipStream_t s) {
+                                  size_t k, double scale0, double scale, bool mode_major, hipStream_t s) {
   SpecArgs<V> a;
+  a.rdist = mode_major ? 1 : (long long)k;
+  a.kdist = mode_major ? (long long)p->batch : 1;
   if (inverse) {
     a.cplx_in = (const V*)src;
     a.real_out = (dev::S<V>*)dst;
@@ -721,8 +723,10 @@ static bool spectral_half_tables(const fyfft_plan* cp) {
 
 template <typename V>
 static fyfft_status exec_spectral_half(const fyfft_plan* p, bool inverse, const void* src, void* dst,
-                                       size_t k, double scale0, double scale, hipStream_t s) {
+                                       size_t k, double scale0, double scale, bool mode_major, hipStream_t s) {
   SpecArgs<V> a;
+  a.rdist = mode_major ? 1 : (long long)k;
+  a.kdist = mode_major ? (long long)p->batch : 1;
   if (inverse) {
     a.cplx_in = (const V*)src;
     a.real_out = (dev::S<V>*)dst;
@@ -749,18 +753,20 @@ static fyfft_status exec_spectral_half(const fyfft_plan* p, bool inverse, const 
 }
 
 static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src, void* dst,
-                             size_t k, double scale0, double scale, void* stream) {
+                             size_t k, double scale0, double scale, int layout, void* stream) {
+  if (layout != FYFFT_MODES_ROW_MAJOR && layout != FYFFT_MODES_MODE_MAJOR) return FYFFT_INVALID_ARGUMENT;
+  const bool mm = layout == FYFFT_MODES_MODE_MAJOR;
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   if (!src || !dst || k < 1 || k > p->n) return FYFFT_INVALID_ARGUMENT;
   if (p->kind != kLds && p->kind != kFourStep) return FYFFT_UNSUPPORTED;
   const hipStream_t s = (hipStream_t)stream;
   if (k <= p->n / 2 + 1 && spectral_half_tables(p))
-    return p->prec == kF64 ? exec_spectral_half<double2>(p, inverse, src, dst, k, scale0, scale, s)
-                           : exec_spectral_half<float2>(p, inverse, src, dst, k, scale0, scale, s);
+    return p->prec == kF64 ? exec_spectral_half<double2>(p, inverse, src, dst, k, scale0, scale, mm, s)
+                           : exec_spectral_half<float2>(p, inverse, src, dst, k, scale0, scale, mm, s);
   if (p->kind != kLds) return FYFFT_UNSUPPORTED;
   if (p->prec == kF64)
-    return exec_spectral<double2>(p, inverse, src, dst, k, scale0, scale, s);
-  return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, s);
+    return exec_spectral<double2>(p, inverse, src, dst, k, scale0, scale, mm, s);
+  return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, mm, s);
 }
 
 static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
@@ -1732,12 +1738,22 @@ fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int d
 
 fyfft_status fyfft_rfft_modes(const fyfft_plan* p, const void* x, void* modes, size_t k,
                               double scale0, double scale, void* stream) {
-  return spectral(p, false, x, modes, k, scale0, scale, stream);
+  return spectral(p, false, x, modes, k, scale0, scale, FYFFT_MODES_ROW_MAJOR, stream);
 }
 
 fyfft_status fyfft_irfft_modes(const fyfft_plan* p, const void* modes, void* y, size_t k,
                                double scale0, double scale, void* stream) {
-  return spectral(p, true, modes, y, k, scale0, scale, stream);
+  return spectral(p, true, modes, y, k, scale0, scale, FYFFT_MODES_ROW_MAJOR, stream);
+}
+
+fyfft_status fyfft_rfft_modes_ex(const fyfft_plan* p, const void* x, void* modes, size_t k, double scale0,
+                                 double scale, int layout, void* stream) {
+  return spectral(p, false, x, modes, k, scale0, scale, layout, stream);
+}
+
+fyfft_status fyfft_irfft_modes_ex(const fyfft_plan* p, const void* modes, void* y, size_t k, double scale0,
+                                  double scale, int layout, void* stream) {
+  return spectral(p, true, modes, y, k, scale0, scale, layout, stream);
 }
 
 fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, int direction) {

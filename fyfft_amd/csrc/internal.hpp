@@ -135,6 +135,7 @@ template <typename V> struct SpecArgs {
   const V* cplx_in = nullptr;          // irfft_modes input, rows x k
   dev::S<V>* real_out = nullptr;       // irfft_modes output, rows x n
   long long rows = 0;
+  long long rdist = 0, kdist = 1;      // mode j of row r at r*rdist + j*kdist
   int n = 0, k = 0;
   int threads = 0, per_block = 1, ps = 4, stride = 0;
   dev::PassDesc pd{};

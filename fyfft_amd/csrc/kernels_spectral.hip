@@ -1,6 +1,11 @@
 // Fused transforms for spectral convolution (rows of real signals whose
 // spectrum is kept only in its first K modes).
 //
+// Mode k of row r sits at r*rdist + k*kdist: row-major (rdist K, kdist 1)
+// or mode-major (rdist 1, kdist rows, the layout of per-mode GEMMs).
+// Workgroups are remapped so that consecutive rows run on one XCD, which
+// lets its L2 combine the mode-major writes of neighbouring rows.
+//
 // k_rfft_modes:  X[k] = c_k * sum_n x[n] exp(-2*pi*i*n*k/L),  k < K
 //   reads a real row (the C2C transform of real input, computed in LDS),
 //   stores only the K kept modes; c_0 = a0, c_k = a (k >= 1).
@@ -31,7 +36,7 @@ __global__ __launch_bounds__(1024) void k_rfft_modes(SpecArgs<V> a) {
   V* lds_all = reinterpret_cast<V*>(smem_raw);
   const int T = a.threads, L = a.n, K = a.k, ps = a.ps;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long row = (long long)blockIdx.x * a.per_block + slot;
+  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
   const bool active = row < a.rows;
   V* lds = lds_all + slot * a.stride;
   if (active) {
@@ -41,8 +46,8 @@ __global__ __launch_bounds__(1024) void k_rfft_modes(SpecArgs<V> a) {
   __syncthreads();
   lds_fft<V, PPT, -1>(lds, L, T, tid, a.pd, a.tw, active, ps);
   if (active) {
-    V* out = a.cplx_out + row * K;
-    for (int k = tid; k < K; k += T) gstore(out + k, cscale(lds[k + (k >> ps)], k == 0 ? a.a0 : a.a));
+    V* out = a.cplx_out + row * a.rdist;
+    for (int k = tid; k < K; k += T) out[k * a.kdist] = cscale(lds[k + (k >> ps)], k == 0 ? a.a0 : a.a);
   }
 }
 
@@ -52,13 +57,13 @@ __global__ __launch_bounds__(1024) void k_irfft_modes(SpecArgs<V> a) {
   V* lds_all = reinterpret_cast<V*>(smem_raw);
   const int T = a.threads, L = a.n, K = a.k, ps = a.ps;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long row = (long long)blockIdx.x * a.per_block + slot;
+  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
   const bool active = row < a.rows;
   V* lds = lds_all + slot * a.stride;
   if (active) {
-    const V* in = a.cplx_in + row * K;
+    const V* in = a.cplx_in + row * a.rdist;
     for (int i = tid; i < L; i += T)
-      lds[i + (i >> ps)] = i < K ? cscale(gload(in + i), i == 0 ? a.a0 : a.a) : V{0, 0};
+      lds[i + (i >> ps)] = i < K ? cscale(in[i * a.kdist], i == 0 ? a.a0 : a.a) : V{0, 0};
   }
   __syncthreads();
   lds_fft<V, PPT, +1>(lds, L, T, tid, a.pd, a.tw, active, ps);
@@ -75,16 +80,17 @@ template <typename V> __device__ __forceinline__ V conj_s(V a) { return V{a.x, -
 // on the fly from the kept modes.
 template <typename V> struct ModesHermSrc {
   static constexpr bool kFromLds = false;
-  gbl_cptr<V> m;  // kept modes of the row
+  gbl_cptr<V> m;  // kept modes of the row, mode j at m[j * kdist]
   gbl_cptr<V> w;  // exp(-2 pi i k / L), k <= h
+  long long kdist;
   int K, h;
   S<V> a0, a;
   __device__ V herm(int j) const {
     const V* M = (const V*)m;
     if (j >= K) return V{0, 0};
     if (j == 0) return V{a0 * M[0].x, 0};
-    if (j == h) return V{a * M[j].x, 0};
-    return cscale(M[j], a * (S<V>)0.5);
+    if (j == h) return V{a * M[j * kdist].x, 0};
+    return cscale(M[j * kdist], a * (S<V>)0.5);
   }
   __device__ V operator()(int k) const {
     const V hk = herm(k), hm = conj_s(herm(h - k));
@@ -114,21 +120,21 @@ __global__ __launch_bounds__(1024) void k_rfft_modes_half(SpecArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int T = a.threads, h = a.n / 2, K = a.k, ps = a.ps;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long row = (long long)blockIdx.x * a.per_block + slot;
+  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
   const bool active = row < a.rows;
   V* lds = as_lds(reinterpret_cast<V*>(smem_raw) + slot * a.stride);
   const V* z = reinterpret_cast<const V*>(a.real_in) + (active ? row : 0) * h;
   F::template run<-1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, h, T, tid, a.pd,
                       a.tw, active, ps);
   if (active) {
-    V* out = a.cplx_out + row * K;
+    V* out = a.cplx_out + row * a.rdist;
     for (int k = tid; k < K; k += T) {
       const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
       const V zk = lds[k0 + (k0 >> ps)], zm = conj_s(lds[km + (km >> ps)]);
       const V e = cscale(cadd(zk, zm), (S<V>)0.5);
       const V d = cscale(csub(zk, zm), (S<V>)0.5);
       const V X = cadd(e, cmul(a.rtw[k], V{d.y, -d.x}));
-      gstore(out + k, cscale(X, k == 0 ? a.a0 : a.a));
+      out[k * a.kdist] = cscale(X, k == 0 ? a.a0 : a.a);
     }
   }
 }
@@ -138,12 +144,13 @@ __global__ __launch_bounds__(1024) void k_irfft_modes_half(SpecArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int T = a.threads, h = a.n / 2, ps = a.ps;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long row = (long long)blockIdx.x * a.per_block + slot;
+  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
   const bool active = row < a.rows;
   V* lds = reinterpret_cast<V*>(smem_raw) + slot * a.stride;
   const long long r = active ? row : 0;
   V* y = reinterpret_cast<V*>(a.real_out) + r * h;
-  F::template run<+1>(ModesHermSrc<V>{(gbl_cptr<V>)(a.cplx_in + r * a.k), (gbl_cptr<V>)a.rtw, a.k, h, a.a0, a.a},
+  F::template run<+1>(ModesHermSrc<V>{(gbl_cptr<V>)(a.cplx_in + r * a.rdist), (gbl_cptr<V>)a.rtw, a.kdist, a.k, h,
+                                      a.a0, a.a},
                       GlobalSink<V>{(gbl_ptr<V>)y, (S<V>)2}, lds, h, T, tid, a.pd, a.tw, active, ps);
 }
 

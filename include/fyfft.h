@@ -258,6 +258,17 @@ fyfft_status fyfft_rfft_modes(const fyfft_plan* plan, const void* x, void* modes
 fyfft_status fyfft_irfft_modes(const fyfft_plan* plan, const void* modes, void* y, size_t k,
                                double scale0, double scale, void* stream);
 
+/* The same transforms with the modes laid out mode-major when layout is
+ * FYFFT_MODES_MODE_MAJOR: mode j of row r at modes[j*rows + r] (rows = the
+ * plan's batch), the layout of one GEMM per mode.  FYFFT_MODES_ROW_MAJOR
+ * is the layout of fyfft_rfft_modes / fyfft_irfft_modes. */
+#define FYFFT_MODES_ROW_MAJOR 0
+#define FYFFT_MODES_MODE_MAJOR 1
+fyfft_status fyfft_rfft_modes_ex(const fyfft_plan* plan, const void* x, void* modes, size_t k,
+                                 double scale0, double scale, int layout, void* stream);
+fyfft_status fyfft_irfft_modes_ex(const fyfft_plan* plan, const void* modes, void* y, size_t k,
+                                  double scale0, double scale, int layout, void* stream);
+
 /* Device bytes the plan holds (tables + workspace). */
 fyfft_status fyfft_plan_workspace_bytes(const fyfft_plan* plan, size_t* bytes);
 

@@ -109,3 +109,22 @@ def test_spectral_conv_vs_analytic(fy, cuda, shape, modes):
     ey, ex, ew = rel(y, yr), rel(xd.grad, xr.grad), rel(wd.grad, wr.grad)
     print(f"spectral conv {shape} K={modes}: y {ey:.2e} dx {ex:.2e} dW {ew:.2e}")
     assert ey < 2e-6 and ex < 2e-6 and ew < 2e-6, (ey, ex, ew)
+
+
+@pytest.mark.parametrize("L,k", [(4096, 512), (6720, 100), (243, 50), (1024, 1024)])
+def test_modes_mode_major_layout(fy, cuda, L, k):
+    """Mode-major output/input equals the row-major one transposed, for the
+    half-length and the full-length forms."""
+    import torch
+    rng = np.random.default_rng(L * 3 + k)
+    x = torch.tensor(rng.standard_normal((4, 6, L)), dtype=torch.float32, device="cuda")
+    rm = fy.rfft_modes(x, k, 0.5, 2.0)
+    mm = fy.rfft_modes(x, k, 0.5, 2.0, mode_major=True)
+    assert mm.shape == (k, 4, 6)
+    assert torch.equal(mm.permute(1, 2, 0), rm)
+    m = torch.tensor(rng.standard_normal((4, 6, k)) + 1j * rng.standard_normal((4, 6, k)),
+                     dtype=torch.complex64, device="cuda")
+    yr = fy.irfft_modes(m, L, 1.0 / L, 2.0 / L)
+    ym = fy.irfft_modes(m.permute(2, 0, 1).contiguous(), L, 1.0 / L, 2.0 / L, mode_major=True)
+    assert ym.shape == (4, 6, L)
+    assert torch.equal(ym, yr)
