@@ -1245,7 +1245,7 @@ static fyfft_status exec_real_fused(const fyfft_plan* p, const void* in, void* o
   a.rtw = (const V*)p->rtw;
   a.scale = (dev::S<V>)(r2c ? 1.0 : 1.0 / (double)len);
   const int mode = half ? (r2c ? kRealR2cHalf : kRealC2rHalf) : (r2c ? kRealR2cFull : kRealC2rFull);
-  HIP_TRY(launch_real_lds(p->prec, mode, &a, g, s));
+  HIP_TRY(launch_real_lds(p->prec, mode, &a, g, sp->lds.fixed >= 0 ? (int)len : 0, s));
   return FYFFT_SUCCESS;
 }
 
@@ -1325,8 +1325,7 @@ static fyfft_status create_real_impl(fyfft_plan** out, size_t n, size_t batch, f
   const size_t len = half ? h : n;
   p->rchunk = fused ? batch : std::min(batch, std::max<size_t>(1, kWorkspaceCap / (len * elem)));
   fyfft_plan* sp = nullptr;
-  fyfft_status st = create_plan_impl(&sp, len, fused ? 1 : p->rchunk, precision,
-                                     fused ? FYFFT_FORM_PLAIN : FYFFT_FORM_AUTO);
+  fyfft_status st = create_plan_impl(&sp, len, fused ? 1 : p->rchunk, precision, FYFFT_FORM_AUTO);
   if (!st) {
     p->subs.push_back(sp);
     if (fused && sp->kind != kLds) st = FYFFT_UNSUPPORTED;

@@ -65,6 +65,24 @@ template <typename V, int N, int T, int... Rs> struct FixedFft {
   }
 };
 
+// I/O policies for whole LDS-resident transforms with source/sink functors:
+// runtime pass list, or compile-time passes (N, threads T, pad shift PS and
+// radices fixed; the launch geometry must match).
+template <typename V, int PPT> struct GenericIo {
+  template <int D, typename Src, typename Dst>
+  static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int n, int T, int tid,
+                                             const PassDesc& pd, const V* tw, bool active, int ps) {
+    lds_fft_io<V, PPT, D>(src, dst, lds, n, T, tid, pd, tw, active, ps);
+  }
+};
+template <typename V, int N, int T, int PS, int... Rs> struct FixedIo {
+  template <int D, typename Src, typename Dst>
+  static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int, int, int tid, const PassDesc&,
+                                             const V* tw, bool active, int) {
+    fixed_io_passes<V, D, N, T, 1, Src, Dst, Rs...>(src, dst, as_lds(lds), PS, tid, tw, active);
+  }
+};
+
 // ------------------------------------------------------------ functors
 template <typename V> using gbl_ptr = __attribute__((address_space(1))) V*;
 

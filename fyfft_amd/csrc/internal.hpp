@@ -195,7 +195,9 @@ template <typename V> struct RealArgs {
   const V* rtw = nullptr;  // exp(-2*pi*i*k/n), k <= n/2 (half forms)
   dev::S<V> scale = 1;
 };
-hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, hipStream_t s);
+// fixed_len: the complex length when g is the geometry of a compile-time
+// kernel of that length (compile-time passes), 0 for the runtime passes.
+hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, int fixed_len, hipStream_t s);
 // Elementwise passes of the multi-pass real forms over `rows` rows of real
 // length n: Post (half spectrum Z -> X[0..n/2], times scale), Pre (X -> Z),
 // Expand (real -> complex), Truncate (full spectrum -> bins 0..n/2, times

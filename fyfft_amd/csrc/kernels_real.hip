@@ -88,13 +88,13 @@ template <typename V> struct RealSink {  // y[i] = Re(v) * scale
 
 }  // namespace
 
-// One workgroup slot per row; MODE selects the form.
-template <typename V, int PPT, int MODE>
+// One workgroup slot per row; MODE selects the form, F the pass policy.
+template <typename V, typename F, int MODE>
 __global__ __launch_bounds__(1024) void k_real_lds(RealArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int T = a.threads;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long t = (long long)blockIdx.x * a.per_block + slot;
+  const long long t = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
   const bool active = t < a.rows;
   const long long row = active ? t : 0;
   V* lds = reinterpret_cast<V*>(smem_raw) + slot * a.stride;
@@ -102,8 +102,8 @@ __global__ __launch_bounds__(1024) void k_real_lds(RealArgs<V> a) {
   const int n = a.n, h = n / 2, ps = a.ps;
   if constexpr (MODE == kRealR2cHalf) {
     const V* z = (const V*)a.in + row * h;
-    lds_fft_io<V, PPT, -1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{ldsp, ps}, lds, h, T, tid, a.pd,
-                           a.tw, active, ps);
+    F::template run<-1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{ldsp, ps}, lds, h, T, tid, a.pd, a.tw, active,
+                        ps);
     if (active) {
       V* X = (V*)a.out + row * (h + 1);
       const V* L = (const V*)ldsp;
@@ -115,19 +115,19 @@ __global__ __launch_bounds__(1024) void k_real_lds(RealArgs<V> a) {
   } else if constexpr (MODE == kRealC2rHalf) {
     const V* X = (const V*)a.in + row * (h + 1);
     V* z = (V*)a.out + row * h;
-    lds_fft_io<V, PPT, +1>(HermHalfSrc<V>{(gbl_cptr<V>)X, (gbl_cptr<V>)a.rtw, h},
-                           GlobalSink<V>{(gbl_ptr<V>)z, a.scale}, lds, h, T, tid, a.pd, a.tw, active, ps);
+    F::template run<+1>(HermHalfSrc<V>{(gbl_cptr<V>)X, (gbl_cptr<V>)a.rtw, h},
+                        GlobalSink<V>{(gbl_ptr<V>)z, a.scale}, lds, h, T, tid, a.pd, a.tw, active, ps);
   } else if constexpr (MODE == kRealR2cFull) {
     const S<V>* x = (const S<V>*)a.in + row * n;
     V* X = (V*)a.out + row * (n / 2 + 1);
-    lds_fft_io<V, PPT, -1>(RealSrc<V>{(const __attribute__((address_space(1))) S<V>*)x},
-                           HalfSink<V>{(gbl_ptr<V>)X, n / 2}, lds, n, T, tid, a.pd, a.tw, active, ps);
+    F::template run<-1>(RealSrc<V>{(const __attribute__((address_space(1))) S<V>*)x},
+                        HalfSink<V>{(gbl_ptr<V>)X, n / 2}, lds, n, T, tid, a.pd, a.tw, active, ps);
   } else {  // kRealC2rFull
     const V* X = (const V*)a.in + row * (n / 2 + 1);
     S<V>* y = (S<V>*)a.out + row * n;
-    lds_fft_io<V, PPT, +1>(HermFullSrc<V>{(gbl_cptr<V>)X, n},
-                           RealSink<V>{(__attribute__((address_space(1))) S<V>*)y, a.scale}, lds, n, T,
-                           tid, a.pd, a.tw, active, ps);
+    F::template run<+1>(HermFullSrc<V>{(gbl_cptr<V>)X, n},
+                        RealSink<V>{(__attribute__((address_space(1))) S<V>*)y, a.scale}, lds, n, T, tid, a.pd,
+                        a.tw, active, ps);
   }
 }
 
@@ -183,14 +183,14 @@ __global__ void k_real_part(const V* W, S<V>* y, long long count, S<V> scale) {
   if (i < count) y[i] = W[i].x * scale;
 }
 
-template <typename V, int PPT>
+template <typename V, typename F>
 static hipError_t go_real(int mode, const RealArgs<V>& a, const LdsGeom& g, hipStream_t s) {
   void (*k)(RealArgs<V>) = nullptr;
   switch (mode) {
-    case kRealR2cHalf: k = k_real_lds<V, PPT, kRealR2cHalf>; break;
-    case kRealC2rHalf: k = k_real_lds<V, PPT, kRealC2rHalf>; break;
-    case kRealR2cFull: k = k_real_lds<V, PPT, kRealR2cFull>; break;
-    case kRealC2rFull: k = k_real_lds<V, PPT, kRealC2rFull>; break;
+    case kRealR2cHalf: k = k_real_lds<V, F, kRealR2cHalf>; break;
+    case kRealC2rHalf: k = k_real_lds<V, F, kRealC2rHalf>; break;
+    case kRealR2cFull: k = k_real_lds<V, F, kRealR2cFull>; break;
+    case kRealC2rFull: k = k_real_lds<V, F, kRealC2rFull>; break;
     default: return hipErrorInvalidValue;
   }
   if (g.lds_bytes > 65536)
@@ -200,15 +200,29 @@ static hipError_t go_real(int mode, const RealArgs<V>& a, const LdsGeom& g, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, hipStream_t s) {
+hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, int fixed_len, hipStream_t s) {
   if (prec == kF64) {
     const auto& a = *static_cast<const RealArgs<double2>*>(args);
     if (a.rows == 0) return hipSuccess;
-    return g.ppt <= 8 ? go_real<double2, 8>(mode, a, g, s) : go_real<double2, 16>(mode, a, g, s);
+    switch (fixed_len) {
+      case 1024: return go_real<double2, FixedIo<double2, 1024, 64, 30, 16, 8, 8>>(mode, a, g, s);
+      case 4096: return go_real<double2, FixedIo<double2, 4096, 256, 30, 16, 16, 16>>(mode, a, g, s);
+    }
+    return g.ppt <= 8 ? go_real<double2, GenericIo<double2, 8>>(mode, a, g, s)
+                      : go_real<double2, GenericIo<double2, 16>>(mode, a, g, s);
   }
   const auto& a = *static_cast<const RealArgs<float2>*>(args);
   if (a.rows == 0) return hipSuccess;
-  return g.ppt <= 16 ? go_real<float2, 16>(mode, a, g, s) : go_real<float2, 32>(mode, a, g, s);
+  switch (fixed_len) {
+    case 256: return go_real<float2, FixedIo<float2, 256, 16, 30, 16, 16>>(mode, a, g, s);
+    case 512: return go_real<float2, FixedIo<float2, 512, 32, 30, 8, 8, 8>>(mode, a, g, s);
+    case 1024: return go_real<float2, FixedIo<float2, 1024, 64, 30, 16, 8, 8>>(mode, a, g, s);
+    case 2048: return go_real<float2, FixedIo<float2, 2048, 128, 30, 16, 16, 8>>(mode, a, g, s);
+    case 4096: return go_real<float2, FixedIo<float2, 4096, 256, 30, 16, 16, 16>>(mode, a, g, s);
+    case 8192: return go_real<float2, FixedIo<float2, 8192, 512, 30, 16, 8, 8, 8>>(mode, a, g, s);
+  }
+  return g.ppt <= 16 ? go_real<float2, GenericIo<float2, 16>>(mode, a, g, s)
+                     : go_real<float2, GenericIo<float2, 32>>(mode, a, g, s);
 }
 
 static inline dim3 grid_of(long long count) { return dim3((unsigned)((count + 255) / 256)); }

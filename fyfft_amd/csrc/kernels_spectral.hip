@@ -100,21 +100,6 @@ template <typename V> struct ModesHermSrc {
   }
 };
 
-template <typename V, int PPT> struct GenericIo {
-  template <int D, typename Src, typename Dst>
-  static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int n, int T, int tid,
-                                             const PassDesc& pd, const V* tw, bool active, int ps) {
-    lds_fft_io<V, PPT, D>(src, dst, lds, n, T, tid, pd, tw, active, ps);
-  }
-};
-template <typename V, int N, int T, int PS, int... Rs> struct FixedIo {
-  template <int D, typename Src, typename Dst>
-  static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int, int, int tid, const PassDesc&,
-                                             const V* tw, bool active, int) {
-    fixed_io_passes<V, D, N, T, 1, Src, Dst, Rs...>(src, dst, as_lds(lds), PS, tid, tw, active);
-  }
-};
-
 template <typename V, typename F>
 __global__ __launch_bounds__(1024) void k_rfft_modes_half(SpecArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
