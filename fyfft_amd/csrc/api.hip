@@ -557,14 +557,46 @@ static fyfft_status plan_rader(fyfft_plan* p) {
   return st;
 }
 
+// Estimated cost of a Bluestein convolution of length c (0: not plannable):
+// c times its pass count (forward and inverse FFT of c, whether in LDS or as
+// four-step column/row passes), times 1.6 for the four-step's extra HBM
+// round trips, times 0.7 when both four-step factors have compile-time
+// sub-transforms.  Measured on MI355X against the smallest smooth length:
+// N = 10007 at M = 20480 (4 passes) instead of 20160 (6): 0.367 -> 0.335 ms
+// for 1024 transforms; N = 1000003 at M = 2^21 instead of 2000376 =
+// 1372 x 1458: 1.29 -> 0.63 ms for 16.
+static double conv_length_cost(size_t c, size_t elem, size_t cap, int prec) {
+  if (!is_smooth7(c)) return 0;
+  LdsGeom g;
+  if (lds_geometry(c, elem, cap, &g)) return (double)c * (double)radices_for(c).size();
+  size_t n1, n2;
+  LdsGeom a, b;
+  if (!four_step_split(c, kFactorCap, &n1, &n2) || !fs_geometry(n1, elem, cap, &a) || !fs_geometry(n2, elem, cap, &b))
+    return 0;
+  auto fixed = [&](size_t len, const LdsGeom& q) {
+    const std::vector<int> r = radices_for(len);
+    dev::PassDesc pd{};
+    pd.npass = (int)r.size();
+    for (size_t i = 0; i < r.size() && i < (size_t)dev::kMaxPasses; ++i) pd.radix[i] = r[i];
+    return find_fs_fixed(prec, pd, (int)len, q.threads) != 0;
+  };
+  const double passes = (double)(radices_for(n1).size() + radices_for(n2).size());
+  return (double)c * passes * 1.6 * (fixed(n1, a) && fixed(n2, b) ? 0.7 : 1.0);
+}
+
 static fyfft_status plan_bluestein(fyfft_plan* p) {
   const size_t n = p->n, elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
+  // the cheapest plannable smooth length in [2N-1, 1.25 (2N-1)], else the
+  // first plannable one above
   size_t m = 0;
-  for (size_t c = 2 * n - 1; c < 4 * (2 * n - 1) + 64; ++c) {
-    if (!is_smooth7(c)) continue;
-    LdsGeom g;
-    if (lds_geometry(c, elem, cap, &g) || fs_possible(c, elem, cap)) { m = c; break; }
+  double best = 0;
+  const size_t lo = 2 * n - 1, hi = lo + lo / 4;
+  for (size_t c = lo; c <= hi; ++c) {
+    const double cost = conv_length_cost(c, elem, cap, p->prec);
+    if (cost > 0 && (m == 0 || cost < best)) { m = c; best = cost; }
   }
+  for (size_t c = hi + 1; !m && c < 4 * lo + 64; ++c)
+    if (conv_length_cost(c, elem, cap, p->prec) > 0) m = c;
   if (!m) return FYFFT_UNSUPPORTED;
   fyfft_status st = plan_conv_structure(p, m, false);
   if (st) return st;
