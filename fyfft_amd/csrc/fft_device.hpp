@@ -273,6 +273,20 @@ __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
   }
 }
 
+// ---------------------------------------------------------------- uniform values
+// Copy of a trivially copyable descriptor with every 32-bit word taken from
+// the first active lane: the compiler then keeps it in scalar registers
+// (a descriptor reached through a pointer argument is otherwise per-lane).
+template <typename T> __device__ __forceinline__ T make_uniform(const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "4-byte multiple expected");
+  struct W { int w[sizeof(T) / 4]; };
+  const W src = __builtin_bit_cast(W, v);
+  W dst;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst.w[i] = __builtin_amdgcn_readfirstlane(src.w[i]);
+  return __builtin_bit_cast(T, dst);
+}
+
 // ---------------------------------------------------------------- XCD mapping
 // The dispatcher hands workgroup b of a launch to XCD (b mod 8).  For passes
 // where several consecutive logical workgroups work on one transform (the
@@ -306,6 +320,18 @@ struct PassDesc {
   int npass;
   int radix[kMaxPasses];
 };
+
+// The pass list packed 4 bits per radix (radices are at most 16, stored as
+// R - 1) so that the runtime pass loops read the radix of pass p with a
+// shift instead of indexing the array, which would keep the descriptor in
+// scratch memory.
+__device__ __forceinline__ unsigned long long pack_radices(const PassDesc& pd) {
+  unsigned long long code = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxPasses; ++i) code |= (unsigned long long)((pd.radix[i] - 1) & 15) << (4 * i);
+  return code;
+}
+__device__ __forceinline__ int radix_at(unsigned long long code, int p) { return (int)((code >> (4 * p)) & 15) + 1; }
 
 __host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -394,9 +420,10 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
                                         const V* __restrict__ tw_gen, bool active, int ps) {
   const lds_ptr<V> lds = (lds_ptr<V>)lds_gen;
   const gbl_cptr<V> tw = (gbl_cptr<V>)tw_gen;
+  const unsigned long long code = pack_radices(pd);
   int Ls = 1;
   for (int p = 0; p < pd.npass; ++p) {
-    const int R = pd.radix[p];
+    const int R = radix_at(code, p);
     switch (R) {
       case 2: lds_pass_ool<V, 2, cdiv(PPT, 2), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
       case 3: lds_pass_ool<V, 3, cdiv(PPT, 3), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
@@ -533,7 +560,7 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   const int P = pd.npass;
   if (P <= 1) {
     if (P == 1)
-      radix_switch(pd.radix[0], [&](auto rc) {
+      radix_switch(radix_at(pack_radices(pd), 0), [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, 1, T, tid, tw, active);
       });
@@ -545,15 +572,16 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
     if constexpr (Dst::kToLds) __syncthreads();
     return;
   }
+  const unsigned long long code = pack_radices(pd);
   int Ls = 1;
   for (int p = 0; p < P; ++p) {
-    radix_switch(pd.radix[p], [&](auto rc) {
+    radix_switch(radix_at(code, p), [&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (p == 0) io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, Ls, T, tid, tw, active);
       else if (p == P - 1) io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, dst, N, Ls, T, tid, tw, active);
       else io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, lk, N, Ls, T, tid, tw, active);
     });
-    Ls *= pd.radix[p];
+    Ls *= radix_at(code, p);
   }
 }
 

@@ -34,7 +34,7 @@ struct TaskCtx {
 
 template <typename V, int DIR, int PPT>
 __device__ __noinline__ void task_lds(const GLen<V>& g, const GTask& t, const TaskCtx<V>& c) {
-  LdsBody<V> b = g.lds;
+  LdsBody<V> b = make_uniform(g.lds);
   if (c.tables) {
     b.chirp = g.chirp_inv;
     b.kern = g.kern_inv;
@@ -52,7 +52,7 @@ __device__ __noinline__ void task_lds(const GLen<V>& g, const GTask& t, const Ta
 template <typename V, int DIR, int PPT, int MODE>
 __device__ __noinline__ void task_fs(const GLen<V>& g, const GTask& t, const TaskCtx<V>& c) {
   constexpr bool col = MODE != kFsPlainB && MODE != kFsConvB;
-  FsArgs<V> a = col ? g.col : g.row;
+  FsArgs<V> a = make_uniform(col ? g.col : g.row);
   if (c.tables) {
     a.chirp = g.chirp_inv;
     a.kern = g.kern_inv;
