@@ -45,6 +45,12 @@ template <typename V, int PPT> struct GenericFft {
                                              const V* tw, bool active, int ps) {
     lds_fft<V, PPT, D>(lds, n, t, tid, pd, tw, active, ps);
   }
+  // the same transform with the first pass reading src and the last writing dst
+  template <int D, typename Src, typename Dst>
+  static __device__ __forceinline__ void run_io(Src src, Dst dst, V* lds, int n, int t, int tid,
+                                                const PassDesc& pd, const V* tw, bool active, int ps) {
+    lds_fft_io<V, PPT, D>(src, dst, lds, n, t, tid, pd, tw, active, ps);
+  }
 };
 
 template <typename V, int D, int N, int T, int Ls, int R, int... Rest>
@@ -62,6 +68,11 @@ template <typename V, int N, int T, int... Rs> struct FixedFft {
   static __device__ __forceinline__ void run(V* lds, int, int, int tid, const PassDesc&,
                                              const V* tw, bool active, int ps) {
     fixed_lds_passes<V, D, N, T, 1, Rs...>(as_lds(lds), tid, tw, active, ps);
+  }
+  template <int D, typename Src, typename Dst>
+  static __device__ __forceinline__ void run_io(Src src, Dst dst, V* lds, int, int, int tid, const PassDesc&,
+                                                const V* tw, bool active, int ps) {
+    fixed_io_passes<V, D, N, T, 1, Src, Dst, Rs...>(src, dst, as_lds(lds), ps, tid, tw, active);
   }
 };
 
@@ -145,6 +156,21 @@ template <typename V> struct BlueSink {  // y[k] = v * chirp[k] * scale, k < N
   S<V> scale;
   __device__ void operator()(int k, V v) const {
     if (k < n) ((V*)y)[k] = cscale(cmul(v, ((const V*)chirp)[k]), scale);
+  }
+};
+
+// Conjugate four-step twiddle at the store: y[k2] = v * conj(w_N^(k2*k1)),
+// w_N^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)].
+template <typename V> struct TwSink {
+  static constexpr bool kToLds = false;
+  gbl_ptr<V> y;
+  gbl_cptr<V> hi, lo;
+  int lo_bits;
+  long long k1;
+  __device__ void operator()(int k2, V v) const {
+    const long long m = (long long)k2 * k1;
+    const V w = cmul(((const V*)hi)[m >> lo_bits], ((const V*)lo)[m & ((1LL << lo_bits) - 1)]);
+    ((V*)y)[k2] = cmulc(v, w);
   }
 };
 
@@ -279,18 +305,17 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
   const int rows = min(C, N1 - r0);
-  const int total = rows * N2;
-  const V* src = tmp + (long long)r0 * N2;
-  staged<P>(threadIdx.x, blockDim.x, total, [&](int i) { return src[i]; },
-               [&](int i, V v) {
-                 const int c = i / N2, n2 = i - c * N2;
-                 lds_all[c * stride + n2 + (n2 >> ps)] = v;
-               });
-  __syncthreads();
   const bool active = slot < rows;
   V* lds = lds_all + (slot < C ? slot : 0) * stride;
-  const int k1 = r0 + slot;
   if constexpr (MODE == kFsPlainB) {
+    const int total = rows * N2;
+    const V* src = tmp + (long long)r0 * N2;
+    staged<P>(threadIdx.x, blockDim.x, total, [&](int i) { return src[i]; },
+              [&](int i, V v) {
+                const int c = i / N2, n2 = i - c * N2;
+                lds_all[c * stride + n2 + (n2 >> ps)] = v;
+              });
+    __syncthreads();
     F::template run<DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
@@ -298,33 +323,21 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
       gstore(out + (long long)(r0 + c) + (long long)N1 * k2,
              cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale));
     }
-  } else {  // kFsConvB
-    F::template run<-1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
-    if (x0 != nullptr && r0 == 0 && threadIdx.x == 0) {
-      // Rader: X[0] = x0 + sum_{n>=1} x[n] = x0 + A[0] (row k1 = 0, k2 = 0)
-      out[0] = cscale(cadd(*x0, lds_all[0]), a.scale);
-    }
-    __syncthreads();
-    if (active) {
-      const V* kr = a.kern + (long long)k1 * N2;
-      staged<P>(tid, a.threads, N2, [&](int k2) { return kr[k2]; },
-                   [&](int k2, V k) {
-                     const int li = k2 + (k2 >> ps);
-                     lds[li] = cmul(lds[li], k);
-                   });
-    }
-    __syncthreads();
-    F::template run<+1>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
-    V* dst = tmp + (long long)r0 * N2;
-    staged<P>(threadIdx.x, blockDim.x, total,
-                 [&](int i) {
-                   const int c = i / N2, n2 = i - c * N2;
-                   return big_twiddle<+1>(a, (long long)n2 * (r0 + c));
-                 },
-                 [&](int i, V w) {
-                   const int c = i / N2, n2 = i - c * N2;
-                   dst[i] = cmul(lds_all[c * stride + n2 + (n2 >> ps)], w);
-                 });
+  } else {  // kFsConvB: one row per slot, read and written in place, contiguous
+    // forward FFT straight from HBM, kernel multiply as the last pass's sink,
+    // inverse FFT with the conjugate four-step twiddle applied at the store.
+    // Rader: X[0] = x0 + sum_{n>=1} x[n] = x0 + A[0] (row 0, bin 0).
+    const long long k1 = r0 + (active ? slot : 0);
+    V* row = tmp + k1 * N2;
+    const bool bin0 = x0 != nullptr && r0 == 0 && slot == 0;
+    const V xz = bin0 ? *x0 : V{0, 0};
+    F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row},
+                           KernSink<V>{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)(a.kern + k1 * N2),
+                                       bin0 ? (gbl_ptr<V>)out : nullptr, xz, a.scale},
+                           lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    F::template run_io<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps},
+                           TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1},
+                           lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
   }
 }
 
