@@ -90,6 +90,12 @@ struct fyfft_plan {
   void* chirp[2] = {nullptr, nullptr};
   int* perm_in = nullptr;
   int* perm_out = nullptr;
+  // four-step Rader: bucketed output permutation (chunk size rb_q; tables
+  // of L entries) and the chunk x L staging buffer it runs through
+  int rb_q = 0;
+  int* rb_lout = nullptr;  // LDS index (closing column pass)
+  int2* rb_out = nullptr;  // {zw position, offset in the chunk of y}
+  void* zw = nullptr;
   // real plans: complex sub-plan subs[0] of length n/2 (half forms) or n
   int real_form = -1;
   // strided plans (many / multi-dimensional): passes run in order, the first
@@ -524,6 +530,55 @@ static fyfft_status store_spectrum(fyfft_plan* p, double2* b, int slot) {
   return FYFFT_SUCCESS;
 }
 
+// Tables of the bucketed Rader output permutation (kernels_rader.hip) for
+// a four-step plan L = N1 x N2 with C columns per column-pass workgroup:
+// conv index n = k1*N2 + n2 belongs to column group b = n2 / C and its
+// result goes to y[pout[n]], in chunk (pout[n] - 1) / Q of y.  Group b's
+// results are sorted by destination chunk (lout: the LDS index to read,
+// c*stride + k1 + pad, for zw position b*N1*C + i); chunk k's elements in
+// group order (out: {zw position, offset in the chunk}).
+static fyfft_status plan_rader_buckets(fyfft_plan* p, const std::vector<int>& pout) {
+  const size_t L = p->m, N1 = p->n1, N2 = p->n2;
+  const int C = p->gcol.per_block, ps = p->gcol.pad_shift;
+  const int stride = dev::lds_len((int)N1, ps);
+  const size_t Q = 4096, K = (L + Q - 1) / Q, G = (N2 + C - 1) / C;
+  std::vector<int> lout(L), dest(L);
+  std::vector<int2> out2(L);
+  std::vector<std::vector<size_t>> out_lo(G, std::vector<size_t>(K + 1, 0));
+  for (size_t b = 0; b < G; ++b) {
+    const size_t c0 = b * (size_t)C, ncols = std::min((size_t)C, N2 - c0), start = c0 * N1;
+    std::vector<size_t> cnt(K + 1, 0);
+    for (size_t k1 = 0; k1 < N1; ++k1)
+      for (size_t c = 0; c < ncols; ++c) ++cnt[((size_t)pout[k1 * N2 + c0 + c] - 1) / Q + 1];
+    for (size_t k = 0; k < K; ++k) cnt[k + 1] += cnt[k];
+    out_lo[b] = cnt;
+    std::vector<size_t> fill = cnt;
+    for (size_t k1 = 0; k1 < N1; ++k1)
+      for (size_t c = 0; c < ncols; ++c) {
+        const size_t d = (size_t)pout[k1 * N2 + c0 + c], i = fill[(d - 1) / Q]++;
+        lout[start + i] = (int)(c * (size_t)stride + k1 + (k1 >> ps));
+        dest[start + i] = (int)d;
+      }
+  }
+  for (size_t k = 0, pos = 0; k < K; ++k)
+    for (size_t b = 0; b < G; ++b) {
+      const size_t start = b * (size_t)C * N1;
+      for (size_t i = out_lo[b][k]; i < out_lo[b][k + 1]; ++i, ++pos)
+        out2[pos] = int2{(int)(start + i), (int)((size_t)dest[start + i] - 1 - k * Q)};
+    }
+  void* d = nullptr;
+  fyfft_status st;
+  if ((st = p->alloc(L * sizeof(int), &d))) return st;
+  p->rb_lout = (int*)d;
+  HIP_TRY(hipMemcpy(d, lout.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  if ((st = p->alloc(L * sizeof(int2), &d))) return st;
+  p->rb_out = (int2*)d;
+  HIP_TRY(hipMemcpy(d, out2.data(), L * sizeof(int2), hipMemcpyHostToDevice));
+  if ((st = p->alloc(p->chunk * L * elem_bytes(p->prec), &p->zw))) return st;
+  p->rb_q = (int)Q;
+  return FYFFT_SUCCESS;
+}
+
 static fyfft_status plan_rader(fyfft_plan* p) {
   const size_t n = p->n, L = n - 1;
   fyfft_status st = plan_conv_structure(p, L, true);
@@ -551,9 +606,11 @@ static fyfft_status plan_rader(fyfft_plan* p) {
     st = e != hipSuccess ? from_hip(e, "rader kernel") : store_spectrum(p, bb, slot);
   }
   (void)hipFree(bb);
+  if (!st && p->kind == kRaderFs) st = plan_rader_buckets(p, pout);
   if (!st)
     p->desc = std::string(p->kind == kRaderLds ? "rader-lds" : "rader-fourstep") + " N=" +
-              std::to_string(n) + " L=" + std::to_string(L) + " g=" + std::to_string(g);
+              std::to_string(n) + " L=" + std::to_string(L) + " g=" + std::to_string(g) +
+              (p->rb_q ? " bucketed-permutation Q=" + std::to_string(p->rb_q) : "");
   return st;
 }
 
@@ -662,9 +719,20 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     a.chirp = (const V*)p->chirp[slot];
     a.x0buf = (V*)p->x0buf;
     int colA, colC = -1, rowB;
+    // Rader: the input permutation as an element gather in the column pass
+    // (measured faster than gathering it through chunked runs as well), the
+    // output permutation through buckets
+    const bool buckets = p->kind == kRaderFs && p->rb_q > 0;
     if (p->kind == kFourStep) { colA = kFsPlainA; rowB = kFsPlainB; }
-    else if (p->kind == kRaderFs) { colA = kFsRaderA; rowB = kFsConvB; colC = kFsRaderC; }
-    else { colA = kFsBlueA; rowB = kFsConvB; colC = kFsBlueC; }
+    else if (p->kind == kRaderFs) {
+      colA = kFsRaderA;
+      rowB = kFsConvB;
+      colC = buckets ? kFsRaderC2 : kFsRaderC;
+    } else { colA = kFsBlueA; rowB = kFsConvB; colC = kFsBlueC; }
+    if (buckets) {
+      a.zw = (V*)p->zw;
+      a.zw_dist = (long long)L;
+    }
 
     fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
     a.perm = p->perm_in;
@@ -674,8 +742,12 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     if (colC >= 0) {
       fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
       a.perm = p->perm_out;
+      a.lidx = p->rb_lout;
       HIP_TRY(launch_fs_pass(p->prec, true, colC, +1, &a, cb, p->gcol, p->fixed_col, s));
     }
+    if (buckets)
+      HIP_TRY(launch_rader_bucket(p->prec, p->zw, a.out, (long long)cb, (long long)L, p->rb_q, p->rb_out,
+                                  (long long)L, (long long)dist, s));
   }
   return FYFFT_SUCCESS;
 }

@@ -226,7 +226,7 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
 // row segments.
 template <typename V, typename F, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* tmp, V* out, V* x0,
-                                            int c0, V* lds_in) {
+                                            int c0, V* lds_in, V* zw = nullptr) {
   constexpr int P = F::kPpt;
   V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c, lc = a.log2c;
@@ -236,20 +236,20 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
   const int total = N1 << lc;
   const int ncols = min(C, N2 - c0);
   staged<P>(threadIdx.x, blockDim.x, total,
-               [&](int i) {
-                 const int n1 = i >> lc, c = i & (C - 1);
-                 const int n = n1 * N2 + c0 + (c < ncols ? c : 0);
-                 V v;
-                 if constexpr (MODE == kFsPlainA) v = gload(x + n);
-                 else if constexpr (MODE == kFsRaderA) v = x[a.perm[n]];
-                 else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
-                 else v = tmp[n];
-                 return v;
-               },
-               [&](int i, V v) {
-                 const int n1 = i >> lc, c = i & (C - 1);
-                 if (c < ncols) lds_all[c * stride + n1 + (n1 >> ps)] = v;
-               });
+            [&](int i) {
+              const int n1 = i >> lc, c = i & (C - 1);
+              const int n = n1 * N2 + c0 + (c < ncols ? c : 0);
+              V v;
+              if constexpr (MODE == kFsPlainA) v = gload(x + n);
+              else if constexpr (MODE == kFsRaderA) v = x[a.perm[n]];
+              else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
+              else v = tmp[n];
+              return v;
+            },
+            [&](int i, V v) {
+              const int n1 = i >> lc, c = i & (C - 1);
+              if (c < ncols) lds_all[c * stride + n1 + (n1 >> ps)] = v;
+            });
   if constexpr (MODE == kFsRaderA) {
     if (c0 == 0 && threadIdx.x == 0) *x0 = x[0];
   }
@@ -257,8 +257,13 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
   const bool active = slot < C && slot < ncols;
   F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, a.tw, active, ps);
   V xz = V{0, 0};
-  if constexpr (MODE == kFsRaderC) xz = *x0;
-  if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
+  if constexpr (MODE == kFsRaderC || MODE == kFsRaderC2) xz = *x0;
+  if constexpr (MODE == kFsRaderC2) {
+    // runs ordered by destination chunk, written contiguously
+    const long long bstart = (long long)c0 * N1;  // this column group's first element
+    for (int i = threadIdx.x; i < N1 * ncols; i += blockDim.x)
+      zw[bstart + i] = cscale(cadd(lds_all[a.lidx[bstart + i]], xz), a.scale);
+  } else if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
     staged<P>(threadIdx.x, blockDim.x, total,
                  [&](int i) {
                    const int k1 = i >> lc, c = i & (C - 1), n2 = c0 + c;

@@ -57,7 +57,8 @@ enum FsMode {
   kFsRaderC = 3,  // inverse column pass, Rader scatter + x0
   kFsBlueC = 4,   // inverse column pass, post-chirp, first n_orig outputs
   kFsPlainB = 5,  // row pass of a plain transform (tmp -> out, transposed)
-  kFsConvB = 6    // row pass of a convolution (fwd, * kernel, inv, conj twiddle)
+  kFsConvB = 6,   // row pass of a convolution (fwd, * kernel, inv, conj twiddle)
+  kFsRaderC2 = 8  // inverse column pass writing bucket runs (zw) + x0
 };
 
 template <typename V> struct FsArgs {
@@ -80,6 +81,11 @@ template <typename V> struct FsArgs {
   const V* kern = nullptr;     // kernel spectrum in [k1][k2] order, 1/L folded in
   V* x0buf = nullptr;          // Rader: x[0] per transform
   dev::S<V> scale = 1;
+  // bucketed Rader output (kFsRaderC2): element i of column group b is
+  // read from LDS element lidx[b*N1*C + i] and written to zw[b*N1*C + i]
+  V* zw = nullptr;
+  long long zw_dist = 0;
+  const int* lidx = nullptr;
 };
 
 // One four-step pass; `args` points at FsArgs<float2> or FsArgs<double2>.
@@ -149,6 +155,14 @@ hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGe
 // kernels' geometry, 0 the runtime passes.
 hipError_t launch_spectral_half(int prec, bool inverse, const void* args, const LdsGeom& g, int fixed_h,
                                 hipStream_t s);
+
+// --- kernels_rader.hip
+// The Rader output permutation of four-step plans through LDS: chunk k of y
+// (Q consecutive elements after y[0]) gathers its elements from zw (runs
+// written by the closing column pass) at pos[i].x, places them at offset
+// pos[i].y and is stored contiguously.  One workgroup per (transform, chunk).
+hipError_t launch_rader_bucket(int prec, const void* zw, void* y, long long rows, long long L, int Q,
+                               const int2* pos, long long zw_dist, long long y_dist, hipStream_t s);
 
 // --- kernels_layout.hip
 // Strided transforms.  Element j of transform (plane p, column c) sits at

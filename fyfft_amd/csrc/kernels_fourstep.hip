@@ -33,7 +33,7 @@ __global__ __launch_bounds__(1024) void k_fs_col(FsArgs<V> a) {
   const int c0 = (int)(lb - t * groups) << a.log2c;
   body_fs_col<V, F, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
                                a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
-                               reinterpret_cast<V*>(smem_raw));
+                               reinterpret_cast<V*>(smem_raw), a.zw ? a.zw + t * a.zw_dist : nullptr);
 }
 
 template <typename V, typename F, int DIR, int MODE>
@@ -77,6 +77,7 @@ static hipError_t dispatch_fs(bool col, int mode, int dir, const FsArgs<V>& a, s
       case kFsRaderA: return go_col<V, F, -1, kFsRaderA>(a, batch, lds, threads, s);
       case kFsBlueA: return go_col<V, F, -1, kFsBlueA>(a, batch, lds, threads, s);
       case kFsRaderC: return go_col<V, F, +1, kFsRaderC>(a, batch, lds, threads, s);
+      case kFsRaderC2: return go_col<V, F, +1, kFsRaderC2>(a, batch, lds, threads, s);
       case kFsBlueC: return go_col<V, F, +1, kFsBlueC>(a, batch, lds, threads, s);
     }
   } else {

@@ -83,6 +83,19 @@ def test_rader_fourstep_65537(fy, orc, cuda, prec):
     check(fy, orc, x, "rader-fourstep", inplace=True)
 
 
+# four-step Rader through the bucketed permutation: L = N-1 split as
+# 140 x 150 / 168 x 180 / 147 x 168 leaves a partial last column group, and
+# L is not a multiple of the 4096-element permutation chunk
+@pytest.mark.parametrize("n", [21001, 30241, 24697])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_rader_fourstep_partial_groups(fy, orc, cuda, n, prec):
+    with fy.Plan(n, 1, fy.F32 if prec == "f32" else fy.F64) as p:
+        assert "bucketed-permutation" in p.describe(), p.describe()
+    x = gauss(np.random.default_rng(n), 3, n, prec)
+    check(fy, orc, x, "rader-fourstep")
+    check(fy, orc, x, "rader-fourstep", inplace=True)
+
+
 @pytest.mark.parametrize("n", [22, 23, 121, 1031, 10006, 10007])
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_bluestein_lds(fy, orc, cuda, n, prec):
