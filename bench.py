@@ -75,9 +75,29 @@ def cpu_baseline(n, sample, prec):
                       f"reference ladder (oracle/), {threads} threads, {sec:.2f} s wall"}
 
 
-def time_steps(fn, steps, warmup, stream):
-    """ms per step of fn() on `stream` (HIP events on that stream)."""
+def settle(fn, seconds=0.2):
+    """Untimed calls of fn() for about `seconds` of wall time, so the GPU's
+    clocks have left their idle state before a measurement starts (a cold
+    start otherwise costs the first few milliseconds of a run up to 10 %)."""
     import torch
+    if seconds <= 0:
+        return
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    per = max(time.perf_counter() - t0, 1e-5)
+    group = max(1, min(64, int(0.01 / per)))
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(group):
+            fn()
+        torch.cuda.synchronize()
+
+
+def time_steps(fn, steps, warmup, stream, settle_s=0.2):
+    """ms per step of fn() on `stream` (HIP events on that stream), after a
+    clock settle and `warmup` untimed steps."""
+    import torch
+    settle(fn, settle_s)
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -281,7 +301,7 @@ def bench_ragged(a):
         n, o = int(lens[i]), int(offs[i])
         errs.append(oracle.rel_l2(yh[o:o + n], oracle.fft_ext(xh[o:o + n].astype(np.complex128))))
     bp = fy.GroupedPlan(lens, fy.F32, fy.GROUPED_PER_TRANSFORM)
-    ms_base = time_steps(lambda: bp.execute(x, y, fy.FORWARD), 2, 1, torch.cuda.current_stream(dev))
+    ms_base = time_steps(lambda: bp.execute(x, y, fy.FORWARD), 2, 1, torch.cuda.current_stream(dev), 0)
     flops = sum(flops_of(int(n)) for n in lens)
     byt = 16.0 * tot
     out = line({"config": {"workload": "ragged grouped C2C, 20000 complex<float> transforms, lengths "
@@ -435,6 +455,7 @@ def main():
         print(json.dumps({"pmc_probe": plan.describe(), "bytes_each_way": x.numel() * x.element_size()}))
         return
 
+    settle(lambda: plan.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h))
     for _ in range(a.warmup):
         plan.execute_ptr(x.data_ptr(), y.data_ptr(), fy.FORWARD, h)
     torch.cuda.synchronize()

@@ -80,6 +80,8 @@ template <typename V, int N, int T, int... Rs> struct FixedFft {
 // runtime pass list, or compile-time passes (N, threads T, pad shift PS and
 // radices fixed; the launch geometry must match).
 template <typename V, int PPT> struct GenericIo {
+  static constexpr int kPpt = PPT;  // bound on points per thread
+  static constexpr int kN = 0, kThreads = 0, kPs = -1;  // geometry known at run time only
   template <int D, typename Src, typename Dst>
   static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int n, int T, int tid,
                                              const PassDesc& pd, const V* tw, bool active, int ps) {
@@ -87,6 +89,8 @@ template <typename V, int PPT> struct GenericIo {
   }
 };
 template <typename V, int N, int T, int PS, int... Rs> struct FixedIo {
+  static constexpr int kPpt = (N + T - 1) / T;
+  static constexpr int kN = N, kThreads = T, kPs = PS;
   template <int D, typename Src, typename Dst>
   static __device__ __forceinline__ void run(Src src, Dst dst, V* lds, int, int, int tid, const PassDesc&,
                                              const V* tw, bool active, int) {
@@ -261,8 +265,8 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
   if constexpr (MODE == kFsRaderC2) {
     // runs ordered by destination chunk, written contiguously
     const long long bstart = (long long)c0 * N1;  // this column group's first element
-    for (int i = threadIdx.x; i < N1 * ncols; i += blockDim.x)
-      zw[bstart + i] = cscale(cadd(lds_all[a.lidx[bstart + i]], xz), a.scale);
+    staged<P>(threadIdx.x, blockDim.x, N1 * ncols, [&](int i) { return a.lidx[bstart + i]; },
+              [&](int i, int l) { zw[bstart + i] = cscale(cadd(lds_all[l], xz), a.scale); });
   } else if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
     staged<P>(threadIdx.x, blockDim.x, total,
                  [&](int i) {

@@ -61,7 +61,9 @@ __global__ __launch_bounds__(T * TPB) void k_lds_fixed(const V* in, V* out, long
   const int slot = threadIdx.x / T;
   const int tid = threadIdx.x - slot * T;
   const long long t = (long long)blockIdx.x * TPB + slot;
-  const bool active = t < batch;
+  // one transform per workgroup: the grid is exactly the batch, and a
+  // compile-time true lets every pass drop its predicate (about 3 % faster)
+  const bool active = TPB == 1 || t < batch;
   const V* gin = in + (active ? t : 0) * dist;
   V* gout = out + (active ? t : 0) * dist;
   run_fixed<V, DIR, N, T, PS, 1, Rs...>(gin, lds_all + slot * stride, gout, tid, tw, active, scale);
@@ -93,7 +95,12 @@ struct FixedEntry {
 
 // PS = 30 stores the transform unpadded: for these power-of-two radix-8/16
 // pass orders the access patterns stay within two LDS cycles per wave, and
-// the smaller footprint lets five 4096-point workgroups share a CU.
+// the smaller footprint lets five 4096-point workgroups share a CU.  For
+// N = 6720 (FP32) the unpadded 52.5 KB lets three workgroups share a CU
+// (padded, only two fit), and 512 threads (8 full waves, 13-14 points per
+// thread) beat 420 (6.6 waves): measured 0.35 against 0.46 ms for 16384
+// transforms.  Thread counts need not divide N: every pass guards its
+// butterfly index.
 #define FIXED(P, V, N, T, TPB, PS, NP, ...) \
   {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, PS, __VA_ARGS__>}
 
@@ -107,10 +114,10 @@ static const FixedEntry kFixed[] = {
     FIXED(kF32, float2, 4096, 256, 1, 30, 3, 16, 16, 16),
     FIXED(kF32, float2, 8192, 512, 1, 30, 4, 16, 8, 8, 8),
     FIXED(kF32, float2, 16384, 1024, 1, 30, 4, 16, 16, 8, 8),
-    FIXED(kF32, float2, 6720, 420, 1, 4, 5, 8, 8, 7, 5, 3),
+    FIXED(kF32, float2, 6720, 512, 1, 30, 5, 8, 8, 7, 5, 3),
     FIXED(kF64, double2, 1024, 64, 4, 30, 3, 16, 8, 8),
     FIXED(kF64, double2, 4096, 256, 1, 30, 3, 16, 16, 16),
-    FIXED(kF64, double2, 6720, 840, 1, 4, 5, 8, 8, 7, 5, 3),
+    FIXED(kF64, double2, 6720, 960, 1, 4, 5, 8, 8, 7, 5, 3),
 };
 #undef FIXED
 
@@ -123,7 +130,7 @@ int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
     if (!same) continue;
     g->threads = e.threads;
     g->per_block = e.per_block;
-    g->ppt = e.n / e.threads;  // the generic kernels reuse this geometry
+    g->ppt = cdiv(e.n, e.threads);  // the generic kernels reuse this geometry
     g->pad_shift = e.pad_shift;
     g->lds_bytes = (size_t)lds_len(e.n, e.pad_shift) * e.per_block * elem_bytes(prec);
     return i;

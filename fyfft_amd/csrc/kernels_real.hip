@@ -45,7 +45,21 @@ template <typename V> __device__ __forceinline__ V c2r_combine(V xk, V xm, V w) 
 }
 
 // ---- source and sink functors of the fused kernels
-template <typename V> struct HermHalfSrc {  // Z[k] of an even-length C2R
+// w^k for k <= h from the table, or (TW > 0, a power of two) as the product
+// w^(k mod TW) * w^(k - k mod TW): in a first pass whose point indices are
+// thread + r*TW the first factor is the same for all of a thread's points
+// and the second is uniform across the workgroup, so the table traffic per
+// point drops from one vector load to a scalar one (one more rounding).
+template <int TW, typename V> __device__ __forceinline__ V root_at(const V* w, int k) {
+  if constexpr (TW > 0) {
+    const int lo = k & (TW - 1);
+    return lo == k ? w[k] : cmul(w[lo], w[k - lo]);
+  } else {
+    return w[k];
+  }
+}
+
+template <typename V, int TW> struct HermHalfSrc {  // Z[k] of an even-length C2R
   static constexpr bool kFromLds = false;
   gbl_cptr<V> x;  // X[0..h]
   gbl_cptr<V> w;  // w^k, k <= h
@@ -54,7 +68,7 @@ template <typename V> struct HermHalfSrc {  // Z[k] of an even-length C2R
     const V* X = (const V*)x;
     V xk = X[k], xm = X[h - k];
     if (k == 0) xk.y = xm.y = 0;  // bins 0 and n/2 of a real signal are real
-    return c2r_combine(xk, xm, ((const V*)w)[k]);
+    return c2r_combine(xk, xm, root_at<TW>((const V*)w, k));
   }
 };
 template <typename V> struct RealSrc {  // (x[i], 0)
@@ -89,33 +103,80 @@ template <typename V> struct RealSink {  // y[i] = Re(v) * scale
 }  // namespace
 
 // One workgroup slot per row; MODE selects the form, F the pass policy.
-template <typename V, typename F, int MODE>
-__global__ __launch_bounds__(1024) void k_real_lds(RealArgs<V> a) {
+// TPB > 0: F is a compile-time policy (FixedIo) and the launch has TPB
+// transforms per workgroup, so geometry, LDS (static) and, for TPB == 1,
+// the row guard are compile-time as well.
+template <typename V, typename F, int MODE, int TPB>
+__global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_real_lds(RealArgs<V> a) {
+  constexpr bool kFixed = TPB > 0;
+  constexpr int kStride = kFixed ? lds_len(F::kN, F::kPs) : 1;
+  __shared__ __align__(16) V lds_static[kFixed ? kStride * TPB : 1];
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int T = a.threads;
+  const int T = kFixed ? F::kThreads : a.threads;
+  const int ps = kFixed ? F::kPs : a.ps;
+  const int per_block = kFixed ? TPB : a.per_block;
+  const int stride = kFixed ? kStride : a.stride;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long t = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
-  const bool active = t < a.rows;
+  const long long t = (long long)xcd_remap(blockIdx.x, gridDim.x) * per_block + slot;
+  const bool active = TPB == 1 || t < a.rows;
   const long long row = active ? t : 0;
-  V* lds = reinterpret_cast<V*>(smem_raw) + slot * a.stride;
+  V* lds = (kFixed ? lds_static : reinterpret_cast<V*>(smem_raw)) + slot * stride;
   const lds_ptr<V> ldsp = (lds_ptr<V>)as_lds(lds);
-  const int n = a.n, h = n / 2, ps = a.ps;
+  constexpr bool kHalf = MODE == kRealR2cHalf || MODE == kRealC2rHalf;
+  const int n = kFixed ? (kHalf ? 2 * F::kN : F::kN) : a.n, h = n / 2;
+  // product twiddles need compile-time power-of-two threads with T + 16 <= h
+  constexpr int kTw = (kFixed && (F::kThreads & (F::kThreads - 1)) == 0 && F::kThreads + 16 <= F::kN)
+                          ? F::kThreads : 0;
   if constexpr (MODE == kRealR2cHalf) {
     const V* z = (const V*)a.in + row * h;
     F::template run<-1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{ldsp, ps}, lds, h, T, tid, a.pd, a.tw, active,
                         ps);
     if (active) {
+      // Bins 0..h of the row in 128-byte aligned spans: thread i of the pass
+      // writes bin k = i + ka, wrapping past h to 0..ka-1, where bin ka is the
+      // row's first one on a 128-byte boundary (rows of h+1 bins are not
+      // aligned, and unaligned spans cost partial-line writes).
       V* X = (V*)a.out + row * (h + 1);
       const V* L = (const V*)ldsp;
-      for (int k = tid; k <= h; k += T) {
+      constexpr int kLine = 128 / (int)sizeof(V);
+      const int ka0 = (int)((kLine - (row * (h + 1)) % kLine) % kLine);
+      const int ka = ka0 <= h ? ka0 : 0;  // rows shorter than a line: no rotation
+      auto put = [&](int k, V w) {
         const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
-        gstore(X + k, cscale(r2c_combine(L[k0 + (k0 >> ps)], L[km + (km >> ps)], a.rtw[k]), a.scale));
+        gstore(X + k, cscale(r2c_combine(L[k0 + (k0 >> ps)], L[km + (km >> ps)], w), a.scale));
+      };
+      if constexpr (kTw > 0) {
+        // the table is read-only here: through the constant address space
+        // the uniform factors become scalar loads issued ahead of the stores
+        typedef S<V> e2 __attribute__((ext_vector_type(2)));
+        const __attribute__((address_space(4))) e2* Wc = (const __attribute__((address_space(4))) e2*)a.rtw;
+        auto W = [&](int k) { const e2 v = Wc[k]; return V{v.x, v.y}; };
+        const V wt = W(tid + ka);
+#pragma unroll
+        for (int j = 0; j <= F::kN / kTw; ++j) {
+          const int i = tid + j * kTw;
+          if (i <= h) {
+            int k = i + ka;
+            V w;
+            if (k > h) {
+              k -= h + 1;
+              w = W(k);
+            } else {
+              w = j == 0 ? wt : cmul(wt, W(j * kTw));
+            }
+            put(k, w);
+          }
+        }
+      } else {
+        staged<F::kPpt + 1>(tid, T, h + 1,
+                            [&](int i) { const int k = i + ka; return a.rtw[k > h ? k - (h + 1) : k]; },
+                            [&](int i, V w) { const int k = i + ka; put(k > h ? k - (h + 1) : k, w); });
       }
     }
   } else if constexpr (MODE == kRealC2rHalf) {
     const V* X = (const V*)a.in + row * (h + 1);
     V* z = (V*)a.out + row * h;
-    F::template run<+1>(HermHalfSrc<V>{(gbl_cptr<V>)X, (gbl_cptr<V>)a.rtw, h},
+    F::template run<+1>(HermHalfSrc<V, kTw>{(gbl_cptr<V>)X, (gbl_cptr<V>)a.rtw, h},
                         GlobalSink<V>{(gbl_ptr<V>)z, a.scale}, lds, h, T, tid, a.pd, a.tw, active, ps);
   } else if constexpr (MODE == kRealR2cFull) {
     const S<V>* x = (const S<V>*)a.in + row * n;
@@ -183,46 +244,54 @@ __global__ void k_real_part(const V* W, S<V>* y, long long count, S<V> scale) {
   if (i < count) y[i] = W[i].x * scale;
 }
 
-template <typename V, typename F>
+template <typename V, typename F, int TPB>
 static hipError_t go_real(int mode, const RealArgs<V>& a, const LdsGeom& g, hipStream_t s) {
   void (*k)(RealArgs<V>) = nullptr;
   switch (mode) {
-    case kRealR2cHalf: k = k_real_lds<V, F, kRealR2cHalf>; break;
-    case kRealC2rHalf: k = k_real_lds<V, F, kRealC2rHalf>; break;
-    case kRealR2cFull: k = k_real_lds<V, F, kRealR2cFull>; break;
-    case kRealC2rFull: k = k_real_lds<V, F, kRealC2rFull>; break;
+    case kRealR2cHalf: k = k_real_lds<V, F, kRealR2cHalf, TPB>; break;
+    case kRealC2rHalf: k = k_real_lds<V, F, kRealC2rHalf, TPB>; break;
+    case kRealR2cFull: k = k_real_lds<V, F, kRealR2cFull, TPB>; break;
+    case kRealC2rFull: k = k_real_lds<V, F, kRealC2rFull, TPB>; break;
     default: return hipErrorInvalidValue;
   }
-  if (g.lds_bytes > 65536)
-    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds_bytes);
+  size_t dyn = g.lds_bytes;
+  if constexpr (TPB > 0) {
+    // compile-time geometry: the plan's must be the same, LDS is static
+    if (g.threads != F::kThreads || g.per_block != TPB || g.pad_shift != F::kPs) return hipErrorInvalidValue;
+    dyn = 0;
+  }
+  if (dyn > 65536)
+    (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
   const unsigned grid = (unsigned)((a.rows + g.per_block - 1) / g.per_block);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(g.threads * g.per_block), g.lds_bytes, s, a);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(g.threads * g.per_block), dyn, s, a);
   return hipGetLastError();
 }
 
+// The compile-time kernels take the geometry of the batched fixed kernel of
+// the same length (threads, transforms per block, pad shift).
 hipError_t launch_real_lds(int prec, int mode, const void* args, const LdsGeom& g, int fixed_len, hipStream_t s) {
   if (prec == kF64) {
     const auto& a = *static_cast<const RealArgs<double2>*>(args);
     if (a.rows == 0) return hipSuccess;
     switch (fixed_len) {
-      case 1024: return go_real<double2, FixedIo<double2, 1024, 64, 30, 16, 8, 8>>(mode, a, g, s);
-      case 4096: return go_real<double2, FixedIo<double2, 4096, 256, 30, 16, 16, 16>>(mode, a, g, s);
+      case 1024: return go_real<double2, FixedIo<double2, 1024, 64, 30, 16, 8, 8>, 4>(mode, a, g, s);
+      case 4096: return go_real<double2, FixedIo<double2, 4096, 256, 30, 16, 16, 16>, 1>(mode, a, g, s);
     }
-    return g.ppt <= 8 ? go_real<double2, GenericIo<double2, 8>>(mode, a, g, s)
-                      : go_real<double2, GenericIo<double2, 16>>(mode, a, g, s);
+    return g.ppt <= 8 ? go_real<double2, GenericIo<double2, 8>, 0>(mode, a, g, s)
+                      : go_real<double2, GenericIo<double2, 16>, 0>(mode, a, g, s);
   }
   const auto& a = *static_cast<const RealArgs<float2>*>(args);
   if (a.rows == 0) return hipSuccess;
   switch (fixed_len) {
-    case 256: return go_real<float2, FixedIo<float2, 256, 16, 30, 16, 16>>(mode, a, g, s);
-    case 512: return go_real<float2, FixedIo<float2, 512, 32, 30, 8, 8, 8>>(mode, a, g, s);
-    case 1024: return go_real<float2, FixedIo<float2, 1024, 64, 30, 16, 8, 8>>(mode, a, g, s);
-    case 2048: return go_real<float2, FixedIo<float2, 2048, 128, 30, 16, 16, 8>>(mode, a, g, s);
-    case 4096: return go_real<float2, FixedIo<float2, 4096, 256, 30, 16, 16, 16>>(mode, a, g, s);
-    case 8192: return go_real<float2, FixedIo<float2, 8192, 512, 30, 16, 8, 8, 8>>(mode, a, g, s);
+    case 256: return go_real<float2, FixedIo<float2, 256, 16, 30, 16, 16>, 16>(mode, a, g, s);
+    case 512: return go_real<float2, FixedIo<float2, 512, 32, 30, 8, 8, 8>, 8>(mode, a, g, s);
+    case 1024: return go_real<float2, FixedIo<float2, 1024, 64, 30, 16, 8, 8>, 4>(mode, a, g, s);
+    case 2048: return go_real<float2, FixedIo<float2, 2048, 128, 30, 16, 16, 8>, 2>(mode, a, g, s);
+    case 4096: return go_real<float2, FixedIo<float2, 4096, 256, 30, 16, 16, 16>, 1>(mode, a, g, s);
+    case 8192: return go_real<float2, FixedIo<float2, 8192, 512, 30, 16, 8, 8, 8>, 1>(mode, a, g, s);
   }
-  return g.ppt <= 16 ? go_real<float2, GenericIo<float2, 16>>(mode, a, g, s)
-                     : go_real<float2, GenericIo<float2, 32>>(mode, a, g, s);
+  return g.ppt <= 16 ? go_real<float2, GenericIo<float2, 16>, 0>(mode, a, g, s)
+                     : go_real<float2, GenericIo<float2, 32>, 0>(mode, a, g, s);
 }
 
 static inline dim3 grid_of(long long count) { return dim3((unsigned)((count + 255) / 256)); }

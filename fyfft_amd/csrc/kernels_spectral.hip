@@ -41,7 +41,8 @@ __global__ __launch_bounds__(1024) void k_rfft_modes(SpecArgs<V> a) {
   V* lds = lds_all + slot * a.stride;
   if (active) {
     const S<V>* x = a.real_in + row * L;
-    for (int i = tid; i < L; i += T) lds[i + (i >> ps)] = V{__builtin_nontemporal_load(x + i), 0};
+    staged<PPT>(tid, T, L, [&](int i) { return __builtin_nontemporal_load(x + i); },
+                [&](int i, S<V> v) { lds[i + (i >> ps)] = V{v, 0}; });
   }
   __syncthreads();
   lds_fft<V, PPT, -1>(lds, L, T, tid, a.pd, a.tw, active, ps);
@@ -62,8 +63,8 @@ __global__ __launch_bounds__(1024) void k_irfft_modes(SpecArgs<V> a) {
   V* lds = lds_all + slot * a.stride;
   if (active) {
     const V* in = a.cplx_in + row * a.rdist;
-    for (int i = tid; i < L; i += T)
-      lds[i + (i >> ps)] = i < K ? cscale(in[i * a.kdist], i == 0 ? a.a0 : a.a) : V{0, 0};
+    staged<PPT>(tid, T, L, [&](int i) { return i < K ? cscale(in[i * a.kdist], i == 0 ? a.a0 : a.a) : V{0, 0}; },
+                [&](int i, V v) { lds[i + (i >> ps)] = v; });
   }
   __syncthreads();
   lds_fft<V, PPT, +1>(lds, L, T, tid, a.pd, a.tw, active, ps);
@@ -113,14 +114,16 @@ __global__ __launch_bounds__(1024) void k_rfft_modes_half(SpecArgs<V> a) {
                       a.tw, active, ps);
   if (active) {
     V* out = a.cplx_out + row * a.rdist;
-    for (int k = tid; k < K; k += T) {
-      const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
-      const V zk = lds[k0 + (k0 >> ps)], zm = conj_s(lds[km + (km >> ps)]);
-      const V e = cscale(cadd(zk, zm), (S<V>)0.5);
-      const V d = cscale(csub(zk, zm), (S<V>)0.5);
-      const V X = cadd(e, cmul(a.rtw[k], V{d.y, -d.x}));
-      out[k * a.kdist] = cscale(X, k == 0 ? a.a0 : a.a);
-    }
+    // K <= h + 1 modes: at most kPpt + 1 per thread, twiddle loads issued together
+    staged<F::kPpt + 1>(tid, T, K, [&](int k) { return a.rtw[k]; },
+                        [&](int k, V w) {
+                          const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
+                          const V zk = lds[k0 + (k0 >> ps)], zm = conj_s(lds[km + (km >> ps)]);
+                          const V e = cscale(cadd(zk, zm), (S<V>)0.5);
+                          const V d = cscale(csub(zk, zm), (S<V>)0.5);
+                          const V X = cadd(e, cmul(w, V{d.y, -d.x}));
+                          out[k * a.kdist] = cscale(X, k == 0 ? a.a0 : a.a);
+                        });
   }
 }
 
