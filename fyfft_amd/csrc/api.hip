@@ -212,17 +212,27 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g) {
   return true;
 }
 
+// Split of a batched four-step plan.  The plain row pass writes its output
+// transposed, in segments of as many rows as a workgroup holds, so it needs
+// 16 rows per workgroup and the balanced split.  The convolution row pass
+// of Rader/Bluestein plans reads and writes its rows in place, so those take
+// the LDS-aware split (small column tiles, long rows): N = 1000003 at
+// M = 2^21 = 256 x 8192 ran 0.53 ms against 0.58 ms for 1024 x 2048, while
+// plain N = 2^21 at 256 x 8192 ran 3x slower than at 1024 x 2048.
+static bool fs_split(size_t n, size_t elem, bool conv, size_t* n1, size_t* n2) {
+  return (conv && four_step_split_lds(n, elem, n1, n2)) || four_step_split(n, kFactorCap, n1, n2);
+}
+
 static bool fs_possible(size_t n, size_t elem, size_t cap) {
   size_t n1, n2;
   LdsGeom a, b;
-  return four_step_split(n, kFactorCap, &n1, &n2) && fs_geometry(n1, elem, cap, &a) &&
-         fs_geometry(n2, elem, cap, &b);
+  return fs_split(n, elem, false, &n1, &n2) && fs_geometry(n1, elem, cap, &a) && fs_geometry(n2, elem, cap, &b);
 }
 
 // Four-step structure for smooth n (tables only; workspace separately).
-static fyfft_status plan_fourstep(fyfft_plan* p, size_t n) {
+static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
   const size_t cap = lds_cap_bytes(p->device), elem = elem_bytes(p->prec);
-  if (!four_step_split(n, kFactorCap, &p->n1, &p->n2)) return FYFFT_UNSUPPORTED;
+  if (!fs_split(n, elem, conv, &p->n1, &p->n2)) return FYFFT_UNSUPPORTED;
   if (!fs_geometry(p->n1, elem, cap, &p->gcol) || !fs_geometry(p->n2, elem, cap, &p->grow))
     return FYFFT_UNSUPPORTED;
   fyfft_status st = make_pass_table(p, p->n1, &p->pdcol, &p->twcol);
@@ -334,7 +344,8 @@ static std::vector<LdsChoice> lds_candidates(fyfft_plan* p, size_t n) {
   LdsGeom g;
   if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) return out;
   const bool f64 = p->prec == kF64;
-  const int alt_ppt = f64 ? (g.ppt == 8 ? 16 : 8) : (g.ppt == 16 ? 32 : 16);
+  const int lo = f64 ? 8 : 16;  // the alternative: twice or half the points per thread
+  const int alt_ppt = g.ppt == lo ? 2 * lo : (g.ppt == 2 * lo ? lo : 2 * lo);
   auto tpb_for = [&](int ppt, int ps) {
     const int T = (int)((n + ppt - 1) / ppt);
     const size_t per = (size_t)dev::lds_len((int)n, ps) * elem_bytes(p->prec);
@@ -477,7 +488,10 @@ static fyfft_status wisdom_parse(const char* text) {
       c.rad.push_back(r);
       prod *= (size_t)r;
     }
-    const bool ppt_ok = prec == kF64 ? (ppt == 8 || ppt == 16) : (ppt == 16 || ppt == 32);
+    // the planner's points per thread: lo or 2*lo, or n spread over 1024 threads
+    const unsigned long long lo = prec == kF64 ? 8 : 16;
+    const bool ppt_ok = (unsigned long long)ppt == lo || (unsigned long long)ppt == 2 * lo ||
+                        (n > lo * 1024 && (unsigned long long)ppt == (n + 1023) / 1024);
     if (prod != n || (prec != kF32 && prec != kF64) || !ppt_ok || (ps != 30 && (ps < 2 || ps > 8)) ||
         tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses)
       return FYFFT_INVALID_ARGUMENT;
@@ -513,7 +527,7 @@ static fyfft_status plan_conv_structure(fyfft_plan* p, size_t m, bool rader) {
     return plan_lds(p, m, &p->lds, false);
   }
   p->kind = rader ? kRaderFs : kBlueFs;
-  fyfft_status st = plan_fourstep(p, m);
+  fyfft_status st = plan_fourstep(p, m, true);
   if (!st) st = alloc_workspace(p, m, rader);
   return st;
 }
@@ -628,7 +642,7 @@ static double conv_length_cost(size_t c, size_t elem, size_t cap, int prec) {
   if (lds_geometry(c, elem, cap, &g)) return (double)c * (double)radices_for(c).size();
   size_t n1, n2;
   LdsGeom a, b;
-  if (!four_step_split(c, kFactorCap, &n1, &n2) || !fs_geometry(n1, elem, cap, &a) || !fs_geometry(n2, elem, cap, &b))
+  if (!fs_split(c, elem, true, &n1, &n2) || !fs_geometry(n1, elem, cap, &a) || !fs_geometry(n2, elem, cap, &b))
     return 0;
   auto fixed = [&](size_t len, const LdsGeom& q) {
     const std::vector<int> r = radices_for(len);
@@ -950,7 +964,7 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
       }
     } else {
       p->kind = kFourStep;
-      st = plan_fourstep(p, n);
+      st = plan_fourstep(p, n, false);
       if (!st) st = alloc_workspace(p, n, false);
       if (!st)
         p->desc = "fourstep N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +

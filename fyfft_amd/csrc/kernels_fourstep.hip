@@ -95,14 +95,15 @@ static hipError_t dispatch_fs(bool col, int mode, int dir, const FsArgs<V>& a, s
 // as planner.cpp radices_for, threads = n / points-per-thread).
 struct FsFixed {
   int prec, n, threads, npass;
-  int radix[3];
+  int radix[4];
 };
 static const FsFixed kFsFixed[] = {
     {kF32, 64, 4, 2, {8, 8}},       {kF32, 128, 8, 2, {16, 8}},      {kF32, 256, 16, 2, {16, 16}},
     {kF32, 512, 32, 3, {8, 8, 8}},  {kF32, 1024, 64, 3, {16, 8, 8}}, {kF32, 2048, 128, 3, {16, 16, 8}},
-    {kF32, 4096, 256, 3, {16, 16, 16}},
+    {kF32, 4096, 256, 3, {16, 16, 16}}, {kF32, 8192, 512, 4, {16, 8, 8, 8}}, {kF32, 16384, 1024, 4, {16, 16, 8, 8}},
     {kF64, 64, 8, 2, {8, 8}},       {kF64, 128, 16, 2, {16, 8}},     {kF64, 256, 32, 2, {16, 16}},
     {kF64, 512, 64, 3, {8, 8, 8}},  {kF64, 1024, 128, 3, {16, 8, 8}}, {kF64, 2048, 256, 3, {16, 16, 8}},
+    {kF64, 4096, 512, 3, {16, 16, 16}}, {kF64, 8192, 1024, 4, {16, 8, 8, 8}},
 };
 
 int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads) {
@@ -127,9 +128,11 @@ static hipError_t dispatch_len(int fixed_n, bool col, int mode, int dir, const F
     case 512: return dispatch_fs<V, FixedFft<V, 512, 512 / P, 8, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
     case 1024: return dispatch_fs<V, FixedFft<V, 1024, 1024 / P, 16, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
     case 2048: return dispatch_fs<V, FixedFft<V, 2048, 2048 / P, 16, 16, 8>>(col, mode, dir, a, batch, lds, threads, s);
-    case 4096:
+    case 4096: return dispatch_fs<V, FixedFft<V, 4096, 4096 / P, 16, 16, 16>>(col, mode, dir, a, batch, lds, threads, s);
+    case 8192: return dispatch_fs<V, FixedFft<V, 8192, 8192 / P, 16, 8, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
+    case 16384:
       if constexpr (sizeof(V) == 8)
-        return dispatch_fs<V, FixedFft<V, 4096, 4096 / P, 16, 16, 16>>(col, mode, dir, a, batch, lds, threads, s);
+        return dispatch_fs<V, FixedFft<V, 16384, 16384 / P, 16, 16, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
       break;
   }
   return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);

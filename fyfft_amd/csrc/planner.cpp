@@ -97,10 +97,16 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
   r.n = (int)n;
   // points per thread: the generic kernels exist for F32 PPT 16/32 and
   // F64 PPT 8/16 (F64 keeps PPT low to stay within the VGPR budget)
-  if (elem_bytes >= 16) r.ppt = n <= 8192 ? 8 : 16;
-  else r.ppt = n <= 16384 ? 16 : 32;
+  const int ppt_lo = elem_bytes >= 16 ? 8 : 16;
+  r.ppt = (int)n <= ppt_lo * 1024 ? ppt_lo : 2 * ppt_lo;
   r.threads = (int)((n + r.ppt - 1) / r.ppt);
-  if (r.threads > 1024) return false;
+  if (r.ppt > ppt_lo) {
+    // beyond ppt_lo * 1024 points: all 1024 threads (more waves to hide
+    // latency with one workgroup per CU), ppt_lo < points per thread <= 2 * ppt_lo
+    r.threads = 1024;
+    r.ppt = (int)((n + 1023) / 1024);
+  }
+  if (r.threads > 1024 || r.ppt > 2 * ppt_lo) return false;
   r.pad_shift = dev::kPadShift;
   size_t per = (size_t)((n - 1) + ((n - 1) >> r.pad_shift) + 1) * elem_bytes;
   if (per > lds_cap) {
@@ -114,6 +120,20 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
   r.per_block = tpb;
   r.lds_bytes = per * (size_t)tpb;
   *g = r;
+  return true;
+}
+
+bool four_step_split_lds(size_t n, size_t elem_bytes, size_t* n1, size_t* n2) {
+  if (!is_smooth7(n)) return false;
+  // column tiles of 16 sub-transforms within 40 KB of LDS (four or more
+  // workgroups per CU) and rows that one workgroup holds
+  const size_t max1 = 40960 / (16 * elem_bytes), max2 = elem_bytes <= 8 ? 16384 : 8192;
+  size_t best = 0;
+  for (size_t d = 2; d * d <= n && d <= max1; ++d)
+    if (n % d == 0 && n / d <= max2) best = d;
+  if (!best) return false;
+  *n1 = best;
+  *n2 = n / best;
   return true;
 }
 

@@ -41,5 +41,8 @@ size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded
 // Four-step split n = n1 * n2 with both factors smooth and LDS-resident
 // (n1 <= n2, as balanced as possible).  False if none exists.
 bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2);
+// The split batched four-step plans prefer: the largest n1 <= sqrt(n) whose
+// 16-column tiles fit 40 KB of LDS, with n2 <= 16384 (FP32) / 8192 (FP64).
+bool four_step_split_lds(size_t n, size_t elem_bytes, size_t* n1, size_t* n2);
 
 }  // namespace fyfft
