@@ -524,7 +524,9 @@ static fyfft_status plan_conv_structure(fyfft_plan* p, size_t m, bool rader) {
   LdsGeom g;
   if (lds_geometry(m, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) {
     p->kind = rader ? kRaderLds : kBlueLds;
-    return plan_lds(p, m, &p->lds, false);
+    fyfft_status st = plan_lds(p, m, &p->lds, false);
+    if (!st) p->lds.fixed = find_conv_fixed(p->prec, p->lds.pd, &p->lds.geom);
+    return st;
   }
   p->kind = rader ? kRaderFs : kBlueFs;
   fyfft_status st = plan_fourstep(p, m, true);
@@ -624,22 +626,31 @@ static fyfft_status plan_rader(fyfft_plan* p) {
   if (!st)
     p->desc = std::string(p->kind == kRaderLds ? "rader-lds" : "rader-fourstep") + " N=" +
               std::to_string(n) + " L=" + std::to_string(L) + " g=" + std::to_string(g) +
-              (p->rb_q ? " bucketed-permutation Q=" + std::to_string(p->rb_q) : "");
+              (p->rb_q ? " bucketed-permutation Q=" + std::to_string(p->rb_q) : "") +
+              (p->kind == kRaderLds && p->lds.fixed >= 0 ? " fixed" : "");
   return st;
 }
 
 // Estimated cost of a Bluestein convolution of length c (0: not plannable):
 // c times its pass count (forward and inverse FFT of c, whether in LDS or as
-// four-step column/row passes), times 1.6 for the four-step's extra HBM
-// round trips, times 0.7 when both four-step factors have compile-time
-// sub-transforms.  Measured on MI355X against the smallest smooth length:
+// four-step column/row passes), times 0.6 for a compile-time LDS kernel
+// (measured 1.4-1.8x faster: N = 10007 at M = 20480 0.33 -> 0.18 ms for
+// 1024, N = 8191 at M = 16384 0.24 -> 0.16 ms for 2048), times 1.6 for the
+// four-step's extra HBM round trips, times 0.7 when both four-step factors
+// have compile-time sub-transforms.  Measured on MI355X against the smallest smooth length:
 // N = 10007 at M = 20480 (4 passes) instead of 20160 (6): 0.367 -> 0.335 ms
 // for 1024 transforms; N = 1000003 at M = 2^21 instead of 2000376 =
 // 1372 x 1458: 1.29 -> 0.63 ms for 16.
 static double conv_length_cost(size_t c, size_t elem, size_t cap, int prec) {
   if (!is_smooth7(c)) return 0;
   LdsGeom g;
-  if (lds_geometry(c, elem, cap, &g)) return (double)c * (double)radices_for(c).size();
+  if (lds_geometry(c, elem, cap, &g)) {
+    const std::vector<int> r = radices_for(c);
+    dev::PassDesc pd{};
+    pd.npass = (int)std::min<size_t>(r.size(), (size_t)dev::kMaxPasses);
+    for (int i = 0; i < pd.npass; ++i) pd.radix[i] = r[i];
+    return (double)c * (double)r.size() * (find_conv_fixed(prec, pd, &g) >= 0 ? 0.6 : 1.0);
+  }
   size_t n1, n2;
   LdsGeom a, b;
   if (!fs_split(c, elem, true, &n1, &n2) || !fs_geometry(n1, elem, cap, &a) || !fs_geometry(n2, elem, cap, &b))
@@ -692,7 +703,8 @@ static fyfft_status plan_bluestein(fyfft_plan* p) {
   (void)hipFree(bb);
   if (!st)
     p->desc = std::string(p->kind == kBlueLds ? "bluestein-lds" : "bluestein-fourstep") + " N=" +
-              std::to_string(n) + " M=" + std::to_string(m);
+              std::to_string(n) + " M=" + std::to_string(m) +
+              (p->kind == kBlueLds && p->lds.fixed >= 0 ? " fixed" : "");
   return st;
 }
 
@@ -788,7 +800,7 @@ static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool
   a.chirp = (const V*)p->chirp[slot];
   a.kern = (const V*)p->kern[slot];
   a.scale = (dev::S<V>)scale;
-  HIP_TRY(launch_conv_lds(p->prec, p->kind == kRaderLds, &a, p->lds.geom, s));
+  HIP_TRY(launch_conv_lds(p->prec, p->kind == kRaderLds, &a, p->lds.geom, p->lds.fixed, s));
   return FYFFT_SUCCESS;
 }
 

@@ -188,32 +188,33 @@ __device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x,
 
 // ------------------------------------------------------------ Rader
 // X[0] = x0 + sum_{n>=1} x[n];  X[g^i] = x0 + (a (*) b)[i],  a[j] = x[g^-j]
-template <typename V, int PPT>
+// F: the I/O policy of the LDS transforms (GenericIo or FixedIo).
+template <typename V, typename F>
 __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                            bool active) {
   V* lds = as_lds(lds_in);
   const V x0 = active ? x[0] : V{0, 0};
   __syncthreads();  // every read of x[0] precedes the write of y[0] (in place)
-  lds_fft_io<V, PPT, -1>(RaderSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<int>)a.perm_in},
-                         KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
-                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-  lds_fft_io<V, PPT, +1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                         RaderSink<V>{(gbl_ptr<V>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
-                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  F::template run<-1>(RaderSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<int>)a.perm_in},
+                      KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
+                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
+                      RaderSink<V>{(gbl_ptr<V>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
+                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
 // ------------------------------------------------------------ Bluestein
 // X[k] = chirp[k] * IFFT(FFT(x * chirp, zero-padded to M) * kernel)[k]
-template <typename V, int PPT>
+template <typename V, typename F>
 __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                           bool active) {
   V* lds = as_lds(lds_in);
-  lds_fft_io<V, PPT, -1>(BlueSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<V>)a.chirp, a.n},
-                         KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, nullptr, V{0, 0}, 1},
-                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-  lds_fft_io<V, PPT, +1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                         BlueSink<V>{(gbl_ptr<V>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
-                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  F::template run<-1>(BlueSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<V>)a.chirp, a.n},
+                      KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, nullptr, V{0, 0}, 1},
+                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
+                      BlueSink<V>{(gbl_ptr<V>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
+                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
 // ------------------------------------------------------------ four-step

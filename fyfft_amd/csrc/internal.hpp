@@ -115,7 +115,11 @@ template <typename V> struct ConvArgs {
   dev::S<V> scale = 1;
 };
 // Fused single-kernel Rader (rader=true) or Bluestein, whole convolution in LDS.
-hipError_t launch_conv_lds(int prec, bool rader, const void* args, const LdsGeom& g, hipStream_t s);
+// fixed: index from find_conv_fixed, or -1 for the runtime passes
+hipError_t launch_conv_lds(int prec, bool rader, const void* args, const LdsGeom& g, int fixed, hipStream_t s);
+// Index of a compile-time convolution kernel of length g->n with the pass
+// list pd (g takes its geometry), or -1.
+int find_conv_fixed(int prec, const dev::PassDesc& pd, LdsGeom* g);
 
 // Table generation (FP64 results in double2 buffers unless noted).
 // c[n] = exp(sign * i*pi*(n^2 mod 2N)/N), n < count

@@ -36,47 +36,113 @@ template <typename V> __device__ __forceinline__ LdsBody<V> conv_body(const Conv
   return b;
 }
 
-template <typename V, int PPT>
-__global__ __launch_bounds__(1024) void k_rader_lds(ConvArgs<V> a) {
+// One transform per workgroup slot.  TPB > 0: F is a compile-time policy
+// (FixedIo) whose geometry the launch has (TPB transforms per workgroup,
+// static LDS, and for TPB == 1 no row guard); TPB == 0: runtime passes.
+template <typename V, typename F, int TPB, bool RADER>
+__global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_conv_lds(ConvArgs<V> a) {
+  constexpr bool kFixed = TPB > 0;
+  constexpr int kStride = kFixed ? lds_len(F::kN, F::kPs) : 1;
+  __shared__ __align__(16) V lds_static[kFixed ? kStride * TPB : 1];
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int slot = threadIdx.x / a.threads, tid = threadIdx.x - slot * a.threads;
-  const long long t = (long long)blockIdx.x * a.per_block + slot;
-  const bool active = t < a.batch;
+  LdsBody<V> b = conv_body(a);
+  if constexpr (kFixed) {
+    b.m = F::kN;
+    b.threads = F::kThreads;
+    b.ps = F::kPs;
+    b.stride = kStride;
+  }
+  const int slot = threadIdx.x / b.threads, tid = threadIdx.x - slot * b.threads;
+  const long long t = (long long)blockIdx.x * (kFixed ? TPB : a.per_block) + slot;
+  const bool active = TPB == 1 || t < a.batch;
   const long long off = (active ? t : 0) * a.dist;
-  body_rader<V, PPT>(conv_body(a), a.in + off, a.out + off,
-                     reinterpret_cast<V*>(smem_raw) + slot * a.stride, tid, active);
+  V* lds = (kFixed ? lds_static : reinterpret_cast<V*>(smem_raw)) + slot * b.stride;
+  if constexpr (RADER) body_rader<V, F>(b, a.in + off, a.out + off, lds, tid, active);
+  else body_blue<V, F>(b, a.in + off, a.out + off, lds, tid, active);
 }
 
-template <typename V, int PPT>
-__global__ __launch_bounds__(1024) void k_blue_lds(ConvArgs<V> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int slot = threadIdx.x / a.threads, tid = threadIdx.x - slot * a.threads;
-  const long long t = (long long)blockIdx.x * a.per_block + slot;
-  const bool active = t < a.batch;
-  const long long off = (active ? t : 0) * a.dist;
-  body_blue<V, PPT>(conv_body(a), a.in + off, a.out + off,
-                    reinterpret_cast<V*>(smem_raw) + slot * a.stride, tid, active);
-}
-
-template <typename V, int PPT>
+template <typename V, typename F, int TPB>
 static hipError_t go_conv(bool rader, const ConvArgs<V>& a, const LdsGeom& g, hipStream_t s) {
-  auto k = rader ? k_rader_lds<V, PPT> : k_blue_lds<V, PPT>;
-  const size_t lds = g.lds_bytes;
+  auto k = rader ? k_conv_lds<V, F, TPB, true> : k_conv_lds<V, F, TPB, false>;
+  size_t lds = g.lds_bytes;
+  if constexpr (TPB > 0) {
+    if (g.threads != F::kThreads || g.per_block != TPB || g.pad_shift != F::kPs || a.m != F::kN)
+      return hipErrorInvalidValue;
+    lds = 0;
+  }
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const unsigned grid = (unsigned)((a.batch + g.per_block - 1) / g.per_block);
   hipLaunchKernelGGL(k, dim3(grid), dim3(g.threads * g.per_block), lds, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_conv_lds(int prec, bool rader, const void* args, const LdsGeom& g, hipStream_t s) {
+// Compile-time convolution transforms for the power-of-two lengths and the
+// 5 * 2^k lengths (20480 = 2^12 * 5 serves Bluestein N = 10007); radices as
+// radices_for (planner.cpp), geometry per entry (the plan takes it).
+using ConvLauncher = hipError_t (*)(bool rader, const void* args, const LdsGeom& g, hipStream_t s);
+template <typename V, int N, int T, int TPB, int PS, int... Rs>
+static hipError_t launch_conv_fixed(bool rader, const void* args, const LdsGeom& g, hipStream_t s) {
+  const auto& a = *static_cast<const ConvArgs<V>*>(args);
+  return go_conv<V, FixedIo<V, N, T, PS, Rs...>, TPB>(rader, a, g, s);
+}
+struct ConvFixed {
+  int prec, n, threads, per_block, pad_shift, npass;
+  int radix[4];
+  ConvLauncher launch;
+};
+#define CFIXED(P, V, N, T, TPB, PS, NP, ...) \
+  {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_conv_fixed<V, N, T, TPB, PS, __VA_ARGS__>}
+static const ConvFixed kConvFixed[] = {
+    CFIXED(kF32, float2, 256, 16, 16, 4, 2, 16, 16),
+    CFIXED(kF32, float2, 512, 32, 8, 4, 3, 8, 8, 8),
+    CFIXED(kF32, float2, 1024, 64, 4, 4, 3, 16, 8, 8),
+    CFIXED(kF32, float2, 1280, 80, 2, 4, 3, 16, 16, 5),
+    CFIXED(kF32, float2, 2048, 128, 2, 4, 3, 16, 16, 8),
+    CFIXED(kF32, float2, 2560, 160, 1, 4, 4, 8, 8, 8, 5),
+    CFIXED(kF32, float2, 4096, 256, 1, 4, 3, 16, 16, 16),
+    CFIXED(kF32, float2, 5120, 320, 1, 4, 4, 16, 8, 8, 5),
+    CFIXED(kF32, float2, 8192, 512, 1, 4, 4, 16, 8, 8, 8),
+    CFIXED(kF32, float2, 10240, 640, 1, 4, 4, 16, 16, 8, 5),
+    CFIXED(kF32, float2, 16384, 1024, 1, 4, 4, 16, 16, 8, 8),
+    CFIXED(kF32, float2, 20480, 1024, 1, 30, 4, 16, 16, 16, 5),
+};
+#undef CFIXED
+
+int find_conv_fixed(int prec, const PassDesc& pd, LdsGeom* g) {
+  for (int i = 0; i < (int)(sizeof(kConvFixed) / sizeof(kConvFixed[0])); ++i) {
+    const ConvFixed& e = kConvFixed[i];
+    if (e.prec != prec || e.n != g->n || e.npass != pd.npass) continue;
+    bool same = true;
+    for (int p = 0; p < pd.npass; ++p) same = same && e.radix[p] == pd.radix[p];
+    if (!same) continue;
+    g->threads = e.threads;
+    g->per_block = e.per_block;
+    g->pad_shift = e.pad_shift;
+    g->ppt = cdiv(e.n, e.threads);
+    g->lds_bytes = (size_t)lds_len(e.n, e.pad_shift) * e.per_block * elem_bytes(prec);
+    return i;
+  }
+  return -1;
+}
+
+hipError_t launch_conv_lds(int prec, bool rader, const void* args, const LdsGeom& g, int fixed, hipStream_t s) {
+  if (fixed >= 0) {
+    if (fixed >= (int)(sizeof(kConvFixed) / sizeof(kConvFixed[0])) || kConvFixed[fixed].prec != prec)
+      return hipErrorInvalidValue;
+    const bool empty = prec == kF64 ? static_cast<const ConvArgs<double2>*>(args)->batch == 0
+                                    : static_cast<const ConvArgs<float2>*>(args)->batch == 0;
+    return empty ? hipSuccess : kConvFixed[fixed].launch(rader, args, g, s);
+  }
   if (prec == kF64) {
     const auto& a = *static_cast<const ConvArgs<double2>*>(args);
     if (a.batch == 0) return hipSuccess;
-    return g.ppt <= 8 ? go_conv<double2, 8>(rader, a, g, s) : go_conv<double2, 16>(rader, a, g, s);
+    return g.ppt <= 8 ? go_conv<double2, GenericIo<double2, 8>, 0>(rader, a, g, s)
+                      : go_conv<double2, GenericIo<double2, 16>, 0>(rader, a, g, s);
   }
   const auto& a = *static_cast<const ConvArgs<float2>*>(args);
   if (a.batch == 0) return hipSuccess;
-  return g.ppt <= 16 ? go_conv<float2, 16>(rader, a, g, s) : go_conv<float2, 32>(rader, a, g, s);
+  return g.ppt <= 16 ? go_conv<float2, GenericIo<float2, 16>, 0>(rader, a, g, s)
+                     : go_conv<float2, GenericIo<float2, 32>, 0>(rader, a, g, s);
 }
 
 // ------------------------------------------------------------ tables

@@ -44,8 +44,8 @@ __device__ __noinline__ void task_lds(const GLen<V>& g, const GTask& t, const Ta
   const bool active = slot < t.count;
   const long long o = active ? c.off[t.first + slot] : 0;
   V* lds = c.lds + (active ? slot : 0) * b.stride;
-  if (g.kind == kGkRader) body_rader<V, PPT>(b, c.in + o, c.out + o, lds, tid, active);
-  else if (g.kind == kGkBlue) body_blue<V, PPT>(b, c.in + o, c.out + o, lds, tid, active);
+  if (g.kind == kGkRader) body_rader<V, GenericIo<V, PPT>>(b, c.in + o, c.out + o, lds, tid, active);
+  else if (g.kind == kGkBlue) body_blue<V, GenericIo<V, PPT>>(b, c.in + o, c.out + o, lds, tid, active);
   else body_lds_smooth<V, PPT, DIR>(b, c.in + o, c.out + o, lds, tid, active);
 }
 

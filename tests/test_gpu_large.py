@@ -104,6 +104,17 @@ def test_bluestein_lds(fy, orc, cuda, n, prec):
     check(fy, orc, unit_phase(np.random.default_rng(n), 4, n, prec), "bluestein-lds")
 
 
+# one prime per compile-time convolution length (M in brackets): 256, 512,
+# 1024, 1280, 2048, 2560, 4096, 5120, 8192, 10240, 16384, 20480
+@pytest.mark.parametrize("n", [103, 223, 419, 521, 821, 1031, 1657, 2053, 3299, 4099, 6563, 8209])
+def test_bluestein_lds_fixed_lengths(fy, orc, cuda, n):
+    with fy.Plan(n, 5, fy.F32) as p:
+        assert "fixed" in p.describe(), p.describe()
+    x = unit_phase(np.random.default_rng(n), 5, n, "f32")
+    check(fy, orc, x, "bluestein-lds")
+    check(fy, orc, x, "bluestein-lds", inplace=True)
+
+
 @pytest.mark.parametrize("n,batch,prec", [(1000003, 2, "f32"), (100003, 2, "f32"),
                                           (10007, 2, "f64")])
 def test_bluestein_fourstep(fy, orc, cuda, n, batch, prec):
