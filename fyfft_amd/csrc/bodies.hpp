@@ -98,6 +98,20 @@ template <typename V, int N, int T, int PS, int... Rs> struct FixedIo {
   }
 };
 
+// w^k for k <= h from the table, or (TW > 0, a power of two) as the product
+// w^(k mod TW) * w^(k - k mod TW): in a first pass whose point indices are
+// thread + r*TW the first factor is the same for all of a thread's points
+// and the second is uniform across the workgroup, so the table traffic per
+// point drops from one vector load to a scalar one (one more rounding).
+template <int TW, typename V> __device__ __forceinline__ V root_at(const V* w, int k) {
+  if constexpr (TW > 0) {
+    const int lo = k & (TW - 1);
+    return lo == k ? w[k] : cmul(w[lo], w[k - lo]);
+  } else {
+    return w[k];
+  }
+}
+
 // ------------------------------------------------------------ functors
 template <typename V> using gbl_ptr = __attribute__((address_space(1))) V*;
 

@@ -45,20 +45,6 @@ template <typename V> __device__ __forceinline__ V c2r_combine(V xk, V xm, V w) 
 }
 
 // ---- source and sink functors of the fused kernels
-// w^k for k <= h from the table, or (TW > 0, a power of two) as the product
-// w^(k mod TW) * w^(k - k mod TW): in a first pass whose point indices are
-// thread + r*TW the first factor is the same for all of a thread's points
-// and the second is uniform across the workgroup, so the table traffic per
-// point drops from one vector load to a scalar one (one more rounding).
-template <int TW, typename V> __device__ __forceinline__ V root_at(const V* w, int k) {
-  if constexpr (TW > 0) {
-    const int lo = k & (TW - 1);
-    return lo == k ? w[k] : cmul(w[lo], w[k - lo]);
-  } else {
-    return w[k];
-  }
-}
-
 template <typename V, int TW> struct HermHalfSrc {  // Z[k] of an even-length C2R
   static constexpr bool kFromLds = false;
   gbl_cptr<V> x;  // X[0..h]

@@ -79,7 +79,7 @@ template <typename V> __device__ __forceinline__ V conj_s(V a) { return V{a.x, -
 
 // Source of the inverse: Z[k] = E[k] + i O[k] from the Hermitian H built
 // on the fly from the kept modes.
-template <typename V> struct ModesHermSrc {
+template <typename V, int TW> struct ModesHermSrc {
   static constexpr bool kFromLds = false;
   gbl_cptr<V> m;  // kept modes of the row, mode j at m[j * kdist]
   gbl_cptr<V> w;  // exp(-2 pi i k / L), k <= h
@@ -96,56 +96,91 @@ template <typename V> struct ModesHermSrc {
   __device__ V operator()(int k) const {
     const V hk = herm(k), hm = conj_s(herm(h - k));
     const V e = cscale(cadd(hk, hm), (S<V>)0.5);
-    const V o = cmulc(cscale(csub(hk, hm), (S<V>)0.5), ((const V*)w)[k]);
+    const V o = cmulc(cscale(csub(hk, hm), (S<V>)0.5), root_at<TW>((const V*)w, k));
     return V{e.x - o.y, e.y + o.x};
   }
 };
 
-template <typename V, typename F>
-__global__ __launch_bounds__(1024) void k_rfft_modes_half(SpecArgs<V> a) {
+// One workgroup slot per row.  TPB > 0: F is a compile-time policy (FixedIo)
+// with TPB rows per workgroup: geometry and LDS (static) are compile-time and
+// for TPB == 1 the row guard is too; w^k is then formed as a product of a
+// per-thread and a uniform factor (root_at) instead of a table load per bin.
+template <typename V, typename F, int TPB>
+__global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_rfft_modes_half(SpecArgs<V> a) {
+  constexpr bool kFixed = TPB > 0;
+  constexpr int kStride = kFixed ? lds_len(F::kN, F::kPs) : 1;
+  __shared__ __align__(16) V lds_static[kFixed ? kStride * TPB : 1];
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int T = a.threads, h = a.n / 2, K = a.k, ps = a.ps;
+  const int T = kFixed ? F::kThreads : a.threads, h = kFixed ? F::kN : a.n / 2, K = a.k;
+  const int ps = kFixed ? F::kPs : a.ps;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
-  const bool active = row < a.rows;
-  V* lds = as_lds(reinterpret_cast<V*>(smem_raw) + slot * a.stride);
+  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * (kFixed ? TPB : a.per_block) + slot;
+  const bool active = TPB == 1 || row < a.rows;
+  V* lds = as_lds((kFixed ? lds_static : reinterpret_cast<V*>(smem_raw)) + slot * (kFixed ? kStride : a.stride));
   const V* z = reinterpret_cast<const V*>(a.real_in) + (active ? row : 0) * h;
   F::template run<-1>(GlobalSrc<V>{(gbl_cptr<V>)z}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, h, T, tid, a.pd,
                       a.tw, active, ps);
   if (active) {
     V* out = a.cplx_out + row * a.rdist;
-    // K <= h + 1 modes: at most kPpt + 1 per thread, twiddle loads issued together
-    staged<F::kPpt + 1>(tid, T, K, [&](int k) { return a.rtw[k]; },
-                        [&](int k, V w) {
-                          const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
-                          const V zk = lds[k0 + (k0 >> ps)], zm = conj_s(lds[km + (km >> ps)]);
-                          const V e = cscale(cadd(zk, zm), (S<V>)0.5);
-                          const V d = cscale(csub(zk, zm), (S<V>)0.5);
-                          const V X = cadd(e, cmul(w, V{d.y, -d.x}));
-                          out[k * a.kdist] = cscale(X, k == 0 ? a.a0 : a.a);
-                        });
+    auto put = [&](int k, V w) {
+      const int k0 = k == h ? 0 : k, km = k == 0 ? 0 : h - k;
+      const V zk = lds[k0 + (k0 >> ps)], zm = conj_s(lds[km + (km >> ps)]);
+      const V e = cscale(cadd(zk, zm), (S<V>)0.5);
+      const V d = cscale(csub(zk, zm), (S<V>)0.5);
+      const V X = cadd(e, cmul(w, V{d.y, -d.x}));
+      out[k * a.kdist] = cscale(X, k == 0 ? a.a0 : a.a);
+    };
+    constexpr int kTw = (kFixed && (F::kThreads & (F::kThreads - 1)) == 0) ? F::kThreads : 0;
+    if constexpr (kTw > 0) {
+      // bins k = tid + j*T (K <= h + 1): w^k = w^tid * w^(jT), the second a
+      // scalar load through the constant address space (read-only table)
+      typedef S<V> e2 __attribute__((ext_vector_type(2)));
+      const __attribute__((address_space(4))) e2* Wc = (const __attribute__((address_space(4))) e2*)a.rtw;
+      auto W = [&](int k) { const e2 v = Wc[k]; return V{v.x, v.y}; };
+      const V wt = W(tid);
+#pragma unroll
+      for (int j = 0; j <= F::kN / kTw; ++j) {
+        const int k = tid + j * kTw;
+        if (k < K) put(k, j == 0 ? wt : cmul(wt, W(j * kTw)));
+      }
+    } else {
+      // K <= h + 1 modes: at most kPpt + 1 per thread, twiddle loads issued together
+      staged<F::kPpt + 1>(tid, T, K, [&](int k) { return a.rtw[k]; }, put);
+    }
   }
 }
 
-template <typename V, typename F>
-__global__ __launch_bounds__(1024) void k_irfft_modes_half(SpecArgs<V> a) {
+template <typename V, typename F, int TPB>
+__global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_irfft_modes_half(SpecArgs<V> a) {
+  constexpr bool kFixed = TPB > 0;
+  constexpr int kStride = kFixed ? lds_len(F::kN, F::kPs) : 1;
+  __shared__ __align__(16) V lds_static[kFixed ? kStride * TPB : 1];
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int T = a.threads, h = a.n / 2, ps = a.ps;
+  const int T = kFixed ? F::kThreads : a.threads, h = kFixed ? F::kN : a.n / 2;
+  const int ps = kFixed ? F::kPs : a.ps;
   const int slot = threadIdx.x / T, tid = threadIdx.x - slot * T;
-  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * a.per_block + slot;
-  const bool active = row < a.rows;
-  V* lds = reinterpret_cast<V*>(smem_raw) + slot * a.stride;
+  const long long row = (long long)xcd_remap(blockIdx.x, gridDim.x) * (kFixed ? TPB : a.per_block) + slot;
+  const bool active = TPB == 1 || row < a.rows;
+  V* lds = (kFixed ? lds_static : reinterpret_cast<V*>(smem_raw)) + slot * (kFixed ? kStride : a.stride);
   const long long r = active ? row : 0;
   V* y = reinterpret_cast<V*>(a.real_out) + r * h;
-  F::template run<+1>(ModesHermSrc<V>{(gbl_cptr<V>)(a.cplx_in + r * a.rdist), (gbl_cptr<V>)a.rtw, a.kdist, a.k, h,
-                                      a.a0, a.a},
+  constexpr int kTw = (kFixed && (F::kThreads & (F::kThreads - 1)) == 0 && F::kThreads + 16 <= F::kN)
+                          ? F::kThreads : 0;
+  F::template run<+1>(ModesHermSrc<V, kTw>{(gbl_cptr<V>)(a.cplx_in + r * a.rdist), (gbl_cptr<V>)a.rtw, a.kdist,
+                                           a.k, h, a.a0, a.a},
                       GlobalSink<V>{(gbl_ptr<V>)y, (S<V>)2}, lds, h, T, tid, a.pd, a.tw, active, ps);
 }
 
-template <typename V, typename F>
+template <typename V, typename F, int TPB>
 static hipError_t go_half(bool inverse, const SpecArgs<V>& a, const LdsGeom& g, hipStream_t s) {
-  auto k = inverse ? k_irfft_modes_half<V, F> : k_rfft_modes_half<V, F>;
-  const size_t lds = (size_t)a.stride * g.per_block * sizeof(V);
+  auto k = inverse ? k_irfft_modes_half<V, F, TPB> : k_rfft_modes_half<V, F, TPB>;
+  size_t lds = (size_t)a.stride * g.per_block * sizeof(V);
+  if constexpr (TPB > 0) {
+    // compile-time geometry: the plan's must be the same, LDS is static
+    if (g.threads != F::kThreads || g.per_block != TPB || g.pad_shift != F::kPs || a.n != 2 * F::kN)
+      return hipErrorInvalidValue;
+    lds = 0;
+  }
   if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const unsigned grid = (unsigned)((a.rows + g.per_block - 1) / g.per_block);
   hipLaunchKernelGGL(k, dim3(grid), dim3(g.threads * g.per_block), lds, s, a);
@@ -160,24 +195,24 @@ hipError_t launch_spectral_half(int prec, bool inverse, const void* args, const 
     const auto& a = *static_cast<const SpecArgs<double2>*>(args);
     if (a.rows == 0) return hipSuccess;
     switch (fixed_h) {
-      case 1024: return go_half<double2, FixedIo<double2, 1024, 64, 30, 16, 8, 8>>(inverse, a, g, s);
-      case 4096: return go_half<double2, FixedIo<double2, 4096, 256, 30, 16, 16, 16>>(inverse, a, g, s);
+      case 1024: return go_half<double2, FixedIo<double2, 1024, 64, 30, 16, 8, 8>, 4>(inverse, a, g, s);
+      case 4096: return go_half<double2, FixedIo<double2, 4096, 256, 30, 16, 16, 16>, 1>(inverse, a, g, s);
     }
-    return g.ppt <= 8 ? go_half<double2, GenericIo<double2, 8>>(inverse, a, g, s)
-                      : go_half<double2, GenericIo<double2, 16>>(inverse, a, g, s);
+    return g.ppt <= 8 ? go_half<double2, GenericIo<double2, 8>, 0>(inverse, a, g, s)
+                      : go_half<double2, GenericIo<double2, 16>, 0>(inverse, a, g, s);
   }
   const auto& a = *static_cast<const SpecArgs<float2>*>(args);
   if (a.rows == 0) return hipSuccess;
   switch (fixed_h) {
-    case 256: return go_half<float2, FixedIo<float2, 256, 16, 30, 16, 16>>(inverse, a, g, s);
-    case 512: return go_half<float2, FixedIo<float2, 512, 32, 30, 8, 8, 8>>(inverse, a, g, s);
-    case 1024: return go_half<float2, FixedIo<float2, 1024, 64, 30, 16, 8, 8>>(inverse, a, g, s);
-    case 2048: return go_half<float2, FixedIo<float2, 2048, 128, 30, 16, 16, 8>>(inverse, a, g, s);
-    case 4096: return go_half<float2, FixedIo<float2, 4096, 256, 30, 16, 16, 16>>(inverse, a, g, s);
-    case 8192: return go_half<float2, FixedIo<float2, 8192, 512, 30, 16, 8, 8, 8>>(inverse, a, g, s);
+    case 256: return go_half<float2, FixedIo<float2, 256, 16, 30, 16, 16>, 16>(inverse, a, g, s);
+    case 512: return go_half<float2, FixedIo<float2, 512, 32, 30, 8, 8, 8>, 8>(inverse, a, g, s);
+    case 1024: return go_half<float2, FixedIo<float2, 1024, 64, 30, 16, 8, 8>, 4>(inverse, a, g, s);
+    case 2048: return go_half<float2, FixedIo<float2, 2048, 128, 30, 16, 16, 8>, 2>(inverse, a, g, s);
+    case 4096: return go_half<float2, FixedIo<float2, 4096, 256, 30, 16, 16, 16>, 1>(inverse, a, g, s);
+    case 8192: return go_half<float2, FixedIo<float2, 8192, 512, 30, 16, 8, 8, 8>, 1>(inverse, a, g, s);
   }
-  return g.ppt <= 16 ? go_half<float2, GenericIo<float2, 16>>(inverse, a, g, s)
-                     : go_half<float2, GenericIo<float2, 32>>(inverse, a, g, s);
+  return g.ppt <= 16 ? go_half<float2, GenericIo<float2, 16>, 0>(inverse, a, g, s)
+                     : go_half<float2, GenericIo<float2, 32>, 0>(inverse, a, g, s);
 }
 
 template <typename V, int PPT>
