@@ -335,7 +335,7 @@ def bench_spectral(a):
 
     def step():
         x.grad = None
-        layer.weight.grad = None
+        layer.weight_kio.grad = None
         y = layer(x)
         y.backward(gy)
     ms = time_steps(step, a.steps, a.warmup, stream)

@@ -65,7 +65,11 @@ class SpectralConv1dFunction(torch.autograd.Function):
 
     # The modes live mode-major ([K, B, C]) between the transforms, so each
     # mode's channel mixing is one GEMM of a batched matmul with no layout
-    # copies of the [B, C, K] mode tensors.
+    # copies of the [B, C, K] mode tensors.  W [Ci, Co, K] is used through its
+    # [K, Ci, Co] permutation, which is a view without a copy when W is stored
+    # mode-major (as SpectralConv1d stores it); the backward GEMMs take the
+    # conjugate transposes as views (BLAS op C) and dW is returned as the same
+    # permuted view of a [K, Ci, Co] result.
 
     @staticmethod
     def forward(ctx, x, w):
@@ -74,7 +78,9 @@ class SpectralConv1dFunction(torch.autograd.Function):
         if 2 * K > L:
             raise ValueError("SpectralConv1d: modes must satisfy 2*K <= L")
         xm = rfft_modes(x, K, mode_major=True)               # [K, B, Ci]
-        wt = w.permute(2, 0, 1).contiguous()                  # [K, Ci, Co]
+        wt = w.permute(2, 0, 1)                               # [K, Ci, Co]
+        if not wt.is_contiguous():
+            wt = wt.contiguous()
         ym = torch.matmul(xm, wt)                             # [K, B, Co]
         y = irfft_modes(ym, L, 1.0 / L, 2.0 / L, mode_major=True)   # [B, Co, L]
         ctx.save_for_backward(xm, wt)
@@ -87,9 +93,8 @@ class SpectralConv1dFunction(torch.autograd.Function):
         L, K = ctx.L, wt.shape[0]
         # dL/dY = S * (1/L) * FFT(g) restricted to the kept modes (S = 1, 2, 2, ...)
         gy = rfft_modes(g.contiguous(), K, 1.0 / L, 2.0 / L, mode_major=True)   # [K, B, Co]
-        gxm = torch.matmul(gy, wt.conj().transpose(1, 2).contiguous())         # [K, B, Ci]
-        gw = torch.matmul(xm.conj().transpose(1, 2), gy)                        # [K, Ci, Co]
-        gw = gw.permute(1, 2, 0).contiguous()                                   # [Ci, Co, K]
+        gxm = torch.matmul(gy, wt.mH)                                           # [K, B, Ci]
+        gw = torch.matmul(xm.mH, gy).permute(1, 2, 0)                           # [Ci, Co, K] view
         # dL/dx = Re(F^H pad(gX)): the real part of the adjoint over K modes
         gx = irfft_modes(gxm, L, 1.0, 1.0, mode_major=True)
         return gx, gw
@@ -97,7 +102,11 @@ class SpectralConv1dFunction(torch.autograd.Function):
 
 class SpectralConv1d(torch.nn.Module):
     """1-D spectral convolution layer: in_channels -> out_channels keeping
-    `modes` Fourier modes; weights ~ N(0, 1/in_channels) complex."""
+    `modes` Fourier modes; weights ~ N(0, 1/in_channels) complex.
+
+    ``weight`` is the [in, out, modes] complex tensor; it is stored mode-major
+    (``weight_kio``, [modes, in, out]) so the per-mode GEMMs use it and produce
+    its gradient without layout copies, and ``weight`` is a view of it."""
 
     def __init__(self, in_channels: int, out_channels: int, modes: int, dtype=torch.complex64,
                  device=None, generator=None):
@@ -106,7 +115,11 @@ class SpectralConv1d(torch.nn.Module):
         real_dt = torch.float32 if dtype == torch.complex64 else torch.float64
         w = torch.randn(in_channels, out_channels, modes, 2, dtype=real_dt, device=device,
                         generator=generator) * (scale / 2 ** 0.5)
-        self.weight = torch.nn.Parameter(torch.view_as_complex(w))
+        self.weight_kio = torch.nn.Parameter(torch.view_as_complex(w).permute(2, 0, 1).contiguous())
+
+    @property
+    def weight(self):
+        return self.weight_kio.permute(1, 2, 0)
 
     def forward(self, x):
         return SpectralConv1dFunction.apply(x, self.weight)

@@ -111,6 +111,28 @@ def test_spectral_conv_vs_analytic(fy, cuda, shape, modes):
     assert ey < 2e-6 and ex < 2e-6 and ew < 2e-6, (ey, ex, ew)
 
 
+def test_spectral_conv_module_matches_function(fy, cuda):
+    """The layer (weights stored mode-major, `weight` a [Ci, Co, K] view)
+    gives the function's output and gradients; the weight gradient lands on
+    the stored parameter in its own layout."""
+    import torch
+    from fyfft_amd.autograd import SpectralConv1d, SpectralConv1dFunction
+    g = torch.Generator(device="cuda").manual_seed(11)
+    layer = SpectralConv1d(8, 6, 40, device="cuda", generator=g)
+    assert layer.weight.shape == (8, 6, 40) and layer.weight_kio.shape == (40, 8, 6)
+    x = torch.randn(3, 8, 512, device="cuda", generator=g, requires_grad=True)
+    gy = torch.randn(3, 6, 512, device="cuda", generator=g)
+    y = layer(x)
+    y.backward(gy)
+    w2 = layer.weight.detach().clone().requires_grad_()
+    x2 = x.detach().clone().requires_grad_()
+    y2 = SpectralConv1dFunction.apply(x2, w2)
+    y2.backward(gy)
+    assert torch.allclose(y, y2, rtol=1e-5, atol=1e-6)
+    assert torch.allclose(x.grad, x2.grad, rtol=1e-5, atol=1e-6)
+    assert torch.allclose(layer.weight_kio.grad.permute(1, 2, 0), w2.grad, rtol=1e-5, atol=1e-6)
+
+
 @pytest.mark.parametrize("L,k", [(4096, 512), (6720, 100), (243, 50), (1024, 1024)])
 def test_modes_mode_major_layout(fy, cuda, L, k):
     """Mode-major output/input equals the row-major one transposed, for the
