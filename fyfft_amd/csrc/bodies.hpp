@@ -16,10 +16,10 @@ namespace dev {
 // Global <-> LDS transfers issue up to 16 loads per thread before the first
 // dependent store: element i = start + j * step for j < P, guarded by
 // i < total.  P must bound the per-thread element count.
-template <int P, typename LoadF, typename StoreF>
+template <int P, typename LoadF, typename StoreF, int KMAX = 16>
 __device__ __forceinline__ void staged(int start, int step, int total, LoadF ld, StoreF st) {
   using T = decltype(ld(0));
-  constexpr int K = P < 16 ? P : 16;
+  constexpr int K = P < KMAX ? P : KMAX;
 #pragma unroll
   for (int j0 = 0; j0 < P; j0 += K) {
     T buf[K];

@@ -57,6 +57,26 @@ template <typename V, int TW> struct HermHalfSrc {  // Z[k] of an even-length C2
     return c2r_combine(xk, xm, root_at<TW>((const V*)w, k));
   }
 };
+// Z[k] of an even-length C2R from X[0..h-1] staged in the transform's LDS
+// buffer (X[h] passed in): the first pass reads both X[k] and X[h-k] from
+// LDS instead of two global loads per point from rows of h+1 bins.
+template <typename V, int TW> struct HermLdsSrc {
+  static constexpr bool kFromLds = true;
+  lds_ptr<V> p;
+  int ps;
+  gbl_cptr<V> w;  // w^k, k <= h
+  V xh;           // X[h]
+  int h;
+  __device__ V operator()(int k) const {
+    const V* L = (const V*)p;
+    const int m = k == 0 ? 0 : h - k;  // X[h] (k = 0) is not in LDS: read bin 0, then select
+    V xk = L[k + (k >> ps)];
+    const V lm = L[m + (m >> ps)];
+    V xm = V{k == 0 ? xh.x : lm.x, k == 0 ? xh.y : lm.y};
+    if (k == 0) xk.y = xm.y = 0;  // bins 0 and n/2 of a real signal are real
+    return c2r_combine(xk, xm, root_at<TW>((const V*)w, k));
+  }
+};
 template <typename V> struct RealSrc {  // (x[i], 0)
   static constexpr bool kFromLds = false;
   const __attribute__((address_space(1))) S<V>* x;
@@ -160,9 +180,41 @@ __global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_real_lds
       }
     }
   } else if constexpr (MODE == kRealC2rHalf) {
+    // X[0..h-1] is staged into LDS with loads in 128-byte aligned spans (bin
+    // k = i + ka as in the R2C store), then the first pass combines X[k] and
+    // X[h-k] from LDS: one aligned load per bin instead of two unaligned ones
     const V* X = (const V*)a.in + row * (h + 1);
     V* z = (V*)a.out + row * h;
-    F::template run<+1>(HermHalfSrc<V, kTw>{(gbl_cptr<V>)X, (gbl_cptr<V>)a.rtw, h},
+    if (active) {
+      constexpr int kLine = 128 / (int)sizeof(V);
+      const int ka0 = (int)((kLine - (row * (h + 1)) % kLine) % kLine);
+      const int ka = ka0 <= h ? ka0 : 0;
+      // loads in flight per thread: up to 16 FP32 / 8 FP64 bins (registers)
+      constexpr int P = F::kPpt + 1, kChunk = sizeof(V) == 16 ? 8 : 16;
+#pragma unroll
+      for (int j0 = 0; j0 < P; j0 += kChunk) {
+        V buf[kChunk];
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int i = tid + (j0 + j) * T;
+          if (j0 + j < P && i <= h) {
+            const int k = i + ka;
+            buf[j] = gload(X + (k > h ? k - (h + 1) : k));
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < kChunk; ++j) {
+          const int i = tid + (j0 + j) * T;
+          if (j0 + j < P && i <= h) {
+            const int k = i + ka > h ? i + ka - (h + 1) : i + ka;
+            if (k < h) lds[k + (k >> ps)] = buf[j];
+          }
+        }
+      }
+    }
+    const V xh = X[h];
+    __syncthreads();
+    F::template run<+1>(HermLdsSrc<V, kTw>{ldsp, ps, (gbl_cptr<V>)a.rtw, xh, h},
                         GlobalSink<V>{(gbl_ptr<V>)z, a.scale}, lds, h, T, tid, a.pd, a.tw, active, ps);
   } else if constexpr (MODE == kRealR2cFull) {
     const S<V>* x = (const S<V>*)a.in + row * n;
