@@ -100,6 +100,7 @@ def _load() -> ctypes.CDLL:
         L.fyfft_irfft_modes.argtypes = [vp, vp, vp, sz, dbl, dbl, vp]
         L.fyfft_rfft_modes_ex.argtypes = [vp, vp, vp, sz, dbl, dbl, i, vp]
         L.fyfft_irfft_modes_ex.argtypes = [vp, vp, vp, sz, dbl, dbl, i, vp]
+        L.fyfft_mix_modes.argtypes = [i, i, vp, vp, vp, sz, sz, sz, sz, vp]
         L.fyfft_plan_create_grouped.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(sz), sz, i, ctypes.c_uint]
         L.fyfft_plan_elements.argtypes = [vp, ctypes.POINTER(sz)]
         L.fyfft_plan_create_ex.argtypes = [ctypes.POINTER(vp), sz, sz, i, ctypes.POINTER(_Options)]
@@ -117,7 +118,7 @@ def _load() -> ctypes.CDLL:
                      "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
                      "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_rfft_modes_ex", "fyfft_irfft_modes_ex",
-                     "fyfft_plan_create_grouped",
+                     "fyfft_mix_modes", "fyfft_plan_create_grouped",
                      "fyfft_plan_elements", "fyfft_plan_create_ex", "fyfft_plan_create_real",
                      "fyfft_plan_create_many", "fyfft_plan_create_nd",
                      "fyfft_wisdom_export",
@@ -332,6 +333,46 @@ def irfft_modes(modes, L: int, scale0: float, scale: float, stream=None, mode_ma
                                         k, float(scale0), float(scale), 1 if mode_major else 0,
                                         ctypes.c_void_p(_stream_handle(stream, modes.device))), "irfft_modes")
     return y
+
+
+MIX_FORWARD, MIX_ADJOINT_X, MIX_GRAD_W = 0, 1, 2
+
+
+def mix_modes(kind: int, p, q, stream=None):
+    """Per-mode complex GEMMs of the spectral convolution on the matrix cores
+    (complex64 device tensors, mode-major): with modes [K, rows, C] and
+    weights [K, Ci, Co],
+      MIX_FORWARD:   p modes [K, R, Ci], q weights -> [K, R, Co]  (p @ q)
+      MIX_ADJOINT_X: p modes [K, R, Co], q weights -> [K, R, Ci]  (p @ q^H)
+      MIX_GRAD_W:    p modes [K, R, Ci], q modes [K, R, Co] -> [K, Ci, Co]  (p^H @ q)."""
+    import torch
+    if p.dtype != torch.complex64 or q.dtype != torch.complex64 or not p.is_cuda or not q.is_cuda:
+        raise TypeError("mix_modes: complex64 device tensors expected")
+    p, q = p.contiguous(), q.contiguous()
+    if p.dim() != 3 or q.dim() != 3 or p.shape[0] != q.shape[0]:
+        raise ValueError("mix_modes: [K, ., .] operands with equal K expected")
+    K = p.shape[0]
+    if kind == MIX_FORWARD:
+        R, ci, co = p.shape[1], p.shape[2], q.shape[2]
+        if q.shape[1] != ci:
+            raise ValueError("mix_modes: weights must be [K, Ci, Co]")
+        out = torch.empty(K, R, co, dtype=p.dtype, device=p.device)
+    elif kind == MIX_ADJOINT_X:
+        R, co, ci = p.shape[1], p.shape[2], q.shape[1]
+        if q.shape[2] != co:
+            raise ValueError("mix_modes: weights must be [K, Ci, Co]")
+        out = torch.empty(K, R, ci, dtype=p.dtype, device=p.device)
+    elif kind == MIX_GRAD_W:
+        R, ci, co = p.shape[1], p.shape[2], q.shape[2]
+        if q.shape[1] != R:
+            raise ValueError("mix_modes: both operands must be [K, R, .]")
+        out = torch.empty(K, ci, co, dtype=p.dtype, device=p.device)
+    else:
+        raise ValueError("mix_modes: unknown kind")
+    _check(_load().fyfft_mix_modes(kind, F32, ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(q.data_ptr()),
+                                   ctypes.c_void_p(out.data_ptr()), K, R, ci, co,
+                                   ctypes.c_void_p(_stream_handle(stream, p.device))), "mix_modes")
+    return out
 
 
 class _Layout(ctypes.Structure):

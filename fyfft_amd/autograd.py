@@ -17,7 +17,8 @@ from __future__ import annotations
 
 import torch
 
-from . import ADJOINT, FORWARD, INVERSE, _run, irfft_modes, rfft_modes
+from . import (ADJOINT, FORWARD, INVERSE, MIX_ADJOINT_X, MIX_FORWARD, MIX_GRAD_W, _run, irfft_modes,
+               mix_modes, rfft_modes)
 
 
 class _FFT(torch.autograd.Function):
@@ -51,25 +52,18 @@ def ifft(x):
     return _IFFT.apply(x)
 
 
-def mix_modes(xm, w):
-    """out[b, o, k] = sum_i xm[b, i, k] * w[i, o, k] as one batched GEMM over
-    the modes ([K, B, Ci] @ [K, Ci, Co])."""
-    a = xm.permute(2, 0, 1).contiguous()
-    b = w.permute(2, 0, 1).contiguous()
-    return torch.matmul(a, b).permute(1, 2, 0).contiguous()
-
-
 class SpectralConv1dFunction(torch.autograd.Function):
     """y = Re(IFFT_L(Z)), Z[0] = Y[0], Z[k] = 2 Y[k] (0 < k < K), Z = 0 above,
     Y[b,o,k] = sum_i X[b,i,k] W[i,o,k], X = FFT_L(x)[..., :K]."""
 
     # The modes live mode-major ([K, B, C]) between the transforms, so each
-    # mode's channel mixing is one GEMM of a batched matmul with no layout
-    # copies of the [B, C, K] mode tensors.  W [Ci, Co, K] is used through its
-    # [K, Ci, Co] permutation, which is a view without a copy when W is stored
-    # mode-major (as SpectralConv1d stores it); the backward GEMMs take the
-    # conjugate transposes as views (BLAS op C) and dW is returned as the same
-    # permuted view of a [K, Ci, Co] result.
+    # mode's channel mixing is one GEMM with no layout copies of the
+    # [B, C, K] mode tensors: complex64 runs the MFMA kernel of
+    # fyfft_mix_modes (forward, p @ q^H and p^H @ q forms), complex128 runs
+    # torch.matmul with the conjugate transposes as views.  W [Ci, Co, K] is
+    # used through its [K, Ci, Co] permutation, a view without a copy when W
+    # is stored mode-major (as SpectralConv1d stores it), and dW is returned
+    # as the same permuted view of a [K, Ci, Co] result.
 
     @staticmethod
     def forward(ctx, x, w):
@@ -81,7 +75,7 @@ class SpectralConv1dFunction(torch.autograd.Function):
         wt = w.permute(2, 0, 1)                               # [K, Ci, Co]
         if not wt.is_contiguous():
             wt = wt.contiguous()
-        ym = torch.matmul(xm, wt)                             # [K, B, Co]
+        ym = mix_modes(MIX_FORWARD, xm, wt) if xm.dtype == torch.complex64 else torch.matmul(xm, wt)
         y = irfft_modes(ym, L, 1.0 / L, 2.0 / L, mode_major=True)   # [B, Co, L]
         ctx.save_for_backward(xm, wt)
         ctx.L = L
@@ -93,8 +87,12 @@ class SpectralConv1dFunction(torch.autograd.Function):
         L, K = ctx.L, wt.shape[0]
         # dL/dY = S * (1/L) * FFT(g) restricted to the kept modes (S = 1, 2, 2, ...)
         gy = rfft_modes(g.contiguous(), K, 1.0 / L, 2.0 / L, mode_major=True)   # [K, B, Co]
-        gxm = torch.matmul(gy, wt.mH)                                           # [K, B, Ci]
-        gw = torch.matmul(xm.mH, gy).permute(1, 2, 0)                           # [Ci, Co, K] view
+        if gy.dtype == torch.complex64:
+            gxm = mix_modes(MIX_ADJOINT_X, gy, wt)                              # [K, B, Ci]
+            gw = mix_modes(MIX_GRAD_W, xm, gy).permute(1, 2, 0)                 # [Ci, Co, K] view
+        else:
+            gxm = torch.matmul(gy, wt.mH)
+            gw = torch.matmul(xm.mH, gy).permute(1, 2, 0)
         # dL/dx = Re(F^H pad(gX)): the real part of the adjoint over K modes
         gx = irfft_modes(gxm, L, 1.0, 1.0, mode_major=True)
         return gx, gw

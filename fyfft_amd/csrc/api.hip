@@ -1884,6 +1884,27 @@ fyfft_status fyfft_irfft_modes_ex(const fyfft_plan* p, const void* modes, void* 
   return spectral(p, true, modes, y, k, scale0, scale, layout, stream);
 }
 
+fyfft_status fyfft_mix_modes(int kind, fyfft_precision precision, const void* p, const void* q, void* out,
+                             size_t k, size_t rows, size_t cin, size_t cout, void* stream) {
+  if (kind != FYFFT_MIX_FORWARD && kind != FYFFT_MIX_ADJOINT_X && kind != FYFFT_MIX_GRAD_W) {
+    g_last_error = "fyfft_mix_modes: unknown kind";
+    return FYFFT_INVALID_ARGUMENT;
+  }
+  if (precision != FYFFT_F32) {
+    g_last_error = "fyfft_mix_modes: only FYFFT_F32 (complex float) operands";
+    return FYFFT_UNSUPPORTED;
+  }
+  if (k == 0 || rows == 0 || cin == 0 || cout == 0) return FYFFT_SUCCESS;
+  if (!p || !q || !out) return FYFFT_INVALID_ARGUMENT;
+  if (k > 65535 || rows > (size_t)1 << 30 || cin > (size_t)1 << 30 || cout > (size_t)1 << 30) {
+    g_last_error = "fyfft_mix_modes: k must be <= 65535 and rows, cin, cout < 2^30";
+    return FYFFT_INVALID_ARGUMENT;
+  }
+  HIP_TRY(launch_mix_modes(kind, p, q, out, (long long)k, (long long)rows, (long long)cin, (long long)cout,
+                           (hipStream_t)stream));
+  return FYFFT_SUCCESS;
+}
+
 fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, int direction) {
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   if (!in || !out) return FYFFT_INVALID_ARGUMENT;

@@ -269,6 +269,23 @@ fyfft_status fyfft_rfft_modes_ex(const fyfft_plan* plan, const void* x, void* mo
 fyfft_status fyfft_irfft_modes_ex(const fyfft_plan* plan, const void* modes, void* y, size_t k,
                                   double scale0, double scale, int layout, void* stream);
 
+/* Mode mixing of the spectral convolution (no counterpart in the reference):
+ * one complex GEMM per kept mode on the matrix cores, with the mode-major
+ * layouts of fyfft_rfft_modes_ex: modes [k][rows][channels], weights
+ * [k][cin][cout] (complex, interleaved re/im).  Only FYFFT_F32 (complex
+ * float) is supported; k <= 65535.  Asynchronous on `stream`.
+ *   FYFFT_MIX_FORWARD:   out[j][r][o] = sum_i p[j][r][i] * q[j][i][o]
+ *                        (p: input modes, q: weights, out: output modes)
+ *   FYFFT_MIX_ADJOINT_X: out[j][r][i] = sum_o p[j][r][o] * conj(q[j][i][o])
+ *                        (p: output-mode gradient, q: weights)
+ *   FYFFT_MIX_GRAD_W:    out[j][i][o] = sum_r conj(p[j][r][i]) * q[j][r][o]
+ *                        (p: input modes, q: output-mode gradient) */
+#define FYFFT_MIX_FORWARD 0
+#define FYFFT_MIX_ADJOINT_X 1
+#define FYFFT_MIX_GRAD_W 2
+fyfft_status fyfft_mix_modes(int kind, fyfft_precision precision, const void* p, const void* q, void* out,
+                             size_t k, size_t rows, size_t cin, size_t cout, void* stream);
+
 /* Device bytes the plan holds (tables + workspace). */
 fyfft_status fyfft_plan_workspace_bytes(const fyfft_plan* plan, size_t* bytes);
 

@@ -136,3 +136,15 @@ def test_strided_plan_argument_checks_without_gpu(fy):
     assert lib.fyfft_plan_create_nd(ctypes.byref(h), 3, dims, 0, 0) == 1                   # batch 0
     bad = (ctypes.c_size_t * 2)(4, 0)
     assert lib.fyfft_plan_create_nd(ctypes.byref(h), 2, bad, 1, 0) == 2                   # dim 0
+
+
+def test_mix_modes_argument_checks_without_gpu(fy):
+    import ctypes
+    L = fy.lib()
+    null = ctypes.c_void_p(0)
+    assert L.fyfft_mix_modes(0, 0, null, null, null, 0, 4, 4, 4, null) == 0      # empty: nothing to do
+    assert L.fyfft_mix_modes(9, 0, null, null, null, 1, 4, 4, 4, null) == 1      # unknown kind
+    assert L.fyfft_mix_modes(0, 1, null, null, null, 1, 4, 4, 4, null) == 3      # FP64: unsupported
+    assert L.fyfft_mix_modes(0, 0, null, null, null, 1, 4, 4, 4, null) == 1      # null operands
+    assert L.fyfft_mix_modes(0, 0, ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8),
+                             70000, 4, 4, 4, null) == 1                          # k > 65535
