@@ -49,6 +49,7 @@ enum RealForm { kRealFusedHalf = 0, kRealFusedFull = 1, kRealHalfPasses = 2, kRe
 constexpr size_t kWorkspaceCap = (size_t)1 << 30;
 // Longest factor of a four-step split (one column/row FFT in LDS).
 constexpr size_t kFactorCap = 4096;
+static_assert(dev::kPadShift == kLdsPadShift, "planner and kernels must pad LDS alike");
 
 fyfft_status from_hip(hipError_t e, const char* what) {
   if (e == hipSuccess) return FYFFT_SUCCESS;

@@ -3,8 +3,6 @@
 
 #include <algorithm>
 
-#include "fft_device.hpp"
-
 namespace fyfft {
 
 // The reference's directly supported lengths (src/foye_FFT_radix_wrap.hpp:93-98).
@@ -107,7 +105,7 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
     r.ppt = (int)((n + 1023) / 1024);
   }
   if (r.threads > 1024 || r.ppt > 2 * ppt_lo) return false;
-  r.pad_shift = dev::kPadShift;
+  r.pad_shift = kLdsPadShift;
   size_t per = (size_t)((n - 1) + ((n - 1) >> r.pad_shift) + 1) * elem_bytes;
   if (per > lds_cap) {
     r.pad_shift = 30;  // no padding

@@ -24,6 +24,11 @@ int strategy_for(size_t n);
 // n is not 7-smooth or n < 2.
 std::vector<int> radices_for(size_t n);
 
+// LDS padding of the planner's geometries: index i is stored at
+// i + (i >> kLdsPadShift) (the device code's dev::kPadShift, checked equal
+// where both are visible).
+constexpr int kLdsPadShift = 4;
+
 // Geometry of one LDS-resident transform group.
 struct LdsGeom {
   int n = 0;         // transform length
