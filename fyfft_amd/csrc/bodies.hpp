@@ -322,7 +322,6 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
 template <typename V, typename F, int DIR, int MODE>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, const V* x0,
                                             int r0, V* lds_in) {
-  constexpr int P = F::kPpt;
   V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
@@ -332,15 +331,11 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
   const bool active = slot < rows;
   V* lds = lds_all + (slot < C ? slot : 0) * stride;
   if constexpr (MODE == kFsPlainB) {
-    const int total = rows * N2;
-    const V* src = tmp + (long long)r0 * N2;
-    staged<P>(threadIdx.x, blockDim.x, total, [&](int i) { return src[i]; },
-              [&](int i, V v) {
-                const int c = i / N2, n2 = i - c * N2;
-                lds_all[c * stride + n2 + (n2 >> ps)] = v;
-              });
-    __syncthreads();
-    F::template run<DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    // each slot's first pass reads its (contiguous) row straight from HBM;
+    // the last pass leaves the row in LDS for the transposed store
+    const V* src = tmp + (long long)(r0 + (active ? slot : 0)) * N2;
+    F::template run_io<DIR>(GlobalSrc<V>{(gbl_cptr<V>)src}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, N2, a.threads,
+                            tid, a.pd, a.tw, active, ps);
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
       if (c >= rows) continue;
