@@ -345,7 +345,7 @@ static std::vector<LdsChoice> lds_candidates(fyfft_plan* p, size_t n) {
   if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) return out;
   const bool f64 = p->prec == kF64;
   const int lo = f64 ? 8 : 16;  // the alternative: twice or half the points per thread
-  const int alt_ppt = g.ppt == lo ? 2 * lo : (g.ppt == 2 * lo ? lo : 2 * lo);
+  const int alt_ppt = g.ppt == lo ? 2 * lo : (g.ppt < lo || g.ppt == 2 * lo ? lo : 2 * lo);
   auto tpb_for = [&](int ppt, int ps) {
     const int T = (int)((n + ppt - 1) / ppt);
     const size_t per = (size_t)dev::lds_len((int)n, ps) * elem_bytes(p->prec);
@@ -484,14 +484,17 @@ static fyfft_status wisdom_parse(const char* text) {
     size_t prod = 1;
     for (char* tok = std::strtok(rad, "x"); tok; tok = std::strtok(nullptr, "x")) {
       const int r = std::atoi(tok);
-      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 16) return FYFFT_INVALID_ARGUMENT;
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 9 && r != 16) return FYFFT_INVALID_ARGUMENT;
       c.rad.push_back(r);
       prod *= (size_t)r;
     }
-    // the planner's points per thread: lo or 2*lo, or n spread over 1024 threads
+    // the planner's points per thread: lo or 2*lo, or what its geometry for n
+    // takes (n spread over 1024 threads, or fewer points for short lengths)
     const unsigned long long lo = prec == kF64 ? 8 : 16;
+    LdsGeom geo;
     const bool ppt_ok = (unsigned long long)ppt == lo || (unsigned long long)ppt == 2 * lo ||
-                        (n > lo * 1024 && (unsigned long long)ppt == (n + 1023) / 1024);
+                        (n >= 1 && n <= (1ULL << 20) &&
+                         lds_geometry((size_t)n, prec == kF64 ? 16 : 8, (size_t)1 << 40, &geo) && geo.ppt == ppt);
     if (prod != n || (prec != kF32 && prec != kF64) || !ppt_ok || (ps != 30 && (ps < 2 || ps > 8)) ||
         tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses)
       return FYFFT_INVALID_ARGUMENT;

@@ -238,13 +238,45 @@ template <int R, int DIR, typename V> struct DftOdd {
   }
 };
 template <int DIR, typename V> struct Dft<3, DIR, V> : DftOdd<3, DIR, V> {};
+
+// cos/sin(2*pi*m/9) for m = 1, 2, 4
+constexpr double kCos9[3] = {0.766044443118978035202392650555416674, 0.173648177666930348851716626769314796,
+                             -0.939692620785908384054109277324731470};
+constexpr double kSin9[3] = {0.642787609686539326322643409907263433, 0.984807753012208059366743024589523014,
+                             0.342020143325668733044099614682259581};
+
+// 9 = 3 x 3: x[3*n1 + n2]; DFT3 over n1, twiddle w9^(n2*k1), DFT3 over n2,
+// output X[k1 + 3*k2].
+template <int DIR, typename V> struct Dft<9, DIR, V> {
+  static __device__ __forceinline__ void run(V* v) {
+    V y[9];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+      V t[3] = {v[n2], v[3 + n2], v[6 + n2]};
+      DftOdd<3, DIR, V>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 3; ++k1) {
+        const int m = n2 * k1;  // 0, 1, 2 or 4
+        const int i = m == 4 ? 2 : (m == 2 ? 1 : 0);
+        y[n2 * 3 + k1] = m ? rotc<DIR>(t[k1], kCos9[i], kSin9[i]) : t[k1];
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+      V t[3] = {y[k1], y[3 + k1], y[6 + k1]};
+      DftOdd<3, DIR, V>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) v[k1 + 3 * k2] = t[k2];
+    }
+  }
+};
 template <int DIR, typename V> struct Dft<5, DIR, V> : DftOdd<5, DIR, V> {};
 template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
 
 // ---------------------------------------------------------------- twiddles
 // Multiply inputs r = 1..R-1 of one butterfly by w^(k*r), where t points at
-// the pass table slice [w^k, w^2k, ..., w^((R-1)k)].  For R = 4, 7, 8 and 16
-// only the power-of-two powers are loaded and the others are formed by at
+// the pass table slice [w^k, w^2k, ..., w^((R-1)k)].  For R = 4, 7, 8, 9 and
+// 16 only the power-of-two powers are loaded and the others are formed by at
 // most three complex products (a few ulp), which halves to quarters the
 // table traffic of the pass; R = 2, 3, 5 load every factor.
 template <int R, int DIR, typename V>
@@ -256,6 +288,12 @@ __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
                      cmul(w3, w8), cmul(w4, w8), cmul(w5, w8), cmul(w6, w8), cmul(w7, w8)};
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = twmul<DIR>(v[r], w[r]);
+  } else if constexpr (R == 9) {
+    const V w1 = t[0], w2 = t[1], w4 = t[3], w8 = t[7];
+    const V w3 = cmul(w1, w2);
+    const V w[9] = {V{1, 0}, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4), w8};
+#pragma unroll
+    for (int r = 1; r < 9; ++r) v[r] = twmul<DIR>(v[r], w[r]);
   } else if constexpr (R == 8 || R == 7) {
     const V w1 = t[0], w2 = t[1], w4 = t[3];
     const V w3 = cmul(w1, w2);
@@ -431,6 +469,7 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
       case 5: lds_pass_ool<V, 5, cdiv(PPT, 5), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
       case 7: lds_pass_ool<V, 7, cdiv(PPT, 7), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
       case 8: lds_pass_ool<V, 8, cdiv(PPT, 8), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 9: lds_pass_ool<V, 9, cdiv(PPT, 9), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
       default: lds_pass_ool<V, 16, cdiv(PPT, 16), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
     }
     Ls *= R;
@@ -542,6 +581,7 @@ template <typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
     case 5: f(RadixC<5>{}); break;
     case 7: f(RadixC<7>{}); break;
     case 8: f(RadixC<8>{}); break;
+    case 9: f(RadixC<9>{}); break;
     default: f(RadixC<16>{}); break;
   }
 }

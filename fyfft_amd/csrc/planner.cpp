@@ -83,7 +83,9 @@ std::vector<int> radices_for(size_t n) {
   }
   for (int i = 0; i < e[7]; ++i) out.push_back(7);
   for (int i = 0; i < e[5]; ++i) out.push_back(5);
-  for (int i = 0; i < e[3]; ++i) out.push_back(3);
+  // threes in pairs as radix 9 (3 x 3 in registers): half the LDS passes
+  for (int i = 0; i < e[3] / 2; ++i) out.push_back(9);
+  if (e[3] % 2) out.push_back(3);
   return out;
 }
 
@@ -103,6 +105,12 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
     // latency with one workgroup per CU), ppt_lo < points per thread <= 2 * ppt_lo
     r.threads = 1024;
     r.ppt = (int)((n + 1023) / 1024);
+  }
+  if (elem_bytes < 16 && r.threads < 32) {
+    // short FP32 transforms (n < 512): n/16 threads each would leave most
+    // lanes waiting through passes of few butterflies; n/8 measured faster
+    r.ppt = 8;
+    r.threads = (int)((n + 7) / 8);
   }
   if (r.threads > 1024 || r.ppt > 2 * ppt_lo) return false;
   r.pad_shift = kLdsPadShift;

@@ -19,9 +19,10 @@ uint64_t power_mod(uint64_t base, uint64_t e, uint64_t mod);
 uint64_t primitive_root(uint64_t p);  // smallest generator of (Z/pZ)*, p prime
 int strategy_for(size_t n);
 
-// Radix passes (each in {2,3,4,5,7,8,16}) for a smooth length: powers of two
-// spread over ceil(log2/4) passes of at most 16, then 7s, 5s, 3s.  Empty if
-// n is not 7-smooth or n < 2.
+// Radix passes (each in {2,3,4,5,7,8,9,16}) for a smooth length: powers of
+// two spread over ceil(log2/4) passes of at most 16, then 7s, 5s, the threes
+// paired as 9s and a last 3 if their count is odd.  Empty if n is not
+// 7-smooth or n < 2.
 std::vector<int> radices_for(size_t n);
 
 // LDS padding of the planner's geometries: index i is stored at

@@ -29,7 +29,7 @@ static void check_length(size_t n, const std::vector<char>& composite) {
     CHECK(!r.empty() && r.size() <= 16);
     size_t prod = 1;
     for (int x : r) {
-      CHECK(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 16);
+      CHECK(x == 2 || x == 3 || x == 4 || x == 5 || x == 7 || x == 8 || x == 9 || x == 16);
       prod *= (size_t)x;
     }
     CHECK(prod == n);

@@ -68,13 +68,15 @@ def test_radices_factor_every_smooth_length(fy):
                 fy.length_radices(n)
             continue
         r = fy.length_radices(n)
-        assert all(x in (2, 3, 4, 5, 7, 8, 16) for x in r)
+        assert all(x in (2, 3, 4, 5, 7, 8, 9, 16) for x in r)
         prod = 1
         for x in r:
             prod *= x
         assert prod == n
     assert fy.length_radices(4096) == [16, 16, 16]
     assert fy.length_radices(6720) == [8, 8, 7, 5, 3]
+    assert fy.length_radices(729) == [9, 9, 9]        # threes paired as radix 9
+    assert fy.length_radices(2187) == [9, 9, 9, 3]
 
 
 def test_invoker_mirror_strategy(fy):
