@@ -184,8 +184,10 @@ static fyfft_status make_pass_table(fyfft_plan* p, size_t n, dev::PassDesc* pd, 
 }
 
 // Plan an LDS-resident smooth transform of length n.
-static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_fixed) {
-  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &L->geom)) return FYFFT_UNSUPPORTED;
+// short_threads: lds_geometry's option for batched plans (not convolutions).
+static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_fixed, bool short_threads) {
+  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &L->geom, short_threads))
+    return FYFFT_UNSUPPORTED;
   void* tw = nullptr;
   fyfft_status st = make_pass_table(p, n, &L->pd, &tw);
   if (st) return st;
@@ -342,7 +344,7 @@ static fyfft_status make_candidate(fyfft_plan* p, size_t n, const LdsChoice& c, 
 static std::vector<LdsChoice> lds_candidates(fyfft_plan* p, size_t n) {
   std::vector<LdsChoice> out;
   LdsGeom g;
-  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) return out;
+  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &g, true)) return out;
   const bool f64 = p->prec == kF64;
   const int lo = f64 ? 8 : 16;  // the alternative: twice or half the points per thread
   const int alt_ppt = g.ppt == lo ? 2 * lo : (g.ppt < lo || g.ppt == 2 * lo ? lo : 2 * lo);
@@ -494,7 +496,7 @@ static fyfft_status wisdom_parse(const char* text) {
     LdsGeom geo;
     const bool ppt_ok = (unsigned long long)ppt == lo || (unsigned long long)ppt == 2 * lo ||
                         (n >= 1 && n <= (1ULL << 20) &&
-                         lds_geometry((size_t)n, prec == kF64 ? 16 : 8, (size_t)1 << 40, &geo) && geo.ppt == ppt);
+                         lds_geometry((size_t)n, prec == kF64 ? 16 : 8, (size_t)1 << 40, &geo, true) && geo.ppt == ppt);
     if (prod != n || (prec != kF32 && prec != kF64) || !ppt_ok || (ps != 30 && (ps < 2 || ps > 8)) ||
         tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses)
       return FYFFT_INVALID_ARGUMENT;
@@ -527,7 +529,7 @@ static fyfft_status plan_conv_structure(fyfft_plan* p, size_t m, bool rader) {
   LdsGeom g;
   if (lds_geometry(m, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) {
     p->kind = rader ? kRaderLds : kBlueLds;
-    fyfft_status st = plan_lds(p, m, &p->lds, false);
+    fyfft_status st = plan_lds(p, m, &p->lds, false, false);
     if (!st) p->lds.fixed = find_conv_fixed(p->prec, p->lds.pd, &p->lds.geom);
     return st;
   }
@@ -846,7 +848,7 @@ static bool spectral_half_tables(const fyfft_plan* cp) {
   const size_t n = p->n, h = n / 2;
   cp->half_state = -1;
   if (n % 2 || h < 2 || !is_smooth7(h)) return false;
-  if (plan_lds(p, h, &p->half, true)) return false;
+  if (plan_lds(p, h, &p->half, true, true)) return false;
   if (p->alloc((h + 1) * elem_bytes(p->prec), &p->half_rtw)) return false;
   if (launch_gen_roots(p->prec, p->half_rtw, (long long)(h + 1), 1, (long long)n, nullptr) != hipSuccess) return false;
   if (hipDeviceSynchronize() != hipSuccess) return false;
@@ -968,7 +970,7 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     LdsGeom g;
     if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
       p->kind = kLds;
-      st = form == FYFFT_FORM_TUNED ? plan_lds_tuned(p, n) : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO);
+      st = form == FYFFT_FORM_TUNED ? plan_lds_tuned(p, n) : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO, true);
       if (!st) {
         const LdsGeom& q = p->lds.geom;
         p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") +

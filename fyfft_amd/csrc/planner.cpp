@@ -91,7 +91,7 @@ std::vector<int> radices_for(size_t n) {
 
 size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap) { return lds_cap / elem_bytes; }
 
-bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
+bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads) {
   if (n < 1 || n > (size_t)1 << 20) return false;
   LdsGeom r;
   r.n = (int)n;
@@ -106,9 +106,7 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g) {
     r.threads = 1024;
     r.ppt = (int)((n + 1023) / 1024);
   }
-  if (elem_bytes < 16 && r.threads < 32) {
-    // short FP32 transforms (n < 512): n/16 threads each would leave most
-    // lanes waiting through passes of few butterflies; n/8 measured faster
+  if (short_threads && elem_bytes < 16 && r.threads < 32) {
     r.ppt = 8;
     r.threads = (int)((n + 7) / 8);
   }

@@ -40,8 +40,12 @@ struct LdsGeom {
   size_t lds_bytes = 0;
 };
 // Fits n (elements of `elem_bytes`) in one workgroup's LDS of `lds_cap`
-// bytes; returns false if it does not fit even unpadded.
-bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g);
+// bytes; returns false if it does not fit even unpadded.  short_threads
+// (batched LDS plans): FP32 lengths below 512 take n/8 threads instead of
+// n/16 (with n/16, fewer than 32, passes of few butterflies leave most lanes
+// waiting; measured 1.5-3x slower).  Four-step sub-transforms keep n/16,
+// which their compile-time forms assume.
+bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads = false);
 size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded
 
 // Four-step split n = n1 * n2 with both factors smooth and LDS-resident

@@ -34,12 +34,14 @@ static void check_length(size_t n, const std::vector<char>& composite) {
     }
     CHECK(prod == n);
     for (size_t elem : {8u, 16u}) {
-      LdsGeom g;
-      if (lds_geometry(n, elem, 163840, &g)) {
-        CHECK(g.threads >= 1 && g.threads <= 1024 && g.per_block >= 1);
-        CHECK((size_t)g.ppt * g.threads >= n);
-        CHECK(g.lds_bytes <= 163840);
-        CHECK((size_t)g.threads * g.per_block <= 1024);
+      for (bool short_threads : {false, true}) {
+        LdsGeom g;
+        if (lds_geometry(n, elem, 163840, &g, short_threads)) {
+          CHECK(g.threads >= 1 && g.threads <= 1024 && g.per_block >= 1);
+          CHECK((size_t)g.ppt * g.threads >= n);
+          CHECK(g.lds_bytes <= 163840);
+          CHECK((size_t)g.threads * g.per_block <= 1024);
+        }
       }
       size_t a = 0, b = 0;
       if (four_step_split(n, 4096, &a, &b)) CHECK(a * b == n && a <= 4096 && b <= 4096 && a <= b);

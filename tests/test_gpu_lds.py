@@ -41,7 +41,7 @@ def run(fy, torch, x, direction, inplace=False):
 
 SMOOTH = [2, 3, 4, 5, 7, 8, 9, 12, 16, 25, 27, 32, 49, 60, 64, 81, 105, 125, 128, 243, 256,
           343, 360, 512, 625, 720, 729, 1000, 1024, 2048, 2187, 2401, 3125, 4096, 4608,
-          6561, 6720, 8192, 15625, 16384, 16807]
+          6561, 6720, 8192, 15625, 16384, 16807, 19683]
 
 
 @pytest.mark.parametrize("n", SMOOTH)
