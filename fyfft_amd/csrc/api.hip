@@ -240,8 +240,8 @@ static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
   fyfft_status st = make_pass_table(p, p->n1, &p->pdcol, &p->twcol);
   if (!st) st = make_pass_table(p, p->n2, &p->pdrow, &p->twrow);
   if (st) return st;
-  p->fixed_col = find_fs_fixed(p->prec, p->pdcol, (int)p->n1, p->gcol.threads);
-  p->fixed_row = find_fs_fixed(p->prec, p->pdrow, (int)p->n2, p->grow.threads);
+  p->fixed_col = find_fs_fixed(p->prec, p->pdcol, (int)p->n1, p->gcol.threads, conv);
+  p->fixed_row = find_fs_fixed(p->prec, p->pdrow, (int)p->n2, p->grow.threads, conv);
   int lg = 0;
   while (((size_t)1 << lg) < n) ++lg;
   p->lo_bits = (lg + 1) / 2;
@@ -665,7 +665,7 @@ static double conv_length_cost(size_t c, size_t elem, size_t cap, int prec) {
     dev::PassDesc pd{};
     pd.npass = (int)r.size();
     for (size_t i = 0; i < r.size() && i < (size_t)dev::kMaxPasses; ++i) pd.radix[i] = r[i];
-    return find_fs_fixed(prec, pd, (int)len, q.threads) != 0;
+    return find_fs_fixed(prec, pd, (int)len, q.threads, true) != 0;
   };
   const double passes = (double)(radices_for(n1).size() + radices_for(n2).size());
   return (double)c * passes * 1.6 * (fixed(n1, a) && fixed(n2, b) ? 0.7 : 1.0);
@@ -1520,7 +1520,7 @@ static fyfft_status plan_strided_pass(fyfft_plan* p, StridedPass& sp, fyfft_prec
     sp.method = kPmColumns;
     st = make_pass_table(p, sp.n, &sp.pd, &sp.tw);
     if (st) return st;
-    sp.fixed_n = find_fs_fixed(p->prec, sp.pd, (int)sp.n, sp.g.threads);
+    sp.fixed_n = find_fs_fixed(p->prec, sp.pd, (int)sp.n, sp.g.threads, true);
     return FYFFT_SUCCESS;
   }
   sp.method = kPmGather;

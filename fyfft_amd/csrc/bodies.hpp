@@ -63,7 +63,7 @@ __device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, b
 // FixedFft: length, threads per transform and radices known at compile time,
 // passes inlined.  The plan must use exactly this geometry and pass list.
 template <typename V, int N, int T, int... Rs> struct FixedFft {
-  static constexpr int kPpt = N / T;
+  static constexpr int kPpt = (N + T - 1) / T;
   template <int D>
   static __device__ __forceinline__ void run(V* lds, int, int, int tid, const PassDesc&,
                                              const V* tw, bool active, int ps) {

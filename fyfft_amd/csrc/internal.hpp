@@ -96,7 +96,14 @@ template <typename V> struct FsArgs {
 hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* args, size_t batch,
                           const LdsGeom& g, int fixed_n, hipStream_t s);
 // Length of a compile-time sub-transform matching (prec, passes, n, threads), or 0.
-int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads);
+// all_modes: the caller runs modes other than the plain column/row passes
+// (convolution plans, the layout kernels), which the odd-radix compile-time
+// forms do not serve.
+int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads, bool all_modes);
+// Plain column (col) or row pass with a compile-time odd-radix sub-transform
+// of length n (a plain-only length of find_fs_fixed).
+hipError_t launch_fs_odd(int prec, int n, bool col, int dir, const void* args, size_t batch, size_t lds,
+                         int threads, hipStream_t s);
 
 // --- kernels_conv.hip
 template <typename V> struct ConvArgs {

@@ -19,97 +19,45 @@
 // Twiddles w_N^m (m < N) come from two FP64-generated tables,
 // w_N^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)], each about sqrt(N)
 // entries, so they stay cache resident for any N.
-#include "bodies.hpp"
+#include "fourstep.hpp"
 
 namespace fyfft {
 using namespace dev;
 
-template <typename V, typename F, int DIR, int MODE>
-__global__ __launch_bounds__(1024) void k_fs_col(FsArgs<V> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int groups = (a.n2 + (1 << a.log2c) - 1) >> a.log2c;
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const long long t = lb / groups;
-  const int c0 = (int)(lb - t * groups) << a.log2c;
-  body_fs_col<V, F, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
-                               a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
-                               reinterpret_cast<V*>(smem_raw), a.zw ? a.zw + t * a.zw_dist : nullptr);
-}
-
-template <typename V, typename F, int DIR, int MODE>
-__global__ __launch_bounds__(1024) void k_fs_row(FsArgs<V> a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int groups = (a.n1 + (1 << a.log2c) - 1) >> a.log2c;
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const long long t = lb / groups;
-  const int r0 = (int)(lb - t * groups) << a.log2c;
-  body_fs_row<V, F, DIR, MODE>(a, a.tmp + t * a.tmp_dist, a.out + t * a.out_dist,
-                               a.x0buf ? a.x0buf + t : nullptr, r0,
-                               reinterpret_cast<V*>(smem_raw));
-}
-
-// ------------------------------------------------------------ launch
-template <typename V, typename F, int DIR, int MODE>
-static hipError_t go_col(const FsArgs<V>& a, size_t batch, size_t lds, int threads, hipStream_t s) {
-  auto k = k_fs_col<V, F, DIR, MODE>;
-  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  const size_t groups = ((size_t)a.n2 + (1 << a.log2c) - 1) >> a.log2c;
-  hipLaunchKernelGGL(k, dim3((unsigned)(groups * batch)), dim3(threads), lds, s, a);
-  return hipGetLastError();
-}
-template <typename V, typename F, int DIR, int MODE>
-static hipError_t go_row(const FsArgs<V>& a, size_t batch, size_t lds, int threads, hipStream_t s) {
-  auto k = k_fs_row<V, F, DIR, MODE>;
-  if (lds > 65536) (void)hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  const size_t groups = ((size_t)a.n1 + (1 << a.log2c) - 1) >> a.log2c;
-  hipLaunchKernelGGL(k, dim3((unsigned)(groups * batch)), dim3(threads), lds, s, a);
-  return hipGetLastError();
-}
-
-template <typename V, typename F>
-static hipError_t dispatch_fs(bool col, int mode, int dir, const FsArgs<V>& a, size_t batch,
-                              size_t lds, int threads, hipStream_t s) {
-  if (col) {
-    switch (mode) {
-      case kFsPlainA:
-        return dir < 0 ? go_col<V, F, -1, kFsPlainA>(a, batch, lds, threads, s)
-                       : go_col<V, F, +1, kFsPlainA>(a, batch, lds, threads, s);
-      case kFsRaderA: return go_col<V, F, -1, kFsRaderA>(a, batch, lds, threads, s);
-      case kFsBlueA: return go_col<V, F, -1, kFsBlueA>(a, batch, lds, threads, s);
-      case kFsRaderC: return go_col<V, F, +1, kFsRaderC>(a, batch, lds, threads, s);
-      case kFsRaderC2: return go_col<V, F, +1, kFsRaderC2>(a, batch, lds, threads, s);
-      case kFsBlueC: return go_col<V, F, +1, kFsBlueC>(a, batch, lds, threads, s);
-    }
-  } else {
-    switch (mode) {
-      case kFsPlainB:
-        return dir < 0 ? go_row<V, F, -1, kFsPlainB>(a, batch, lds, threads, s)
-                       : go_row<V, F, +1, kFsPlainB>(a, batch, lds, threads, s);
-      case kFsConvB: return go_row<V, F, -1, kFsConvB>(a, batch, lds, threads, s);
-    }
-  }
-  return hipErrorInvalidValue;
-}
-
-// Compile-time sub-transforms for the power-of-two factor lengths (radices
-// as planner.cpp radices_for, threads = n / points-per-thread).
+// Compile-time sub-transforms (radices as planner.cpp radices_for, threads =
+// ceil(n / points-per-thread) as the four-step geometry takes them): every
+// mode for the power-of-two lengths; the plain modes only for the odd-radix
+// factors of the reference table's four-step lengths (59049 = 243 x 243,
+// 78125 = 125 x 625, FP64 15625 / 16807 / 19683), which live in
+// kernels_fourstep_odd.hip.
 struct FsFixed {
   int prec, n, threads, npass;
   int radix[4];
+  bool plain_only;
 };
 static const FsFixed kFsFixed[] = {
-    {kF32, 64, 4, 2, {8, 8}},       {kF32, 128, 8, 2, {16, 8}},      {kF32, 256, 16, 2, {16, 16}},
-    {kF32, 512, 32, 3, {8, 8, 8}},  {kF32, 1024, 64, 3, {16, 8, 8}}, {kF32, 2048, 128, 3, {16, 16, 8}},
-    {kF32, 4096, 256, 3, {16, 16, 16}}, {kF32, 8192, 512, 4, {16, 8, 8, 8}}, {kF32, 16384, 1024, 4, {16, 16, 8, 8}},
-    {kF64, 64, 8, 2, {8, 8}},       {kF64, 128, 16, 2, {16, 8}},     {kF64, 256, 32, 2, {16, 16}},
-    {kF64, 512, 64, 3, {8, 8, 8}},  {kF64, 1024, 128, 3, {16, 8, 8}}, {kF64, 2048, 256, 3, {16, 16, 8}},
-    {kF64, 4096, 512, 3, {16, 16, 16}}, {kF64, 8192, 1024, 4, {16, 8, 8, 8}},
+    {kF32, 64, 4, 2, {8, 8}, false},       {kF32, 128, 8, 2, {16, 8}, false},
+    {kF32, 256, 16, 2, {16, 16}, false},   {kF32, 512, 32, 3, {8, 8, 8}, false},
+    {kF32, 1024, 64, 3, {16, 8, 8}, false}, {kF32, 2048, 128, 3, {16, 16, 8}, false},
+    {kF32, 4096, 256, 3, {16, 16, 16}, false}, {kF32, 8192, 512, 4, {16, 8, 8, 8}, false},
+    {kF32, 16384, 1024, 4, {16, 16, 8, 8}, false},
+    {kF64, 64, 8, 2, {8, 8}, false},       {kF64, 128, 16, 2, {16, 8}, false},
+    {kF64, 256, 32, 2, {16, 16}, false},   {kF64, 512, 64, 3, {8, 8, 8}, false},
+    {kF64, 1024, 128, 3, {16, 8, 8}, false}, {kF64, 2048, 256, 3, {16, 16, 8}, false},
+    {kF64, 4096, 512, 3, {16, 16, 16}, false}, {kF64, 8192, 1024, 4, {16, 8, 8, 8}, false},
+    {kF32, 49, 4, 2, {7, 7}, true},        {kF32, 81, 6, 2, {9, 9}, true},
+    {kF32, 125, 8, 3, {5, 5, 5}, true},    {kF32, 243, 16, 3, {9, 9, 3}, true},
+    {kF32, 343, 22, 3, {7, 7, 7}, true},   {kF32, 625, 40, 4, {5, 5, 5, 5}, true},
+    {kF64, 49, 7, 2, {7, 7}, true},        {kF64, 81, 11, 2, {9, 9}, true},
+    {kF64, 125, 16, 3, {5, 5, 5}, true},   {kF64, 243, 31, 3, {9, 9, 3}, true},
+    {kF64, 343, 43, 3, {7, 7, 7}, true},   {kF64, 625, 79, 4, {5, 5, 5, 5}, true},
 };
 
-int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads) {
+int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads, bool all_modes) {
   for (int i = 0; i < (int)(sizeof(kFsFixed) / sizeof(kFsFixed[0])); ++i) {
     const FsFixed& e = kFsFixed[i];
-    if (e.prec != prec || e.n != n || e.threads != threads || e.npass != pd.npass) continue;
+    if (e.prec != prec || e.n != n || e.threads != threads || e.npass != pd.npass || (all_modes && e.plain_only))
+      continue;
     bool same = true;
     for (int p = 0; p < pd.npass; ++p) same = same && e.radix[p] == pd.radix[p];
     if (same) return e.n;
@@ -121,6 +69,11 @@ template <typename V, int PPT>
 static hipError_t dispatch_len(int fixed_n, bool col, int mode, int dir, const FsArgs<V>& a,
                                size_t batch, size_t lds, int threads, hipStream_t s) {
   constexpr int P = sizeof(V) == 8 ? 16 : 8;  // points per thread of the fixed forms
+  if (fixed_n & 1) {  // odd-radix forms: plain modes only
+    if (mode == (col ? kFsPlainA : kFsPlainB))
+      return launch_fs_odd(sizeof(V) == 16 ? kF64 : kF32, fixed_n, col, dir, &a, batch, lds, threads, s);
+    return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);
+  }
   switch (fixed_n) {
     case 64: return dispatch_fs<V, FixedFft<V, 64, 64 / P, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
     case 128: return dispatch_fs<V, FixedFft<V, 128, 128 / P, 16, 8>>(col, mode, dir, a, batch, lds, threads, s);

@@ -59,7 +59,8 @@ def test_fourstep_f32(fy, orc, cuda, n, batch):
     check(fy, orc, gauss(np.random.default_rng(n), batch, n, "f32"), "fourstep")
 
 
-@pytest.mark.parametrize("n,batch", [(16384, 2), (65536, 2)])
+@pytest.mark.parametrize("n,batch", [(16384, 2), (65536, 2), (15625, 3), (16807, 2), (19683, 2),
+                                     (59049, 1), (78125, 1)])  # odd: compile-time odd-radix factors
 def test_fourstep_f64(fy, orc, cuda, n, batch):
     check(fy, orc, gauss(np.random.default_rng(n), batch, n, "f64"), "fourstep")
 
