@@ -29,7 +29,7 @@ enum Kind { kTrivial = 0, kLds = 1, kFourStep = 2, kRaderLds = 3, kRaderFs = 4, 
 struct Layout2 {
   long long stride = 1, cdist = 0, pdist = 0;
 };
-enum PassMethod { kPmRows = 0, kPmColumns = 1, kPmGather = 2 };
+enum PassMethod { kPmRows = 0, kPmColumns = 1, kPmGather = 2, kPmColFour = 3 };
 struct StridedPass {
   size_t n = 0, ncols = 1, planes = 1;
   Layout2 in, out;
@@ -39,7 +39,13 @@ struct StridedPass {
   dev::PassDesc pd{};
   void* tw = nullptr;
   int fixed_n = 0;
-  size_t chunk = 0;           // gather: transforms per workspace chunk
+  size_t chunk = 0;           // gather: transforms per workspace chunk; colfour: columns
+  // colfour: the column axis split as n = n1 * n2, two column passes
+  size_t n1 = 0, n2 = 0;
+  LdsGeom g2;
+  dev::PassDesc pd2{};
+  void *tw2 = nullptr, *tw_hi = nullptr, *tw_lo = nullptr;
+  int fixed2 = 0, lo_bits = 0;
 };
 
 // Forms of a real (R2C / C2R) plan.
@@ -1523,6 +1529,36 @@ static fyfft_status plan_strided_pass(fyfft_plan* p, StridedPass& sp, fyfft_prec
     sp.fixed_n = find_fs_fixed(p->prec, sp.pd, (int)sp.n, sp.g.threads, true);
     return FYFFT_SUCCESS;
   }
+  // Adjacent transforms too long for one column tile: two column passes over
+  // n = n1 x n2 (lengths n1 then n2, the twiddle w_n^(n2*k1) between them)
+  // through a workspace of `chunk` columns, each pass reading and writing
+  // row segments of adjacent columns -- two round trips instead of the
+  // gather's three.
+  size_t n1 = 0, n2 = 0;
+  if (sp.in.cdist == 1 && sp.out.cdist == 1 && sp.ncols >= 16 && is_smooth7(sp.n) &&
+      four_step_split(sp.n, kFactorCap, &n1, &n2) && fs_geometry(n1, elem, cap, &sp.g) &&
+      fs_geometry(n2, elem, cap, &sp.g2) && (size_t)sp.g.per_block * elem >= 64 &&
+      (size_t)sp.g2.per_block * elem >= 64) {
+    sp.method = kPmColFour;
+    sp.n1 = n1;
+    sp.n2 = n2;
+    st = make_pass_table(p, n1, &sp.pd, &sp.tw);
+    if (!st) st = make_pass_table(p, n2, &sp.pd2, &sp.tw2);
+    if (st) return st;
+    sp.fixed_n = find_fs_fixed(p->prec, sp.pd, (int)n1, sp.g.threads, true);
+    sp.fixed2 = find_fs_fixed(p->prec, sp.pd2, (int)n2, sp.g2.threads, true);
+    int lg = 0;
+    while (((size_t)1 << lg) < sp.n) ++lg;
+    sp.lo_bits = (lg + 1) / 2;
+    const size_t lo = (size_t)1 << sp.lo_bits, hi = (sp.n >> sp.lo_bits) + 1;
+    if ((st = p->alloc(lo * elem, &sp.tw_lo)) || (st = p->alloc(hi * elem, &sp.tw_hi))) return st;
+    HIP_TRY(launch_gen_roots(p->prec, sp.tw_lo, (long long)lo, 1, (long long)sp.n, nullptr));
+    HIP_TRY(launch_gen_roots(p->prec, sp.tw_hi, (long long)hi, (long long)lo, (long long)sp.n, nullptr));
+    size_t cw = std::max<size_t>(16, kWorkspaceCap / (sp.n * elem)) & ~(size_t)15;
+    sp.chunk = std::min(sp.ncols, cw);
+    *ws_elems = std::max(*ws_elems, sp.chunk * sp.n);
+    return FYFFT_SUCCESS;
+  }
   sp.method = kPmGather;
   sp.chunk = std::min(count, std::max<size_t>(1, kWorkspaceCap / (sp.n * elem)));
   st = create_plan_impl(&sub, sp.n, sp.chunk, precision, FYFFT_FORM_AUTO);
@@ -1556,6 +1592,62 @@ static fyfft_status exec_columns(const fyfft_plan* p, const StridedPass& sp, con
   return FYFFT_SUCCESS;
 }
 
+// The two column passes of a long column axis, per plane and column chunk:
+// pass 1 transforms columns of length n1 (rows n1*n2*stride apart, inner
+// plane n2 at offset n2*stride) into ws[(k1*n2 + n2)*cw + c] with the twiddle
+// w_n^(n2*k1); pass 2 transforms length n2 along ws and writes
+// X[(k1 + n1*k2)*stride + c].
+template <typename V>
+static fyfft_status exec_colfour(const fyfft_plan* p, const StridedPass& sp, const void* in, void* out,
+                                 int direction, hipStream_t s) {
+  const int dir = direction == FYFFT_FORWARD ? -1 : +1;
+  V* ws = (V*)p->lws;
+  const long long n1 = (long long)sp.n1, n2 = (long long)sp.n2;
+  for (size_t pl = 0; pl < sp.planes; ++pl)
+    for (size_t c0 = 0; c0 < sp.ncols; c0 += sp.chunk) {
+      const long long cw = (long long)std::min(sp.chunk, sp.ncols - c0);
+      ColArgs<V> a;
+      a.in = (const V*)in + (long long)pl * sp.in.pdist + (long long)c0;
+      a.out = ws;
+      a.istride = n2 * sp.in.stride;
+      a.ipdist2 = sp.in.stride;
+      a.ostride = n2 * cw;
+      a.opdist2 = cw;
+      a.inner = (int)n2;
+      a.n = (int)n1;
+      a.ncols = (int)cw;
+      a.log2c = ilog2_exact(sp.g.per_block);
+      a.threads = sp.g.threads;
+      a.ps = sp.g.pad_shift;
+      a.stride = dev::lds_len((int)n1, sp.g.pad_shift);
+      a.pd = sp.pd;
+      a.tw = (const V*)sp.tw;
+      a.tw_hi = (const V*)sp.tw_hi;
+      a.tw_lo = (const V*)sp.tw_lo;
+      a.lo_bits = sp.lo_bits;
+      HIP_TRY(launch_columns(p->prec, dir, &a, (size_t)n2, sp.g, sp.fixed_n, s));
+      ColArgs<V> b;
+      b.in = ws;
+      b.out = (V*)out + (long long)pl * sp.out.pdist + (long long)c0;
+      b.istride = cw;
+      b.ipdist2 = n2 * cw;
+      b.ostride = n1 * sp.out.stride;
+      b.opdist2 = sp.out.stride;
+      b.inner = (int)n1;
+      b.n = (int)n2;
+      b.ncols = (int)cw;
+      b.log2c = ilog2_exact(sp.g2.per_block);
+      b.threads = sp.g2.threads;
+      b.ps = sp.g2.pad_shift;
+      b.stride = dev::lds_len((int)n2, sp.g2.pad_shift);
+      b.pd = sp.pd2;
+      b.tw = (const V*)sp.tw2;
+      b.scale = (dev::S<V>)(direction == FYFFT_INVERSE ? 1.0 / (double)sp.n : 1.0);
+      HIP_TRY(launch_columns(p->prec, dir, &b, (size_t)n1, sp.g2, sp.fixed2, s));
+    }
+  return FYFFT_SUCCESS;
+}
+
 static fyfft_status exec_strided(const fyfft_plan* p, const void* in, void* out, int direction, hipStream_t s) {
   for (size_t i = 0; i < p->passes.size(); ++i) {
     const StridedPass& sp = p->passes[i];
@@ -1566,6 +1658,9 @@ static fyfft_status exec_strided(const fyfft_plan* p, const void* in, void* out,
     } else if (sp.method == kPmColumns) {
       st = p->prec == kF64 ? exec_columns<double2>(p, sp, src, out, direction, s)
                            : exec_columns<float2>(p, sp, src, out, direction, s);
+    } else if (sp.method == kPmColFour) {
+      st = p->prec == kF64 ? exec_colfour<double2>(p, sp, src, out, direction, s)
+                           : exec_colfour<float2>(p, sp, src, out, direction, s);
     } else {
       const size_t count = sp.ncols * sp.planes;
       for (size_t t0 = 0; t0 < count && !st; t0 += sp.chunk) {
@@ -1584,7 +1679,7 @@ static fyfft_status exec_strided(const fyfft_plan* p, const void* in, void* out,
 }
 
 static const char* method_name(int m) {
-  return m == kPmRows ? "rows" : (m == kPmColumns ? "columns" : "gather");
+  return m == kPmRows ? "rows" : (m == kPmColumns ? "columns" : (m == kPmColFour ? "colfour" : "gather"));
 }
 
 static fyfft_status finish_strided(fyfft_plan* p, size_t ws_elems, const std::string& head) {
@@ -1598,8 +1693,12 @@ static fyfft_status finish_strided(fyfft_plan* p, size_t ws_elems, const std::st
   p->desc = head;
   for (const StridedPass& sp : p->passes)
     p->desc += std::string(" | n=") + std::to_string(sp.n) + " " + method_name(sp.method) +
-               (sp.sub ? " [" + sp.sub->desc + "]" : " [C=" + std::to_string(sp.g.per_block) + " " +
-                                                          radix_string(sp.pd) + "]");
+               (sp.sub ? " [" + sp.sub->desc + "]"
+                : sp.method == kPmColFour
+                    ? " [" + std::to_string(sp.n1) + "(" + radix_string(sp.pd) + ") x " + std::to_string(sp.n2) + "(" +
+                          radix_string(sp.pd2) + ") C=" + std::to_string(sp.g.per_block) + "/" +
+                          std::to_string(sp.g2.per_block) + " chunk=" + std::to_string(sp.chunk) + "]"
+                    : " [C=" + std::to_string(sp.g.per_block) + " " + radix_string(sp.pd) + "]");
   return FYFFT_SUCCESS;
 }
 

@@ -200,6 +200,17 @@ template <typename V> struct ColArgs {
   dev::PassDesc pd{};
   const V* tw = nullptr;
   dev::S<V> scale = 1;
+  // inner planes (the two passes of a long column axis split as A x B):
+  // launch plane q is outer plane q / inner, inner plane b = q % inner at
+  // extra offsets b * ipdist2 / b * opdist2
+  int inner = 1;
+  long long ipdist2 = 0, opdist2 = 0;
+  // if set, output element k of inner plane b is multiplied by w_N^(b*k)
+  // (w_N^m = tw_hi[m >> lo_bits] * tw_lo[m & (2^lo_bits - 1)], conjugated
+  // for the inverse direction)
+  const V* tw_hi = nullptr;
+  const V* tw_lo = nullptr;
+  int lo_bits = 0;
 };
 hipError_t launch_columns(int prec, int dir, const void* args, size_t planes, const LdsGeom& g, int fixed_n,
                           hipStream_t s);

@@ -9,7 +9,7 @@ using namespace dev;
 // Columns [c0, c0 + C) of one plane: C*n elements read as n rows of C
 // adjacent values, C column transforms in LDS, written back the same way.
 template <typename V, typename F, int DIR>
-__device__ __forceinline__ void body_columns(const ColArgs<V>& a, const V* x, V* y, int c0, V* lds_in) {
+__device__ __forceinline__ void body_columns(const ColArgs<V>& a, const V* x, V* y, int c0, V* lds_in, int b) {
   constexpr int P = F::kPpt;
   V* lds_all = as_lds(lds_in);
   const int N = a.n, lc = a.log2c, C = 1 << lc;
@@ -30,10 +30,18 @@ __device__ __forceinline__ void body_columns(const ColArgs<V>& a, const V* x, V*
   __syncthreads();
   const bool active = slot < ncols;
   F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N, a.threads, tid, a.pd, a.tw, active, ps);
+  const bool twiddle = a.tw_hi != nullptr;
+  const int lo_mask = (1 << a.lo_bits) - 1;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
     const int k = i >> lc, c = i & (C - 1);
-    if (c < ncols)
-      gstore(y + (long long)k * a.ostride + c0 + c, cscale(lds_all[c * stride + k + (k >> ps)], a.scale));
+    if (c < ncols) {
+      V v = lds_all[c * stride + k + (k >> ps)];
+      if (twiddle) {
+        const int m = b * k;
+        v = twmul<DIR>(v, cmul(a.tw_hi[m >> a.lo_bits], a.tw_lo[m & lo_mask]));
+      }
+      gstore(y + (long long)k * a.ostride + c0 + c, cscale(v, a.scale));
+    }
   }
 }
 
@@ -42,9 +50,12 @@ __global__ __launch_bounds__(1024) void k_columns(ColArgs<V> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int groups = (a.ncols + (1 << a.log2c) - 1) >> a.log2c;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const long long p = lb / groups;
-  const int c0 = (int)(lb - p * groups) << a.log2c;
-  body_columns<V, F, DIR>(a, a.in + p * a.ipdist, a.out + p * a.opdist, c0, reinterpret_cast<V*>(smem_raw));
+  const long long q = lb / groups;
+  const int c0 = (int)(lb - q * groups) << a.log2c;
+  const long long p = q / a.inner;
+  const int b = (int)(q - p * a.inner);
+  body_columns<V, F, DIR>(a, a.in + p * a.ipdist + b * a.ipdist2, a.out + p * a.opdist + b * a.opdist2, c0,
+                          reinterpret_cast<V*>(smem_raw), b);
 }
 
 template <typename V, typename F>

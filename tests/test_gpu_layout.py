@@ -27,6 +27,7 @@ def oracle_nd(orc, a, axes, inverse=False):
 ND_CASES = [
     ((64, 48), 1), ((1000, 24), 2), ((256, 4096), 1), ((7, 3), 3), ((20000, 3), 1), ((65536, 2), 1),
     ((16, 32, 64), 2), ((5, 7, 9), 1), ((3, 1, 4), 2), ((1000,), 3), ((1, 1), 2), ((12, 10007), 1),
+    ((8192, 64), 2), ((6000, 4, 5), 1),   # long first axes: two column passes (colfour)
 ]
 
 
@@ -62,6 +63,8 @@ def test_nd_methods_and_helpers(fy, orc, cuda):
     assert "rows" in d and "columns" in d, d
     d = fy.NdPlan((65536, 2), 1, fy.F32).describe()
     assert "gather" in d, d
+    d = fy.NdPlan((8192, 64), 2, fy.F32).describe()
+    assert "colfour" in d, d
     rng = np.random.default_rng(3)
     x = rng.standard_normal((2, 3, 40, 50)) + 1j * rng.standard_normal((2, 3, 40, 50))
     xd = torch.tensor(x, dtype=torch.complex64, device="cuda")
@@ -98,6 +101,8 @@ MANY_CASES = [
     (50, 4, 3, 200, 2, 150, "gather"),
     (6720, 2, 2, 1, 2, 1, "gather"),
     (600, 64, 64, 1, 64, 1, "columns"),
+    (8192, 32, 32, 1, 32, 1, "colfour"),    # long adjacent columns: two column passes
+    (6000, 20, 20, 1, 24, 1, "colfour"),    # 75 x 80, output pitch differs
 ]
 
 
