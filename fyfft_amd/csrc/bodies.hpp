@@ -200,6 +200,16 @@ __device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x,
                           a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
+// Convolutions whose compile-time transform runs 640 or more threads stage
+// their gathers (Rader permutation, Bluestein chirp) through LDS eight points
+// per thread at a time: as first-pass sources and last-pass sinks the
+// gathers kept every point's index and both factors live at once and spilled
+// (M = 20480: 168 B per lane; staged, none), 1.23 against 1.52 ms for 8192
+// transforms of N = 10007.  Smaller and runtime-pass forms read and write
+// straight from the passes (the extra LDS round trip measured 15-30 %
+// slower there).
+template <typename F> constexpr bool kStageConv = F::kThreads >= 640;
+
 // ------------------------------------------------------------ Rader
 // X[0] = x0 + sum_{n>=1} x[n];  X[g^i] = x0 + (a (*) b)[i],  a[j] = x[g^-j]
 // F: the I/O policy of the LDS transforms (GenericIo or FixedIo).
@@ -208,13 +218,58 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
                                            bool active) {
   V* lds = as_lds(lds_in);
   const V x0 = active ? x[0] : V{0, 0};
-  __syncthreads();  // every read of x[0] precedes the write of y[0] (in place)
-  F::template run<-1>(RaderSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<int>)a.perm_in},
+  if constexpr (!kStageConv<F>) {
+    __syncthreads();  // every read of x[0] precedes the write of y[0] (in place)
+    F::template run<-1>(RaderSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<int>)a.perm_in},
+                        KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
+                        lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+    F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
+                        RaderSink<V>{(gbl_ptr<V>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
+                        lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+    return;
+  }
+  // the permuted input gathered into LDS eight points per thread at a time;
+  // every read of x precedes the barrier, so the writes of y below may
+  // overwrite it (in place)
+  if (active) {
+    const int* perm = a.perm_in;
+    for (int j0 = tid; j0 < a.m; j0 += 8 * a.threads) {
+      V v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = j0 + q * a.threads;
+        if (j < a.m) v[q] = x[perm[j]];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = j0 + q * a.threads;
+        if (j < a.m) lds[j + (j >> a.ps)] = v[q];
+      }
+    }
+  }
+  __syncthreads();
+  F::template run<-1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
                       KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
                       lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                      RaderSink<V>{(gbl_ptr<V>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
-                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps}, LdsSink<V>{(lds_ptr<V>)lds, a.ps}, lds, a.m, a.threads,
+                      tid, a.pd, a.tw, active, a.ps);
+  // y[g^i] = (c[i] + x0) * scale, staged like the input
+  if (active) {
+    const int* perm = a.perm_out;
+    for (int i0 = tid; i0 < a.m; i0 += 8 * a.threads) {
+      V v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q * a.threads;
+        if (i < a.m) v[q] = cscale(cadd(lds[i + (i >> a.ps)], x0), a.scale);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int i = i0 + q * a.threads;
+        if (i < a.m) y[perm[i]] = v[q];
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------ Bluestein
@@ -223,12 +278,55 @@ template <typename V, typename F>
 __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
                                           bool active) {
   V* lds = as_lds(lds_in);
-  F::template run<-1>(BlueSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<V>)a.chirp, a.n},
+  if constexpr (!kStageConv<F>) {
+    F::template run<-1>(BlueSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<V>)a.chirp, a.n},
+                        KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, nullptr, V{0, 0}, 1},
+                        lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+    F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
+                        BlueSink<V>{(gbl_ptr<V>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
+                        lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+    return;
+  }
+  // x * chirp staged into LDS (zero past N)
+  if (active) {
+    const V* chirp = a.chirp;
+    for (int i0 = tid; i0 < a.m; i0 += 8 * a.threads) {
+      V v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = i0 + j * a.threads;
+        v[j] = i < a.n ? cmul(x[i], chirp[i]) : V{0, 0};
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int i = i0 + j * a.threads;
+        if (i < a.m) lds[i + (i >> a.ps)] = v[j];
+      }
+    }
+  }
+  __syncthreads();
+  F::template run<-1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
                       KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, nullptr, V{0, 0}, 1},
                       lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                      BlueSink<V>{(gbl_ptr<V>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
-                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
+  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps}, LdsSink<V>{(lds_ptr<V>)lds, a.ps}, lds, a.m, a.threads,
+                      tid, a.pd, a.tw, active, a.ps);
+  // y = result * chirp * scale for k < N, staged like the input
+  if (active) {
+    const V* chirp = a.chirp;
+    for (int k0 = tid; k0 < a.n; k0 += 8 * a.threads) {
+      V v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + j * a.threads;
+        if (k < a.n) v[j] = cmul(lds[k + (k >> a.ps)], chirp[k]);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + j * a.threads;
+        if (k < a.n) y[k] = cscale(v[j], a.scale);
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------ four-step
