@@ -190,9 +190,11 @@ static fyfft_status make_pass_table(fyfft_plan* p, size_t n, dev::PassDesc* pd, 
 }
 
 // Plan an LDS-resident smooth transform of length n.
-// short_threads: lds_geometry's option for batched plans (not convolutions).
-static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_fixed, bool short_threads) {
-  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &L->geom, short_threads))
+// short_threads, small_radix: lds_geometry's options for batched plans (not
+// convolutions; small_radix for the C2C kernel only).
+static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_fixed, bool short_threads,
+                             bool small_radix = false) {
+  if (!lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(p->device), &L->geom, short_threads, small_radix))
     return FYFFT_UNSUPPORTED;
   void* tw = nullptr;
   fyfft_status st = make_pass_table(p, n, &L->pd, &tw);
@@ -976,7 +978,7 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     LdsGeom g;
     if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
       p->kind = kLds;
-      st = form == FYFFT_FORM_TUNED ? plan_lds_tuned(p, n) : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO, true);
+      st = form == FYFFT_FORM_TUNED ? plan_lds_tuned(p, n) : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO, true, true);
       if (!st) {
         const LdsGeom& q = p->lds.geom;
         p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") +
@@ -1470,6 +1472,9 @@ static fyfft_status create_real_impl(fyfft_plan** out, size_t n, size_t batch, f
   if (!st) {
     p->subs.push_back(sp);
     if (fused && sp->kind != kLds) st = FYFFT_UNSUPPORTED;
+    // the real kernels have no lean small-radix form: default geometry
+    if (!st && fused && sp->lds.geom.small_radix && !lds_geometry(len, elem, cap, &sp->lds.geom, true, false))
+      st = FYFFT_UNSUPPORTED;
   }
   if (!st && half) {
     st = p->alloc((h + 1) * elem, &p->rtw);

@@ -372,6 +372,9 @@ __device__ __forceinline__ unsigned long long pack_radices(const PassDesc& pd) {
 __device__ __forceinline__ int radix_at(unsigned long long code, int p) { return (int)((code >> (4 * p)) & 15) + 1; }
 
 __host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+// FP32 runtime-pass kernels bounded at 8 points per thread serve pass lists
+// without radix 16 only (the planner's small-radix geometry).
+template <typename V, int PPT> constexpr bool kNo16 = sizeof(V) == 8 && PPT <= 8;
 
 // ---------------------------------------------------------------- pass
 // One Stockham pass of radix R over the transform of length N held by T
@@ -470,7 +473,10 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
       case 7: lds_pass_ool<V, 7, cdiv(PPT, 7), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
       case 8: lds_pass_ool<V, 8, cdiv(PPT, 8), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
       case 9: lds_pass_ool<V, 9, cdiv(PPT, 9), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      default: lds_pass_ool<V, 16, cdiv(PPT, 16), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      default:
+        if constexpr (kNo16<V, PPT>) lds_pass_ool<V, 8, cdiv(PPT, 8), DIR>(lds, N, Ls, T, tid, tw, active, ps);
+        else lds_pass_ool<V, 16, cdiv(PPT, 16), DIR>(lds, N, Ls, T, tid, tw, active, ps);
+        break;
     }
     Ls *= R;
   }
@@ -573,7 +579,9 @@ __device__ __forceinline__ void fixed_io_passes(Src src, Dst dst, V* lds, int ps
 }
 
 template <int R> struct RadixC { static constexpr int value = R; };
-template <typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
+// NO16: the pass list has no radix 16 (the caller guarantees it), so that
+// case is not instantiated and the registers are sized for radix 9 at most.
+template <bool NO16 = false, typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
   switch (R) {
     case 2: f(RadixC<2>{}); break;
     case 3: f(RadixC<3>{}); break;
@@ -582,7 +590,10 @@ template <typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
     case 7: f(RadixC<7>{}); break;
     case 8: f(RadixC<8>{}); break;
     case 9: f(RadixC<9>{}); break;
-    default: f(RadixC<16>{}); break;
+    default:
+      if constexpr (NO16) f(RadixC<8>{});
+      else f(RadixC<16>{});
+      break;
   }
 }
 
@@ -600,7 +611,7 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   const int P = pd.npass;
   if (P <= 1) {
     if (P == 1)
-      radix_switch(radix_at(pack_radices(pd), 0), [&](auto rc) {
+      radix_switch<kNo16<V, PPT>>(radix_at(pack_radices(pd), 0), [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, 1, T, tid, tw, active);
       });
@@ -615,7 +626,7 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   const unsigned long long code = pack_radices(pd);
   int Ls = 1;
   for (int p = 0; p < P; ++p) {
-    radix_switch(radix_at(code, p), [&](auto rc) {
+    radix_switch<kNo16<V, PPT>>(radix_at(code, p), [&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (p == 0) io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, Ls, T, tid, tw, active);
       else if (p == P - 1) io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, dst, N, Ls, T, tid, tw, active);

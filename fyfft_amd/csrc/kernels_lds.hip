@@ -237,6 +237,7 @@ hipError_t launch_lds(int prec, int dir, const LdsLaunch& L, const void* in, voi
   if (prec == kF64)
     return L.geom.ppt <= 8 ? launch_generic_t<double2, 8>(dir, L, in, out, batch, dist, scale, s)
                            : launch_generic_t<double2, 16>(dir, L, in, out, batch, dist, scale, s);
+  if (L.geom.small_radix) return launch_generic_t<float2, 8>(dir, L, in, out, batch, dist, scale, s);
   return L.geom.ppt <= 16 ? launch_generic_t<float2, 16>(dir, L, in, out, batch, dist, scale, s)
                           : launch_generic_t<float2, 32>(dir, L, in, out, batch, dist, scale, s);
 }

@@ -91,7 +91,7 @@ std::vector<int> radices_for(size_t n) {
 
 size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap) { return lds_cap / elem_bytes; }
 
-bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads) {
+bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads, bool small_radix) {
   if (n < 1 || n > (size_t)1 << 20) return false;
   LdsGeom r;
   r.n = (int)n;
@@ -109,6 +109,14 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool 
   if (short_threads && elem_bytes < 16 && r.threads < 32) {
     r.ppt = 8;
     r.threads = (int)((n + 7) / 8);
+  }
+  if (small_radix && elem_bytes < 16 && n >= 128 && n < 2048) {
+    const std::vector<int> rad = radices_for(n);
+    if (std::find(rad.begin(), rad.end(), 16) == rad.end()) {
+      r.ppt = 8;
+      r.threads = (int)((n + 7) / 8);
+      r.small_radix = true;
+    }
   }
   if (r.threads > 1024 || r.ppt > 2 * ppt_lo) return false;
   r.pad_shift = kLdsPadShift;

@@ -38,14 +38,20 @@ struct LdsGeom {
   int per_block = 1; // transforms per workgroup (TPB)
   int pad_shift = 4; // LDS index i -> i + (i >> pad_shift)
   size_t lds_bytes = 0;
+  bool small_radix = false;  // FP32, no radix-16 pass: the leaner 8-points-per-thread kernel
 };
 // Fits n (elements of `elem_bytes`) in one workgroup's LDS of `lds_cap`
 // bytes; returns false if it does not fit even unpadded.  short_threads
 // (batched LDS plans): FP32 lengths below 512 take n/8 threads instead of
 // n/16 (with n/16, fewer than 32, passes of few butterflies leave most lanes
 // waiting; measured 1.5-3x slower).  Four-step sub-transforms keep n/16,
-// which their compile-time forms assume.
-bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads = false);
+// which their compile-time forms assume.  small_radix (batched FP32 C2C
+// plans, 128 <= n < 2048, no radix-16 pass): n/8 threads and the leaner
+// runtime-pass kernel sized for radix 9 at most (81 VGPRs against 116;
+// measured 4-10 % faster for 168...1000, slower for longer lengths and for
+// the real-transform kernels, which keep the default).
+bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads = false,
+                  bool small_radix = false);
 size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded
 
 // Four-step split n = n1 * n2 with both factors smooth and LDS-resident

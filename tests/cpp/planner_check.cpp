@@ -34,13 +34,17 @@ static void check_length(size_t n, const std::vector<char>& composite) {
     }
     CHECK(prod == n);
     for (size_t elem : {8u, 16u}) {
-      for (bool short_threads : {false, true}) {
+      for (int opt = 0; opt < 4; ++opt) {
         LdsGeom g;
-        if (lds_geometry(n, elem, 163840, &g, short_threads)) {
+        if (lds_geometry(n, elem, 163840, &g, (opt & 1) != 0, (opt & 2) != 0)) {
           CHECK(g.threads >= 1 && g.threads <= 1024 && g.per_block >= 1);
           CHECK((size_t)g.ppt * g.threads >= n);
           CHECK(g.lds_bytes <= 163840);
           CHECK((size_t)g.threads * g.per_block <= 1024);
+          if (g.small_radix) {  // the lean kernel: no radix-16 pass, at most 8 points per thread
+            for (int x : r) CHECK(x != 16);
+            CHECK((size_t)g.threads * 8 >= n && elem == 8);
+          }
         }
       }
       size_t a = 0, b = 0;

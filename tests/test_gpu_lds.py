@@ -66,7 +66,8 @@ def test_forward_inverse_vs_oracle(fy, orc, cuda, n, prec):
 
 
 @pytest.mark.parametrize("n,batch", [(4096, 257), (1024, 1029), (64, 1001), (6720, 5), (2, 7),
-                                     (3, 513), (8, 1001), (25, 70), (49, 300)])  # short: staged copies
+                                     (3, 513), (8, 1001), (25, 70), (49, 300),  # short: staged copies
+                                     (600, 77), (1000, 33)])  # lean small-radix kernel
 def test_batch_and_inplace(fy, orc, cuda, n, batch):
     import torch
     rng = np.random.default_rng(batch)
