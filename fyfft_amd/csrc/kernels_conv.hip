@@ -105,6 +105,17 @@ static const ConvFixed kConvFixed[] = {
     CFIXED(kF32, float2, 10240, 640, 1, 4, 4, 16, 16, 8, 5),
     CFIXED(kF32, float2, 16384, 1024, 1, 4, 4, 16, 16, 8, 8),
     CFIXED(kF32, float2, 20480, 1024, 1, 30, 4, 16, 16, 16, 5),
+    // FP64: the runtime-pass convolution kernel keeps call frames in scratch
+    CFIXED(kF64, double2, 256, 32, 8, 4, 2, 16, 16),
+    CFIXED(kF64, double2, 512, 64, 4, 4, 3, 8, 8, 8),
+    CFIXED(kF64, double2, 1024, 128, 2, 4, 3, 16, 8, 8),
+    CFIXED(kF64, double2, 1280, 160, 1, 4, 3, 16, 16, 5),
+    CFIXED(kF64, double2, 2048, 256, 1, 4, 3, 16, 16, 8),
+    CFIXED(kF64, double2, 2560, 320, 1, 4, 4, 8, 8, 8, 5),
+    CFIXED(kF64, double2, 4096, 512, 1, 4, 3, 16, 16, 16),
+    CFIXED(kF64, double2, 5120, 640, 1, 4, 4, 16, 8, 8, 5),
+    CFIXED(kF64, double2, 8192, 1024, 1, 4, 4, 16, 8, 8, 8),
+    CFIXED(kF64, double2, 10240, 1024, 1, 30, 4, 16, 16, 8, 5),
 };
 #undef CFIXED
 
