@@ -95,6 +95,7 @@ MANY_CASES = [
     (4096, 6, 1, 4100, 1, 4100, "rows"),
     (65536, 3, 1, 70000, 1, 70000, "rows"),
     (10007, 3, 1, 10010, 1, 10010, "rows"),
+    (49, 300, 1, 52, 1, 52, "rows"),        # short length, rows apart: not the staged block copy
     (1000, 5, 1, 1024, 1, 1030, "gather"),
     (100, 7, 7, 1, 7, 1, "columns"),
     (256, 40, 40, 1, 48, 1, "columns"),
