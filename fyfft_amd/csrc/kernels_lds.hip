@@ -197,12 +197,15 @@ static const FixedEntry kFixed[] = {
     // odd-radix lengths of the reference's table, one butterfly of the
     // largest radix per thread and pass, unpadded (odd strides spread over
     // the banks): 56-67 % of 8 TB/s against 34-51 % for the runtime-pass
-    // kernel; half the threads measured slower, and 81 gains nothing
-    FIXED(kF32, float2, 125, 25, 10, 30, 3, 5, 5, 5),
-    FIXED(kF32, float2, 243, 81, 3, 30, 3, 9, 9, 3),
-    FIXED(kF32, float2, 343, 49, 5, 30, 3, 7, 7, 7),
-    FIXED(kF32, float2, 625, 125, 2, 30, 4, 5, 5, 5, 5),
-    FIXED(kF32, float2, 729, 81, 3, 30, 3, 9, 9, 9),
+    // kernel; half the threads measured slower.  Up to 729 the staged block
+    // copy is faster again (odd-length rows straddle cache lines when each
+    // pass reads its own: N = 343 0.211 -> 0.183 ms, FP64 243 0.225 -> 0.183);
+    // from 2187 on (one transform per workgroup) it measured slower
+    SFIXED(kF32, float2, 125, 25, 10, 3, 5, 5, 5),
+    SFIXED(kF32, float2, 243, 81, 3, 3, 9, 9, 3),
+    SFIXED(kF32, float2, 343, 49, 5, 3, 7, 7, 7),
+    SFIXED(kF32, float2, 625, 125, 2, 4, 5, 5, 5, 5),
+    SFIXED(kF32, float2, 729, 81, 3, 3, 9, 9, 9),
     FIXED(kF32, float2, 2187, 243, 1, 30, 4, 9, 9, 9, 3),
     FIXED(kF32, float2, 2401, 343, 1, 30, 4, 7, 7, 7, 7),
     FIXED(kF32, float2, 3125, 625, 1, 30, 5, 5, 5, 5, 5, 5),
@@ -251,13 +254,13 @@ static const FixedEntry kFixed[] = {
     // FP64 forms of the same lengths and 256 / 512 / 2048 / 8192 (the
     // runtime-pass FP64 kernel keeps call frames in scratch): 1.4-2.9x
     // faster, 52-74 % of 8 TB/s
-    FIXED(kF64, double2, 125, 25, 10, 30, 3, 5, 5, 5),
-    FIXED(kF64, double2, 243, 27, 9, 30, 3, 9, 9, 3),
+    SFIXED(kF64, double2, 125, 25, 10, 3, 5, 5, 5),
+    SFIXED(kF64, double2, 243, 27, 9, 3, 9, 9, 3),
     FIXED(kF64, double2, 256, 32, 8, 30, 2, 16, 16),
     FIXED(kF64, double2, 512, 64, 4, 30, 3, 8, 8, 8),
-    FIXED(kF64, double2, 343, 49, 5, 30, 3, 7, 7, 7),
-    FIXED(kF64, double2, 625, 125, 2, 30, 4, 5, 5, 5, 5),
-    FIXED(kF64, double2, 729, 81, 3, 30, 3, 9, 9, 9),
+    SFIXED(kF64, double2, 343, 49, 5, 3, 7, 7, 7),
+    SFIXED(kF64, double2, 625, 125, 2, 4, 5, 5, 5, 5),
+    SFIXED(kF64, double2, 729, 81, 3, 3, 9, 9, 9),
     FIXED(kF64, double2, 2048, 256, 1, 30, 3, 16, 16, 8),
     FIXED(kF64, double2, 2187, 243, 1, 30, 4, 9, 9, 9, 3),
     FIXED(kF64, double2, 2401, 343, 1, 30, 4, 7, 7, 7, 7),
