@@ -333,10 +333,19 @@ template <typename T> __device__ __forceinline__ T make_uniform(const T& v) {
 // cache lines are fetched into one L2 instead of eight.  It is a bijection
 // on [0, total) for any total.
 constexpr int kXcds = 8;
+// Each XCD walks its range from a different eighth of it, so the workgroups
+// that run at the same moment on different XCDs are not a power-of-two
+// number of transforms apart (measured 2-5 % faster for the four-step,
+// column and convolution passes: N = 65536 x 2048 0.858 -> 0.843 ms,
+// Bluestein N = 1000003 x 16 0.534 -> 0.506 ms).
 __device__ __forceinline__ int xcd_remap(int b, int total) {
   const int per = total / kXcds, rem = total % kXcds;
   const int x = b % kXcds, i = b / kXcds;
-  return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+  const int len = x < rem ? per + 1 : per;
+  const int start = x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per;
+  int j = i + (int)(((long long)len * x) / kXcds);
+  if (j >= len) j -= len;
+  return start + j;
 }
 
 // ---------------------------------------------------------------- address spaces
