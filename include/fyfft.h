@@ -229,7 +229,8 @@ fyfft_status fyfft_plan_strategy(const fyfft_plan* plan, int* strategy);
 fyfft_status fyfft_length_strategy(size_t n, int* strategy);
 
 /* Radix passes the planner chooses for a 2/3/5/7-smooth length held in one
- * workgroup's LDS, or for each factor of a four-step split (no GPU needed).
+ * workgroup's LDS, or for each factor of a four-step split (no GPU needed):
+ * each in {2,3,4,5,7,8,9,16}, threes paired as 9.
  * Writes up to `max` radices, sets *count; FYFFT_UNSUPPORTED if N is not
  * smooth.  The counterpart of the reference's split planner
  * (mix_radix.hpp:253-352, 577-598). */
