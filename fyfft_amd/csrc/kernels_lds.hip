@@ -266,7 +266,7 @@ static const FixedEntry kFixed[] = {
     FIXED(kF64, double2, 2401, 343, 1, 30, 4, 7, 7, 7, 7),
     FIXED(kF64, double2, 3125, 625, 1, 30, 5, 5, 5, 5, 5, 5),
     FIXED(kF64, double2, 6561, 729, 1, 30, 4, 9, 9, 9, 9),
-    FIXED(kF64, double2, 8192, 1024, 1, 30, 4, 16, 8, 8, 8),
+    FIXED(kF64, double2, 8192, 1024, 1, 4, 4, 16, 8, 8, 8),  // padded: 2.5 % faster
 };
 #undef FIXED
 #undef SFIXED
