@@ -202,7 +202,7 @@ static const FixedEntry kFixed[] = {
     // pass reads its own: N = 343 0.211 -> 0.183 ms, FP64 243 0.225 -> 0.183);
     // from 2187 on (one transform per workgroup) it measured slower
     SFIXED(kF32, float2, 125, 25, 10, 3, 5, 5, 5),
-    SFIXED(kF32, float2, 243, 81, 3, 3, 9, 9, 3),
+    SFIXED(kF32, float2, 243, 27, 9, 3, 9, 9, 3),
     SFIXED(kF32, float2, 343, 49, 5, 3, 7, 7, 7),
     SFIXED(kF32, float2, 625, 125, 2, 4, 5, 5, 5, 5),
     SFIXED(kF32, float2, 729, 81, 3, 3, 9, 9, 9),
@@ -233,7 +233,7 @@ static const FixedEntry kFixed[] = {
     SFIXED(kF64, double2, 7, 1, 128, 1, 7),
     SFIXED(kF64, double2, 8, 1, 128, 1, 8),
     SFIXED(kF64, double2, 9, 1, 128, 1, 9),
-    SFIXED(kF64, double2, 128, 8, 32, 2, 16, 8),
+    SFIXED(kF64, double2, 128, 16, 16, 2, 16, 8),
     SFIXED(kF32, float2, 16, 1, 256, 1, 16),
     SFIXED(kF32, float2, 25, 5, 51, 2, 5, 5),
     SFIXED(kF32, float2, 27, 3, 85, 2, 9, 3),
