@@ -383,7 +383,7 @@ __device__ __forceinline__ int radix_at(unsigned long long code, int p) { return
 __host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 // FP32 runtime-pass kernels bounded at 8 points per thread serve pass lists
 // without radix 16 only (the planner's small-radix geometry).
-template <typename V, int PPT> constexpr bool kNo16 = sizeof(V) == 8 && PPT <= 8;
+template <typename V, int PPT> constexpr bool kNo16 = (sizeof(V) == 8 && PPT <= 8) || (sizeof(V) == 16 && PPT <= 4);
 
 // ---------------------------------------------------------------- pass
 // One Stockham pass of radix R over the transform of length N held by T

@@ -330,6 +330,7 @@ hipError_t launch_lds(int prec, int dir, const LdsLaunch& L, const void* in, voi
   // distances run the runtime-pass kernel on the same geometry and table
   if (L.fixed >= 0 && (!kFixed[L.fixed].staged || dist == (size_t)L.geom.n))
     return kFixed[L.fixed].launch(dir, in, out, batch, dist, L.tw, scale, s);
+  if (prec == kF64 && L.geom.small_radix) return launch_generic_t<double2, 4>(dir, L, in, out, batch, dist, scale, s);
   if (prec == kF64)
     return L.geom.ppt <= 8 ? launch_generic_t<double2, 8>(dir, L, in, out, batch, dist, scale, s)
                            : launch_generic_t<double2, 16>(dir, L, in, out, batch, dist, scale, s);

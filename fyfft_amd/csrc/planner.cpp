@@ -110,11 +110,12 @@ bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool 
     r.ppt = 8;
     r.threads = (int)((n + 7) / 8);
   }
-  if (small_radix && elem_bytes < 16 && n >= 128 && n < 2048) {
+  const bool fp64 = elem_bytes >= 16;
+  if (small_radix && n >= (fp64 ? 64u : 128u) && n < (fp64 ? 400u : 2048u)) {
     const std::vector<int> rad = radices_for(n);
     if (std::find(rad.begin(), rad.end(), 16) == rad.end()) {
-      r.ppt = 8;
-      r.threads = (int)((n + 7) / 8);
+      r.ppt = fp64 ? 4 : 8;
+      r.threads = (int)((n + r.ppt - 1) / r.ppt);
       r.small_radix = true;
     }
   }

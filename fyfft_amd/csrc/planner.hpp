@@ -45,11 +45,13 @@ struct LdsGeom {
 // (batched LDS plans): FP32 lengths below 512 take n/8 threads instead of
 // n/16 (with n/16, fewer than 32, passes of few butterflies leave most lanes
 // waiting; measured 1.5-3x slower).  Four-step sub-transforms keep n/16,
-// which their compile-time forms assume.  small_radix (batched FP32 C2C
-// plans, 128 <= n < 2048, no radix-16 pass): n/8 threads and the leaner
-// runtime-pass kernel sized for radix 9 at most (81 VGPRs against 116;
-// measured 4-10 % faster for 168...1000, slower for longer lengths and for
-// the real-transform kernels, which keep the default).
+// which their compile-time forms assume.  small_radix (batched C2C plans
+// without a radix-16 pass; FP32 128 <= n < 2048, FP64 64 <= n < 400): n/8
+// (FP32) or n/4 (FP64) threads and the leaner runtime-pass kernel sized for
+// radix 9 at most (FP32 81 VGPRs against 116; FP64 116 VGPRs and no
+// call-frame scratch against 128 + 244 B): measured 4-10 % faster for FP32
+// 168...1000 and 15-34 % for FP64 100...200, slower for longer lengths and
+// for the real-transform kernels, which keep the default.
 bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads = false,
                   bool small_radix = false);
 size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded

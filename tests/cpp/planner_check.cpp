@@ -43,7 +43,7 @@ static void check_length(size_t n, const std::vector<char>& composite) {
           CHECK((size_t)g.threads * g.per_block <= 1024);
           if (g.small_radix) {  // the lean kernel: no radix-16 pass, at most 8 points per thread
             for (int x : r) CHECK(x != 16);
-            CHECK((size_t)g.threads * 8 >= n && elem == 8);
+            CHECK((size_t)g.threads * (elem == 8 ? 8 : 4) >= n);
           }
         }
       }
