@@ -34,9 +34,22 @@ def sources() -> list[str]:
                   if f.endswith((".hip", ".cpp")))
 
 
-def _compile(src: str) -> str:
+def _headers() -> list[str]:
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    inc = os.path.join(ROOT, "include")
+    return hdrs + [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith((".h", ".hpp"))]
+
+
+def _stale(src: str, obj: str, newest_header: float) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return t < os.path.getmtime(src) or t < newest_header
+
+
+def _compile(src: str, extra: tuple = ()) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *CFLAGS, "-x", "hip", "-c", src, "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *extra, "-x", "hip", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
@@ -47,13 +60,22 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None) -> str:
+def build(jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every source (incremental: only those older than their
+    object's inputs -- the source itself or any header) and link."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in srcs]
+    todo = srcs
+    if incremental:
+        newest = max(os.path.getmtime(h) for h in _headers())
+        todo = [s for s, o in zip(srcs, objs) if _stale(s, o, newest)]
+    jobs = jobs or min(max(1, len(todo)), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    if incremental and not todo and os.path.exists(LIB):
+        return LIB
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -77,5 +99,5 @@ def build_dropin_test() -> str:
 
 
 if __name__ == "__main__":
-    print(build())
+    print(build(incremental="--incremental" in sys.argv))
     print(build_dropin_test())

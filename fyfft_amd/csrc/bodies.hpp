@@ -56,7 +56,7 @@ template <typename V, int PPT> struct GenericFft {
 template <typename V, int D, int N, int T, int Ls, int R, int... Rest>
 __device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, bool active, int ps) {
   constexpr int MAXB = cdiv(N / R, T);
-  stockham_pass<V, R, MAXB, D, false, false>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
+  stockham_pass<V, R, MAXB, D, false, false, tw_all(Ls, R)>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
   if constexpr (sizeof...(Rest) > 0) fixed_lds_passes<V, D, N, T, Ls * R, Rest...>(lds, tid, tw, active, ps);
 }
 

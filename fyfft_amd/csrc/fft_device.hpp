@@ -49,9 +49,22 @@ template <int DIR, typename V> __device__ __forceinline__ V twmul(V a, V w) {
 template <int DIR, typename V> __device__ __forceinline__ V rot90(V a) {
   if constexpr (DIR < 0) return V{a.y, -a.x}; else return V{-a.y, a.x};
 }
-// a * (c + DIR*i*s)
+// a * (c + DIR*i*s) for constants c, s.  In FP32 each constant is split
+// into its rounded value and the rounding remainder (c = c_hi + c_lo), and
+// the remainder's product is added first: the codelets' rotations by
+// exp(i*pi*m/8) and exp(2*pi*i*m/9) act on most outputs of every pass, so
+// the constants' own rounding would otherwise add a correlated error of up
+// to 2e-8 per pass (N = 4096 forward: 1.36e-7 -> 1.19e-7 rel-L2 together
+// with the depth-one twiddle products of apply_twiddles).
 template <int DIR, typename V> __device__ __forceinline__ V rotc(V a, double c, double s) {
-  const S<V> cc = (S<V>)c, ss = (S<V>)(DIR < 0 ? -s : s);
+  const double sd = DIR < 0 ? -s : s;
+  const S<V> cc = (S<V>)c, ss = (S<V>)sd;
+  if constexpr (sizeof(S<V>) == 4) {
+    const float cl = (float)(c - (double)cc), sl = (float)(sd - (double)ss);
+    const float lre = a.x * cl - a.y * sl, lim = a.x * sl + a.y * cl;
+    return V{__builtin_fmaf(a.x, cc, __builtin_fmaf(-a.y, ss, lre)),
+             __builtin_fmaf(a.x, ss, __builtin_fmaf(a.y, cc, lim))};
+  }
   return V{a.x * cc - a.y * ss, a.x * ss + a.y * cc};
 }
 
@@ -275,41 +288,49 @@ template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
 
 // ---------------------------------------------------------------- twiddles
 // Multiply inputs r = 1..R-1 of one butterfly by w^(k*r), where t points at
-// the pass table slice [w^k, w^2k, ..., w^((R-1)k)].  For R = 4, 7, 8, 9 and
-// 16 only the power-of-two powers are loaded and the others are formed by at
-// most three complex products (a few ulp), which halves to quarters the
-// table traffic of the pass; R = 2, 3, 5 load every factor.
-template <int R, int DIR, typename V>
+// the pass table slice [w^k, w^2k, ..., w^((R-1)k)].  For R = 7, 8, 9 and 16
+// a subset of the powers is loaded and each other power is ONE product of
+// two loaded ones (R = 16: w^a * w^b with a in {1,2,3}, b in {4,8,12};
+// R = 9: a in {1,2}, b in {3,6}; R = 8: w^1 times {2,4,6}; R = 7: w^1 or
+// w^2 times w^4), so every factor carries at most three roundings; chains
+// of two or three products measurably raised the transform's error
+// (N = 4096 forward: 1.36e-7 rel-L2 with w^15 = ((w^1 w^2) w^4) w^8, 1.195e-7
+// with the depth-one products).  R = 2..5, and every radix when ALL is set,
+// load every factor: compile-time passes set ALL where the pass's table
+// slice holds at most 256 entries (it stays in the L1 cache, so the extra
+// loads are nearly free: N = 4096, all factors of pass 1, 1.17e-7).
+template <int R, int DIR, bool ALL = false, typename V>
 __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
-  if constexpr (R == 16) {
-    const V w1 = t[0], w2 = t[1], w4 = t[3], w8 = t[7];
-    const V w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
-    const V w[16] = {V{1, 0}, w1, w2, w3, w4, w5, w6, w7, w8, cmul(w1, w8), cmul(w2, w8),
-                     cmul(w3, w8), cmul(w4, w8), cmul(w5, w8), cmul(w6, w8), cmul(w7, w8)};
+  if constexpr (ALL || R <= 5) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], t[r - 1]);
+  } else if constexpr (R == 16) {
+    const V w1 = t[0], w2 = t[1], w3 = t[2], w4 = t[3], w8 = t[7], w12 = t[11];
+    const V w[16] = {V{1, 0}, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4), w8,
+                     cmul(w1, w8), cmul(w2, w8), cmul(w3, w8), w12, cmul(w1, w12), cmul(w2, w12),
+                     cmul(w3, w12)};
 #pragma unroll
     for (int r = 1; r < 16; ++r) v[r] = twmul<DIR>(v[r], w[r]);
   } else if constexpr (R == 9) {
-    const V w1 = t[0], w2 = t[1], w4 = t[3], w8 = t[7];
-    const V w3 = cmul(w1, w2);
-    const V w[9] = {V{1, 0}, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4), w8};
+    const V w1 = t[0], w2 = t[1], w3 = t[2], w6 = t[5];
+    const V w[9] = {V{1, 0}, w1, w2, w3, cmul(w1, w3), cmul(w2, w3), w6, cmul(w1, w6), cmul(w2, w6)};
 #pragma unroll
     for (int r = 1; r < 9; ++r) v[r] = twmul<DIR>(v[r], w[r]);
-  } else if constexpr (R == 8 || R == 7) {
+  } else if constexpr (R == 8) {
+    const V w1 = t[0], w2 = t[1], w4 = t[3], w6 = t[5];
+    const V w[8] = {V{1, 0}, w1, w2, cmul(w1, w2), w4, cmul(w1, w4), w6, cmul(w1, w6)};
+#pragma unroll
+    for (int r = 1; r < 8; ++r) v[r] = twmul<DIR>(v[r], w[r]);
+  } else {  // R == 7
     const V w1 = t[0], w2 = t[1], w4 = t[3];
-    const V w3 = cmul(w1, w2);
-    const V w[8] = {V{1, 0}, w1, w2, w3, w4, cmul(w1, w4), cmul(w2, w4), cmul(w3, w4)};
+    const V w[7] = {V{1, 0}, w1, w2, cmul(w1, w2), w4, cmul(w1, w4), cmul(w2, w4)};
 #pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], w[r]);
-  } else if constexpr (R == 4) {
-    const V w1 = t[0], w2 = t[1];
-    v[1] = twmul<DIR>(v[1], w1);
-    v[2] = twmul<DIR>(v[2], w2);
-    v[3] = twmul<DIR>(v[3], cmul(w1, w2));
-  } else {
-#pragma unroll
-    for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], t[r - 1]);
+    for (int r = 1; r < 7; ++r) v[r] = twmul<DIR>(v[r], w[r]);
   }
 }
+// Compile-time passes of span Ls and radix R load every twiddle factor when
+// the pass's table slice (Ls * (R - 1) entries) is small.
+__host__ __device__ constexpr bool tw_all(int Ls, int R) { return Ls > 1 && Ls * (R - 1) <= 256; }
 
 // ---------------------------------------------------------------- uniform values
 // Copy of a trivially copyable descriptor with every 32-bit word taken from
@@ -393,7 +414,7 @@ template <typename V, int PPT> constexpr bool kNo16 = (sizeof(V) == 8 && PPT <= 
 // place: every read completes (barrier) before any write.  A barrier
 // follows LDS writes so the next pass sees them.  `scale` is applied only
 // on global stores.
-template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G>
+template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G, bool TWALL = false>
 __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int Ls, int T, int tid,
                                               const V* __restrict__ tw, bool active, S<V> scale,
                                               int ps) {
@@ -419,7 +440,7 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
           const int idx = j + r * nb;
           v[b][r] = SRC_G ? gload(src + idx) : src[idx + (idx >> ps)];
         }
-        if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
+        if (Ls > 1) apply_twiddles<R, DIR, TWALL>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
       }
       if (Ls > 1) {
@@ -513,7 +534,7 @@ template <typename V> struct LdsSink {
   __device__ void operator()(int i, V v) const { ((V*)p)[i + (i >> ps)] = v; }
 };
 
-template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
+template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst, bool TWALL = false>
 __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, int tid,
                                         const V* __restrict__ tw, bool active) {
   const int nb = N / R;
@@ -533,7 +554,7 @@ __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, 
       if (active && j < nb) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[b][r] = src(j + r * nb);
-        if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
+        if (Ls > 1) apply_twiddles<R, DIR, TWALL>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
       }
       if (Ls > 1) {
@@ -580,10 +601,11 @@ __device__ __forceinline__ void fixed_io_passes(Src src, Dst dst, V* lds, int ps
   constexpr int MAXB = cdiv(N / R, T);
   const LdsSrc<V> ls{(lds_ptr<V>)lds, ps};
   const LdsSink<V> lk{(lds_ptr<V>)lds, ps};
-  if constexpr (first && last) io_pass<V, R, MAXB, DIR>(src, dst, N, Ls, T, tid, tw, active);
-  else if constexpr (first) io_pass<V, R, MAXB, DIR>(src, lk, N, Ls, T, tid, tw, active);
-  else if constexpr (last) io_pass<V, R, MAXB, DIR>(ls, dst, N, Ls, T, tid, tw, active);
-  else io_pass<V, R, MAXB, DIR>(ls, lk, N, Ls, T, tid, tw, active);
+  constexpr bool all = tw_all(Ls, R);
+  if constexpr (first && last) io_pass<V, R, MAXB, DIR, Src, Dst, all>(src, dst, N, Ls, T, tid, tw, active);
+  else if constexpr (first) io_pass<V, R, MAXB, DIR, Src, LdsSink<V>, all>(src, lk, N, Ls, T, tid, tw, active);
+  else if constexpr (last) io_pass<V, R, MAXB, DIR, LdsSrc<V>, Dst, all>(ls, dst, N, Ls, T, tid, tw, active);
+  else io_pass<V, R, MAXB, DIR, LdsSrc<V>, LdsSink<V>, all>(ls, lk, N, Ls, T, tid, tw, active);
   if constexpr (!last) fixed_io_passes<V, DIR, N, T, Ls * R, Src, Dst, Rest...>(src, dst, lds, ps, tid, tw, active);
 }
 

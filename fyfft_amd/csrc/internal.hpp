@@ -40,6 +40,22 @@ struct LdsLaunch {
   int fixed = -1;            // index of a compile-time specialised kernel, -1 = generic
 };
 
+// True if a runtime-pass kernel instantiated for at most `ppt` points per
+// thread (without its radix-16 case when no16) covers every pass of pd for a
+// transform of n points on `threads` threads: each pass of radix R then has
+// at most cdiv(ppt, R) butterflies per thread.  The launchers check it so a
+// geometry that disagrees with the chosen instantiation fails loudly instead
+// of leaving outputs unwritten.
+inline bool generic_fits(const dev::PassDesc& pd, int n, int threads, int ppt, bool no16) {
+  if (threads <= 0 || (long long)ppt * threads < (long long)n) return false;
+  long long prod = 1;
+  for (int p = 0; p < pd.npass; ++p) {
+    if (no16 && pd.radix[p] == 16) return false;
+    prod *= pd.radix[p];
+  }
+  return pd.npass == 0 || prod == n;
+}
+
 // --- kernels_lds.hip
 // Batched LDS-resident transform: `batch` transforms of length N, transform
 // b at in + b*dist (elements).  scale multiplies the output.

@@ -74,7 +74,7 @@ __device__ __forceinline__ void run_fixed(const V* gin, V* lds, V* gout, int tid
   constexpr int MAXB = cdiv(N / R, T);
   const V* src = first ? gin : lds;
   V* dst = last ? gout : lds;
-  stockham_pass<V, R, MAXB, DIR, first, last>(src, dst, N, Ls, T, tid, tw, active, scale, PS);
+  stockham_pass<V, R, MAXB, DIR, first, last, tw_all(Ls, R)>(src, dst, N, Ls, T, tid, tw, active, scale, PS);
   if constexpr (!last) run_fixed<V, DIR, N, T, PS, Ls * R, Rest...>(gin, lds, gout, tid, tw, active, scale);
 }
 
@@ -285,6 +285,9 @@ int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
     g->threads = e.threads;
     g->per_block = e.per_block;
     g->ppt = cdiv(e.n, e.threads);  // the generic kernels reuse this geometry
+    // the kernel's own geometry replaces the planner's: its points per thread
+    // are not those of the lean small-radix instantiation
+    g->small_radix = false;
     g->pad_shift = e.pad_shift;
     g->lds_bytes = (size_t)lds_len(e.n, e.pad_shift) * e.per_block * elem_bytes(prec);
     return i;
@@ -297,6 +300,7 @@ template <typename V, int PPT>
 static hipError_t launch_generic_t(int dir, const LdsLaunch& L, const void* in, void* out,
                                    size_t batch, size_t dist, double scale, hipStream_t s) {
   const LdsGeom& g = L.geom;
+  if (!generic_fits(L.pd, g.n, g.threads, PPT, kNo16<V, PPT>)) return hipErrorInvalidValue;
   // staged copies for transforms of 3 or more points and fewer than 8
   // threads stored back to back (measured: 2.5-6x faster for N = 3..32;
   // slower from 8 threads on, where each thread's runs already fill cache

@@ -107,11 +107,16 @@ def test_bluestein_lds(fy, orc, cuda, n, prec):
 
 # one prime per compile-time convolution length (M in brackets): 256, 512,
 # 1024, 1280, 2048, 2560, 4096, 5120, 8192, 10240, 16384, 20480
+# FP64 has compile-time convolutions up to M = 10240 (N = 4099); from
+# M = 5120 on they stage the gather through LDS
 @pytest.mark.parametrize("n", [103, 223, 419, 521, 821, 1031, 1657, 2053, 3299, 4099, 6563, 8209])
-def test_bluestein_lds_fixed_lengths(fy, orc, cuda, n):
-    with fy.Plan(n, 5, fy.F32) as p:
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_bluestein_lds_fixed_lengths(fy, orc, cuda, n, prec):
+    if prec == "f64" and n > 4099:
+        pytest.skip("FP64 compile-time convolutions stop at M = 10240")
+    with fy.Plan(n, 5, fy.F32 if prec == "f32" else fy.F64) as p:
         assert "fixed" in p.describe(), p.describe()
-    x = unit_phase(np.random.default_rng(n), 5, n, "f32")
+    x = unit_phase(np.random.default_rng(n), 5, n, prec)
     check(fy, orc, x, "bluestein-lds")
     check(fy, orc, x, "bluestein-lds", inplace=True)
 

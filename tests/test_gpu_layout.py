@@ -104,6 +104,13 @@ MANY_CASES = [
     (600, 64, 64, 1, 64, 1, "columns"),
     (8192, 32, 32, 1, 32, 1, "colfour"),    # long adjacent columns: two column passes
     (6000, 20, 20, 1, 24, 1, "colfour"),    # 75 x 80, output pitch differs
+    # rows apart from lengths whose batched plan takes a staged compile-time
+    # kernel (243) or the lean small-radix kernel (105): the runtime-pass
+    # kernel must then run on the compile-time kernel's geometry
+    (243, 5, 1, 256, 1, 256, "rows"),
+    (105, 9, 1, 112, 1, 112, "rows"),
+    (360, 6, 1, 368, 1, 368, "rows"),
+    (729, 4, 1, 800, 1, 800, "rows"),
 ]
 
 

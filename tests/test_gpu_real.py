@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 CASES = [
     # (n, rows, form)
     (2, 7, "fused full"), (4, 5, "fused half"), (16, 9, "fused half"), (1000, 6, "fused half"),
-    (4096, 4, "fused half"), (6720, 3, "fused half"), (20000, 2, "fused half"),
+    (1024, 5, "fused half"), (4096, 4, "fused half"), (6720, 3, "fused half"), (20000, 2, "fused half"),
     (1, 3, "full-length transform"), (3, 5, "fused full"), (15, 4, "fused full"), (243, 3, "fused full"),
     (1125, 3, "fused full"), (16807, 2, "fused full"),
     (65536, 3, "half-length transform"), (1 << 21, 1, "half-length transform"),

@@ -199,3 +199,20 @@ def test_checks_catch_planted_mistakes(orc):
     # bit-reversed output order
     rev = np.array([int(format(i, "06b")[::-1], 2) for i in range(n)])
     assert orc.rel_l2(ref[rev], ref) > 0.1
+
+
+def test_reference_input_stream_pinned(orc):
+    """The regenerated example.cpp stream reproduces the reference's N = 2
+    figure exactly: a 2-point transform is one rounded add and subtract in
+    any implementation, so only the input values decide its error (3.6e-8
+    in golden/reference_facts.json)."""
+    import json
+    from tests.refinputs import reference_inputs
+    facts = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_facts.json")))
+    x = reference_inputs(2)
+    y = np.array([x[0] + x[1], x[0] - x[1]], np.complex64)
+    err = orc.rel_l2(y, orc.dft_naive(x.astype(np.complex128)))
+    assert abs(err - facts["fp32_fwd_rel_l2"]["2"]) < 0.05e-8, err
+    # first values of std::mt19937(0) (known-answer): 2357136044, 2546248239
+    raw = np.random.RandomState(0)._bit_generator.random_raw(2)
+    assert list(raw) == [2357136044, 2546248239]
