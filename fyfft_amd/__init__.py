@@ -90,6 +90,7 @@ def _load() -> ctypes.CDLL:
         L.fyfft_plan_destroy.argtypes = [vp]
         L.fyfft_execute.argtypes = [vp, vp, vp, i, vp]
         L.fyfft_execute_host.argtypes = [vp, vp, vp, i]
+        L.fyfft_execute_noalloc.argtypes = [vp, vp, vp, i, vp]
         L.fyfft_plan_strategy.argtypes = [vp, ip]
         L.fyfft_length_strategy.argtypes = [sz, ip]
         L.fyfft_length_radices.argtypes = [sz, ip, i, ip]
@@ -114,7 +115,7 @@ def _load() -> ctypes.CDLL:
         L.fyfft_status_string.restype = ctypes.c_char_p
         L.fyfft_last_error.argtypes = []
         L.fyfft_last_error.restype = ctypes.c_char_p
-        for name in ("fyfft_plan_create", "fyfft_plan_destroy", "fyfft_execute",
+        for name in ("fyfft_plan_create", "fyfft_plan_destroy", "fyfft_execute", "fyfft_execute_noalloc",
                      "fyfft_execute_host", "fyfft_plan_strategy", "fyfft_length_strategy",
                      "fyfft_length_radices", "fyfft_plan_workspace_bytes", "fyfft_plan_describe",
                      "fyfft_rfft_modes", "fyfft_irfft_modes", "fyfft_rfft_modes_ex", "fyfft_irfft_modes_ex",

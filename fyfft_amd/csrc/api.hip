@@ -733,16 +733,36 @@ static void fill_pass(FsArgs<V>& a, const LdsGeom& g, size_t len, const dev::Pas
   a.tw = (const V*)tw;
 }
 
+// Device memory a multi-pass execution runs through: `chunk` transforms'
+// worth of convolution-length workspace (tmp), Rader x[0] slots and the
+// bucketed-permutation staging (zw).  The plan's own, or carved from the
+// library's static scratch by fyfft_execute_noalloc.
+struct Workspace {
+  void* tmp = nullptr;
+  void* x0buf = nullptr;
+  void* zw = nullptr;
+  size_t chunk = 0;
+};
+
+static Workspace plan_workspace(const fyfft_plan* p) {
+  Workspace w;
+  w.tmp = p->tmp;
+  w.x0buf = p->x0buf;
+  w.zw = p->zw;
+  w.chunk = p->chunk;
+  return w;
+}
+
 template <typename V>
 static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int dir, bool inv_tables,
-                                  double scale, size_t rows, size_t dist, hipStream_t s) {
+                                  double scale, size_t rows, size_t dist, const Workspace& ws, hipStream_t s) {
   const size_t n = p->n, L = (p->kind == kFourStep) ? n : p->m;
-  for (size_t b0 = 0; b0 < rows; b0 += p->chunk) {
-    const size_t cb = std::min(p->chunk, rows - b0);
+  for (size_t b0 = 0; b0 < rows; b0 += ws.chunk) {
+    const size_t cb = std::min(ws.chunk, rows - b0);
     FsArgs<V> a;
     a.in = in + b0 * dist;
     a.out = out + b0 * dist;
-    a.tmp = (V*)p->tmp;
+    a.tmp = (V*)ws.tmp;
     a.in_dist = (long long)dist;
     a.out_dist = (long long)dist;
     a.tmp_dist = (long long)L;
@@ -756,7 +776,7 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     const int slot = inv_tables ? 1 : 0;
     a.kern = (const V*)p->kern[slot];
     a.chirp = (const V*)p->chirp[slot];
-    a.x0buf = (V*)p->x0buf;
+    a.x0buf = (V*)ws.x0buf;
     int colA, colC = -1, rowB;
     // Rader: the input permutation as an element gather in the column pass
     // (measured faster than gathering it through chunked runs as well), the
@@ -769,7 +789,7 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
       colC = buckets ? kFsRaderC2 : kFsRaderC;
     } else { colA = kFsBlueA; rowB = kFsConvB; colC = kFsBlueC; }
     if (buckets) {
-      a.zw = (V*)p->zw;
+      a.zw = (V*)ws.zw;
       a.zw_dist = (long long)L;
     }
 
@@ -785,7 +805,7 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
       HIP_TRY(launch_fs_pass(p->prec, true, colC, +1, &a, cb, p->gcol, p->fixed_col, s));
     }
     if (buckets)
-      HIP_TRY(launch_rader_bucket(p->prec, p->zw, a.out, (long long)cb, (long long)L, p->rb_q, p->rb_out,
+      HIP_TRY(launch_rader_bucket(p->prec, ws.zw, a.out, (long long)cb, (long long)L, p->rb_q, p->rb_out,
                                   (long long)L, (long long)dist, s));
   }
   return FYFFT_SUCCESS;
@@ -1314,7 +1334,7 @@ static fyfft_status exec_strided(const fyfft_plan* p, const void* in, void* out,
 // dist: elements between consecutive transforms (0: the length n).  Only the
 // plain batched kinds accept dist != n.
 static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out, int direction, size_t rows,
-                                 hipStream_t s, size_t dist = 0) {
+                                 hipStream_t s, size_t dist = 0, const Workspace* wsp = nullptr) {
   if (dist == 0) dist = p->n;
   if (dist != p->n && p->kind != kTrivial && p->kind != kLds && p->kind != kFourStep && p->kind != kRaderFs &&
       p->kind != kBlueFs && p->kind != kRaderLds && p->kind != kBlueLds)
@@ -1339,9 +1359,10 @@ static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out,
     case kRaderFs:
     case kBlueFs: {
       const int d = dir == kForward ? -1 : +1;
+      const Workspace ws = wsp ? *wsp : plan_workspace(p);
       if (p->prec == kF64)
-        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, rows, dist, s);
-      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, rows, dist, s);
+        return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, rows, dist, ws, s);
+      return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, rows, dist, ws, s);
     }
     case kRaderLds:
     case kBlueLds:
@@ -1825,6 +1846,68 @@ static fyfft_status create_nd_impl(fyfft_plan** out, int rank, const size_t* dim
   return FYFFT_SUCCESS;
 }
 
+// ------------------------------------------------------------ no-allocation execution
+// Workspace of fyfft_execute_noalloc: a file-scope device array, one copy per
+// GPU at its own address, held from module load until the process ends.  In
+// host code the array's name is only a stand-in for the symbol: its device
+// address is taken once per device with hipGetSymbolAddress (that device
+// current) and only that address is handed to kernels.  Every call on one
+// GPU shares the array, so calls that overlap in time on one GPU race.
+constexpr size_t kScratchBytes = FYFFT_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kScratchDevices] = {};
+
+static fyfft_status scratch_of_current_device(int dev, void** ptr) {
+  if (dev < 0 || dev >= kScratchDevices) {
+    g_last_error = "fyfft_execute_noalloc: device index beyond the supported 64";
+    return FYFFT_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return from_hip(e, "hipGetSymbolAddress(scratch)");
+    if (!a) {
+      g_last_error = "hipGetSymbolAddress(scratch) returned a null address";
+      return FYFFT_DEVICE_ERROR;
+    }
+    g_scratch_ptr[dev] = a;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return FYFFT_SUCCESS;
+}
+
+// Carves a four-step plan's workspace out of the scratch array: as many
+// transforms per chunk as the array holds (tmp and, for bucketed Rader
+// plans, zw: one convolution length each per transform; Rader x[0] slots),
+// each region 256-byte aligned.  None of it needs clearing: every pass
+// writes what the next one reads.
+static fyfft_status scratch_workspace(const fyfft_plan* p, void* base, Workspace* w) {
+  const size_t elem = elem_bytes(p->prec), L = p->kind == kFourStep ? p->n : p->m;
+  const bool x0 = p->kind == kRaderFs, zw = p->kind == kRaderFs && p->rb_q > 0;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  auto need = [&](size_t c) { return up(c * L * elem) + (zw ? up(c * L * elem) : 0) + (x0 ? up(c * elem) : 0); };
+  const size_t per = L * elem * (zw ? 2 : 1) + (x0 ? elem : 0);
+  size_t chunk = std::min(p->batch, kScratchBytes / per);
+  while (chunk > 0 && need(chunk) > kScratchBytes) --chunk;
+  if (chunk == 0) {
+    g_last_error = "fyfft_execute_noalloc: one transform needs " + std::to_string(need(1)) +
+                   " bytes of workspace, more than the " + std::to_string(kScratchBytes) +
+                   "-byte static scratch (use fyfft_execute)";
+    return FYFFT_INVALID_LENGTH;
+  }
+  char* b = (char*)base;
+  w->chunk = chunk;
+  w->tmp = b;
+  b += up(chunk * L * elem);
+  w->zw = zw ? b : nullptr;
+  if (zw) b += up(chunk * L * elem);
+  w->x0buf = x0 ? b : nullptr;
+  return FYFFT_SUCCESS;
+}
+
 extern "C" {
 
 const char* fyfft_status_string(fyfft_status s) {
@@ -1972,6 +2055,40 @@ fyfft_status fyfft_plan_destroy(fyfft_plan* p) {
 fyfft_status fyfft_execute(const fyfft_plan* p, const void* in, void* out, int direction, void* stream) {
   if (!valid(p)) return FYFFT_INVALID_PLAN;
   return execute_rows(p, in, out, direction, p->batch, (hipStream_t)stream);
+}
+
+fyfft_status fyfft_execute_noalloc(const fyfft_plan* p, const void* in, void* out, int direction, void* stream) {
+  if (!valid(p)) return FYFFT_INVALID_PLAN;
+  const hipStream_t s = (hipStream_t)stream;
+  switch (p->kind) {
+    case kTrivial:
+    case kLds:
+    case kRaderLds:
+    case kBlueLds:
+    case kMfma:
+    case kLowp:  // single-kernel plans: no workspace at all
+      return execute_rows(p, in, out, direction, p->batch, s);
+    case kFourStep:
+    case kRaderFs:
+    case kBlueFs: {
+      int dev = 0;
+      HIP_TRY(hipGetDevice(&dev));
+      if (dev != p->device) {
+        g_last_error = "fyfft_execute_noalloc: the plan belongs to device " + std::to_string(p->device) +
+                       ", the current device is " + std::to_string(dev);
+        return FYFFT_INVALID_ARGUMENT;
+      }
+      void* base = nullptr;
+      fyfft_status st = scratch_of_current_device(dev, &base);
+      Workspace w;
+      if (!st) st = scratch_workspace(p, base, &w);
+      if (st) return st;
+      return execute_rows(p, in, out, direction, p->batch, s, 0, &w);
+    }
+    default:
+      g_last_error = "fyfft_execute_noalloc: grouped, real, strided and multi-dimensional plans are not served";
+      return FYFFT_UNSUPPORTED;
+  }
 }
 
 fyfft_status fyfft_rfft_modes(const fyfft_plan* p, const void* x, void* modes, size_t k,

@@ -217,6 +217,24 @@ fyfft_status fyfft_plan_destroy(fyfft_plan* plan);
 fyfft_status fyfft_execute(const fyfft_plan* plan, const void* in, void* out,
                            int direction, void* stream);
 
+/* fyfft_execute without any allocation (no counterpart in the reference):
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Single-
+ * kernel plans (LDS-resident smooth, Rader and Bluestein lengths, 16-bit
+ * storage) run exactly as fyfft_execute.  Four-step plans (smooth lengths
+ * beyond one workgroup's LDS, four-step Rader and Bluestein) run their
+ * passes through a static device scratch array of FYFFT_NOALLOC_SCRATCH_BYTES
+ * (64 MiB per GPU, held from library load until the process ends) instead
+ * of the plan's workspace, in chunks of as many transforms as it holds;
+ * FYFFT_INVALID_LENGTH if one transform's workspace exceeds it.  Must be
+ * called with the plan's device current.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU (other streams, other
+ * threads) race.  Grouped, real, strided and multi-dimensional plans return
+ * FYFFT_UNSUPPORTED. */
+#define FYFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+fyfft_status fyfft_execute_noalloc(const fyfft_plan* plan, const void* in, void* out,
+                                   int direction, void* stream);
+
 /* dispatch<invert> with the reference's HOST-pointer semantics: copies the
  * batch to the device, runs it, copies back and synchronises before
  * returning.  in == out is allowed. */
