@@ -7,6 +7,17 @@ through the C ABI (fyfft_execute).  Prints ONE JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--batch 65536]
 
+The same line carries `configs`: every other configuration of the metric,
+timed in the same run (20 timed steps after 5 warm-up steps each): the
+N=1024 single-transform latency (host and device pointers), mixed N=6720
+(FP32, FP64), the prime lengths (Rader 65537, Bluestein 10007 and 1000003),
+the ragged grouped form against one launch per transform, the spectral
+convolution forward+backward and the BF16/FP16 storage forms; each with ms,
+roofline fraction and rel-L2 against the FP64 oracle.  --no-configs skips
+them (profiling runs).  `cpu_baseline` is the oracle's C port timed on this
+host; `cpu_reference_survey` holds the compiled reference's own CPU figures
+from the survey container (the reference does not build here).
+
 Other configurations (one JSON line each, same fields where they apply):
   --workload mixed     N=6720 complex<float> fwd+inv, batch 16384 (--precision f64: batch 4096)
   --workload prime     Rader N=65537 x 2048, Bluestein N=10007 x 1024 and N=1000003 x 16
@@ -40,6 +51,10 @@ def parse():
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-baseline", action="store_true",
+                    help="ragged: skip the one-launch-per-transform baselines (kernel-trace runs)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="headline only: skip the other configurations (profiling runs)")
     ap.add_argument("--cpu-sample", type=int, default=65536,
                     help="transforms in the CPU baseline sample (65536 x 4096 points: "
                          "about 20 s of CPU time over 16 threads)")
@@ -61,10 +76,24 @@ def read_traffic(n, prec):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(n, sample, prec):
+    """The oracle's FP32 C port of the reference ladder, timed live on this
+    host: all cores up to the 16 of one GPU's host share."""
     import numpy as np
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    nproc = os.cpu_count() or 1
+    threads = max(1, min(16, nproc))
     rng = np.random.default_rng(0)
     x = rng.standard_normal((sample, 2 * n), dtype=np.float32).view(np.complex64)
     _, sec = oracle.fft32_batch_timed(x, False, threads)
@@ -72,7 +101,28 @@ def cpu_baseline(n, sample, prec):
     return {"value": round(flops / sec / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
             "kind": "port",
             "sample": f"{sample} of the {n}-point transforms, FP32 scalar C port of the "
-                      f"reference ladder (oracle/), {threads} threads, {sec:.2f} s wall"}
+                      f"reference ladder (oracle/), {threads} threads of nproc={nproc} "
+                      f"({cpu_model()}), {sec:.2f} s wall"}
+
+
+# The compiled reference (AVX2) measured in the survey container: it cannot
+# be built on this host (its sources need Windows headers), so its own
+# figures are reported as measured there, labelled, never re-synthesised.
+CPU_REFERENCE_SURVEY = {
+    "label": "survey container: 8 vCPU Intel Xeon @ 2.7 GHz, g++ 11.4 -O2 -mavx2 -mfma, "
+             "one FFT1D_C2C_invoker per thread, out of place, best of 3 (not this host)",
+    "unit": "GFLOP/s",
+    "rows": [
+        {"n": 1024, "strategy": "kernel_direct", "us_per_transform_1t": 3.40, "gflops_1t": 15.05, "gflops_8t": 81.6},
+        {"n": 4096, "strategy": "kernel_direct", "us_per_transform_1t": 15.74, "gflops_1t": 15.61, "gflops_8t": 69.5},
+        {"n": 16384, "strategy": "kernel_direct", "us_per_transform_1t": 77.1, "gflops_1t": 14.88, "gflops_8t": 70.5},
+        {"n": 6720, "strategy": "mix_radix", "us_per_transform_1t": 126.5, "gflops_1t": 3.38, "gflops_8t": 14.2},
+        {"n": 10007, "strategy": "rader", "us_per_transform_1t": 595, "gflops_1t": 1.12, "gflops_8t": 8.58},
+        {"n": 10006, "strategy": "bluestein", "us_per_transform_1t": 566, "gflops_1t": 1.17, "gflops_8t": 7.99},
+    ],
+    "unsupported_by_reference": ["N=65537 (falls to an infeasible O(N^2) DFT)", "N=1000003 and ragged lengths "
+                                 "> 80000 (beyond max_input_length)", "FP64, batching, BF16, autograd"],
+}
 
 
 def settle(fn, seconds=0.2):
@@ -273,7 +323,7 @@ def bench_nd(a):
                           "traffic": None})
 
 
-def bench_ragged(a):
+def bench_ragged(a, compact_out=False):
     import numpy as np
     import torch
     import fyfft_amd as fy
@@ -300,8 +350,11 @@ def bench_ragged(a):
     for i in pick:
         n, o = int(lens[i]), int(offs[i])
         errs.append(oracle.rel_l2(yh[o:o + n], oracle.fft_ext(xh[o:o + n].astype(np.complex128))))
-    bp = fy.GroupedPlan(lens, fy.F32, fy.GROUPED_PER_TRANSFORM)
-    ms_base = time_steps(lambda: bp.execute(x, y, fy.FORWARD), 2, 1, torch.cuda.current_stream(dev), 0)
+    ms_base = float("nan")
+    if not getattr(a, "no_baseline", False):
+        bp = fy.GroupedPlan(lens, fy.F32, fy.GROUPED_PER_TRANSFORM)
+        ms_base = time_steps(lambda: bp.execute(x, y, fy.FORWARD), 2, 1, torch.cuda.current_stream(dev), 0)
+        bp.close()
     flops = sum(flops_of(int(n)) for n in lens)
     byt = 16.0 * tot
     out = line({"config": {"workload": "ragged grouped C2C, 20000 complex<float> transforms, lengths "
@@ -318,7 +371,40 @@ def bench_ragged(a):
                per_transform_baseline={"ms_per_step": round(ms_base, 3),
                                        "speedup_grouped": round(ms_base / ms, 2)})
     gp.close()
-    bp.close()
+    del y
+    # BF16 storage of the same workload: FP32 arithmetic, 8 bytes per element
+    xb = torch.view_as_real(x).to(torch.bfloat16)
+    yb = torch.empty_like(xb)
+    gpb = fy.GroupedPlan(lens, fy.BF16, fy.GROUPED_GRAPH)
+    with torch.cuda.stream(s):
+        ms16 = time_steps(lambda: gpb.execute(xb, yb, fy.FORWARD, stream=s), a.steps, a.warmup, s)
+    xq = xb.float().cpu().numpy().astype(np.float64)
+    yq = yb.float().cpu().numpy().astype(np.float64)
+    worst = 0.0
+    for i in pick:
+        n, o = int(lens[i]), int(offs[i])
+        ex = oracle.fft_ext(xq[o:o + n, 0] + 1j * xq[o:o + n, 1])
+        ext = np.stack([ex.real, ex.imag], -1)
+        floor = np.linalg.norm(torch.from_numpy(ext).to(torch.bfloat16).double().numpy() - ext) / np.linalg.norm(ext)
+        err = np.linalg.norm(yq[o:o + n] - ext) / np.linalg.norm(ext)
+        worst = max(worst, err / floor)
+    ms16_base = float("nan")
+    if not getattr(a, "no_baseline", False):
+        bpb = fy.GroupedPlan(lens, fy.BF16, fy.GROUPED_PER_TRANSFORM)
+        ms16_base = time_steps(lambda: bpb.execute(xb, yb, fy.FORWARD), 2, 1, torch.cuda.current_stream(dev), 0)
+        bpb.close()
+    out["bf16"] = {"ms_per_step": round(ms16, 4), "value": round(flops / ms16 / 1e6, 2),
+                   "roofline_frac": round(8.0 * tot / (ms16 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                   "error_vs_bf16_rounding_floor_max": round(worst, 4),
+                   "per_transform_baseline_ms": round(ms16_base, 3),
+                   "speedup_grouped": round(ms16_base / ms16, 2), "plan": gpb.describe()}
+    gpb.close()
+    if compact_out:
+        return {"ms_per_step": out["ms_per_step"], "value": out["value"], "unit": out["unit"],
+                "roofline_frac": out["roofline"]["frac"], "rel_l2_vs_fp64_max": out["rel_l2_vs_fp64_max"],
+                "per_transform_baseline_ms": out["per_transform_baseline"]["ms_per_step"],
+                "speedup_grouped": out["per_transform_baseline"]["speedup_grouped"],
+                "workload": out["config"]["workload"], "plan": out["config"]["plan"], "bf16": out["bf16"]}
     return out
 
 
@@ -395,6 +481,91 @@ def bench_lowprec(a):
                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
                           "frac": round(tot_bytes / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                           "traffic": None})
+
+
+def bench_latency(a):
+    """The reference's own configuration: one N = 1024 forward transform of
+    complex<float> on HOST buffers (dispatch<false>, synchronous), as a
+    drop-in caller runs it; plus the same transform on device pointers."""
+    import numpy as np
+    import torch
+    import fyfft_amd as fy
+    import oracle
+    n, calls = 1024, 2000
+    rng = np.random.default_rng(0)
+    x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex64)
+    inv = fy.FFT1D_C2C_invoker(n)
+    xf, out = x.view(np.float32), np.empty(2 * n, np.float32)
+    for _ in range(100):
+        inv.dispatch(xf, out, invert=False)
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        for _ in range(calls // 5):
+            inv.dispatch(xf, out, invert=False)
+        ts.append((time.perf_counter() - t0) / (calls // 5))
+    host_us = min(ts) * 1e6
+    err = oracle.rel_l2(out.view(np.complex64), oracle.dft_naive(x.astype(np.complex128)))
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    with fy.Plan(n, 1, fy.F32) as p:
+        h = torch.cuda.current_stream().cuda_stream
+        for _ in range(100):
+            p.execute_ptr(xd.data_ptr(), yd.data_ptr(), fy.FORWARD, h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            p.execute_ptr(xd.data_ptr(), yd.data_ptr(), fy.FORWARD, h)
+            torch.cuda.synchronize()
+        dev_us = (time.perf_counter() - t0) / calls * 1e6
+    return {"workload": "one forward C2C N=1024 complex<float>, batch 1 (latency-bound)",
+            "us_per_call_host_pointers": round(host_us, 2),
+            "us_per_call_device_pointers": round(dev_us, 2),
+            "reference_cpu_us_1t_survey": 3.40, "rel_l2_vs_fp64": err,
+            "note": "host path = copy in, launch, copy out, synchronise; the reference's 3.40 us is "
+                    "one AVX2 core in the survey container"}
+
+
+def bench_configs(a):
+    """Every configuration of the metric, compact, for the default line."""
+    import argparse as _ap
+    import torch
+    import fyfft_amd as fy
+    sub = _ap.Namespace(**vars(a))
+    sub.steps, sub.warmup = 20, 5
+    keep = ("n", "batch", "dtype", "direction", "form", "plan", "ms", "hbm_frac", "rel_l2_vs_fp64",
+            "storage_rounding_floor")
+
+    def compact(line_):
+        rows = line_["config"].get("cases", [])
+        return {"ms_per_step": line_["ms_per_step"], "value": line_["value"], "unit": line_["unit"],
+                "roofline_frac": line_.get("roofline", {}).get("frac"),
+                "cases": [{k: r[k] for k in keep if k in r} for r in rows]}
+
+    out = {}
+    steps = [
+        ("config1_latency", lambda: bench_latency(sub)),
+        ("mixed", lambda: compact(bench_transforms(sub, [(6720, 16384, fy.F32, (fy.FORWARD, fy.INVERSE)),
+                                                        (6720, 4096, fy.F64, (fy.FORWARD, fy.INVERSE))],
+                                                   "mixed"))),
+        ("prime", lambda: compact(bench_transforms(sub, [(65537, 2048, fy.F32, (fy.FORWARD,)),
+                                                        (10007, 1024, fy.F32, (fy.FORWARD,)),
+                                                        (1000003, 16, fy.F32, (fy.FORWARD,))], "prime"))),
+        ("ragged", lambda: bench_ragged(sub, compact_out=True)),
+        ("spectral", lambda: {k: v for k, v in bench_spectral(sub).items()
+                              if k in ("value", "unit", "ms_per_step", "config")}),
+        ("lowprec", lambda: compact(bench_lowprec(sub))),
+    ]
+    for name, fn in steps:
+        t0 = time.perf_counter()
+        try:
+            out[name] = fn()
+        except Exception as e:  # report, keep the other configurations
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+        out[name]["bench_seconds"] = round(time.perf_counter() - t0, 1)
+        torch.cuda.empty_cache()
+        print(f"# config {name} done in {out[name]['bench_seconds']} s", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -512,9 +683,14 @@ def main():
         out["plain_form"] = {"plan": pp.describe(), "ms_per_step": round(ms_plain, 5),
                              "hbm_frac": round(alg_bytes / (ms_plain * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                              "speedup_of_value": round(ms_plain / ms, 3)}
+    plan.close()
+    del x, y
+    torch.cuda.empty_cache()
+    if not a.no_configs:
+        out["configs"] = bench_configs(a)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, min(a.cpu_sample, batch), a.precision)
-    plan.close()
+        out["cpu_reference_survey"] = CPU_REFERENCE_SURVEY
     print(json.dumps(out), flush=True)
 
 

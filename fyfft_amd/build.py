@@ -34,17 +34,21 @@ def sources() -> list[str]:
                   if f.endswith((".hip", ".cpp")))
 
 
-def _headers() -> list[str]:
+def _headers(src: str) -> list[str]:
+    """Headers a source may include: every header under csrc/, and for
+    sources that include the public C ABI header, include/fyfft.h."""
     hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
-    inc = os.path.join(ROOT, "include")
-    return hdrs + [os.path.join(inc, f) for f in os.listdir(inc) if f.endswith((".h", ".hpp"))]
+    with open(src) as f:
+        if "fyfft.h" in f.read():
+            hdrs.append(os.path.join(ROOT, "include", "fyfft.h"))
+    return hdrs
 
 
-def _stale(src: str, obj: str, newest_header: float) -> bool:
+def _stale(src: str, obj: str) -> bool:
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    return t < os.path.getmtime(src) or t < newest_header
+    return t < os.path.getmtime(src) or any(t < os.path.getmtime(h) for h in _headers(src))
 
 
 def _compile(src: str, extra: tuple = ()) -> str:
@@ -69,8 +73,7 @@ def build(jobs: int | None = None, incremental: bool = False) -> str:
     objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in srcs]
     todo = srcs
     if incremental:
-        newest = max(os.path.getmtime(h) for h in _headers())
-        todo = [s for s, o in zip(srcs, objs) if _stale(s, o, newest)]
+        todo = [s for s, o in zip(srcs, objs) if _stale(s, o)]
     jobs = jobs or min(max(1, len(todo)), max(1, min(8, os.cpu_count() or 1)))
     with cf.ThreadPoolExecutor(jobs) as ex:
         list(ex.map(_compile, todo))

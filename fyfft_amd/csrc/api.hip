@@ -48,6 +48,21 @@ struct StridedPass {
   int fixed2 = 0, lo_bits = 0;
 };
 
+// One family launch of a grouped plan: the tasks [base, base + count) of the
+// device task table, split per XCD at xbase, with its workgroup size class,
+// dynamic LDS, grid and estimated cost (for ordering and branch balance).
+struct GList {
+  int fam = 0, cls = 0, hi = 0;
+  int base = 0, count = 0;
+  int xbase[dev::kXcds + 1] = {};
+  size_t lds = 0;
+  unsigned grid = 0;
+  double cost = 0;
+};
+// Parallel branches the family launches of one phase are spread over
+// (the box's hardware queues per process).
+constexpr int kGroupBranches = 4;
+
 // Forms of a real (R2C / C2R) plan.
 enum RealForm { kRealFusedHalf = 0, kRealFusedFull = 1, kRealHalfPasses = 2, kRealFullPasses = 3 };
 
@@ -126,20 +141,23 @@ struct fyfft_plan {
   std::vector<size_t> h_off;                 // transform -> packed element offset
   void *d_tasks = nullptr, *d_lens = nullptr, *d_off = nullptr, *d_woff = nullptr;
   void *gws = nullptr, *gx0 = nullptr;
-  // per (phase, workgroup-size class): task range, XCD split, LDS, grid
-  int task_base[3][kGroupClasses] = {}, task_count[3][kGroupClasses] = {};
-  int task_xbase[3][kGroupClasses][dev::kXcds + 1] = {};
-  size_t phase_lds[3][kGroupClasses] = {};
-  unsigned phase_grid[3][kGroupClasses] = {};
+  std::vector<GList> glists;                 // family launches, by phase then decreasing cost
+  hipStream_t gbranch[kGroupBranches] = {};  // branch streams of multi-family phases
+  hipEvent_t gfork[3] = {}, gjoin[3][kGroupBranches] = {};
+  void* gstage = nullptr;  // 16-bit per-transform baseline: FP32 copy of the longest transform
   mutable std::mutex graph_mu;
   mutable std::map<std::tuple<const void*, void*, int, hipStream_t>, hipGraphExec_t> graphs;
   std::vector<void*> allocs;
   size_t dev_bytes = 0;
   std::string desc;
-  // staging for fyfft_execute_host (lazily allocated, reused)
+  // staging for fyfft_execute_host (lazily allocated, reused): a device
+  // buffer, a pinned host buffer of the same size (null if pinned memory
+  // was refused: then pageable copies) and a stream of the plan's own
   mutable std::mutex host_mu;
   mutable void* d_io = nullptr;
+  mutable void* h_io = nullptr;
   mutable size_t d_io_bytes = 0;
+  mutable hipStream_t io_stream = nullptr;
 
   fyfft_status alloc(size_t bytes, void** p) {
     HIP_TRY(hipMalloc(p, bytes));
@@ -149,9 +167,18 @@ struct fyfft_plan {
   }
   ~fyfft_plan() {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    for (hipStream_t b : gbranch)
+      if (b) (void)hipStreamDestroy(b);
+    for (int ph = 0; ph < 3; ++ph) {
+      if (gfork[ph]) (void)hipEventDestroy(gfork[ph]);
+      for (hipEvent_t e : gjoin[ph])
+        if (e) (void)hipEventDestroy(e);
+    }
     for (fyfft_plan* sp : subs) delete sp;
     for (void* p : allocs) (void)hipFree(p);
     if (d_io) (void)hipFree(d_io);
+    if (h_io) (void)hipHostFree(h_io);
+    if (io_stream) (void)hipStreamDestroy(io_stream);
     magic = 0;
   }
 };
@@ -515,7 +542,7 @@ static fyfft_status wisdom_parse(const char* text) {
   return FYFFT_SUCCESS;
 }
 
-static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, int);
+static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, int, int force = -1);
 
 static fyfft_status fft_f64_inplace(double2* buf, size_t len) {
   fyfft_plan* sub = nullptr;
@@ -932,8 +959,12 @@ static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src,
   return exec_spectral<float2>(p, inverse, src, dst, k, scale0, scale, mm, s);
 }
 
+// force: -1 the library's strategy; kBluestein plans a Bluestein convolution
+// for any N > 1; kRader a Rader convolution for primes whose N - 1 is
+// 7-smooth (other N keep the library's choice); the other strategies keep
+// the library's choice (the caller checks eligibility).
 static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
-                                     int form) {
+                                     int form, int force) {
   if (!out) return FYFFT_INVALID_ARGUMENT;
   *out = nullptr;
   if (batch == 0) return FYFFT_INVALID_ARGUMENT;
@@ -994,6 +1025,12 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
   } else if (n == 1) {
     p->kind = kTrivial;
     p->desc = "trivial N=1";
+  } else if (force == kBluestein) {
+    p->strategy = kBluestein;
+    st = plan_bluestein(p);
+  } else if (force == kRader && n >= 3 && is_prime(n) && is_smooth7(n - 1)) {
+    p->strategy = kRader;
+    st = plan_rader(p);
   } else if (is_smooth7(n)) {
     LdsGeom g;
     if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
@@ -1141,12 +1178,20 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap) 
 template <typename V>
 static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vector<size_t>>& by_len) {
   const size_t cap = lds_cap_bytes(p->device), elem = sizeof(V);
+  constexpr int kLo = sizeof(V) == 8 ? 16 : 8;  // points-per-thread bound of the lo family kernels
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || ncu <= 0) ncu = 256;
   std::vector<GLen<V>> lens;
   std::vector<long long> off, woff;
   struct T { GTask t; double cost; };
-  std::vector<T> phase[3][kGroupClasses];
+  // (family, class, hi) -> tasks, LDS
+  std::map<std::tuple<int, int, int>, std::vector<T>> lists;
+  std::map<std::tuple<int, int, int>, size_t> list_lds;
+  auto add = [&](int fam, int cls, int hi, const T& t, size_t lds) {
+    const auto key = std::make_tuple(fam, cls, hi);
+    lists[key].push_back(t);
+    list_lds[key] = std::max(list_lds[key], lds);
+  };
   long long wsize = 0;
   int li = 0;
   for (const auto& kv : by_len) {
@@ -1166,78 +1211,87 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
     if (g.kind <= kGkBlue) {
       const size_t m = (size_t)g.lds.m;
       const double conv = g.kind == kGkLds ? 1.0 : 2.0;
-      const int k = lp.lds_cls, pb = lp.per_block;
+      const int fam = g.kind == kGkLds ? kGfLds : (g.kind == kGkRader ? kGfRader : kGfBlue);
+      const int hi = g.lds_ppt > kLo, k = lp.lds_cls, pb = lp.per_block;
+      if (hi && kGroupClassThreads[k] != 1024) return FYFFT_UNSUPPORTED;
       for (size_t i = 0; i < idx.size(); i += pb) {
         const int c = (int)std::min<size_t>(pb, idx.size() - i);
-        phase[0][k].push_back({GTask{li, 0, first + (int)i, c}, conv * c * m * lg});
+        add(fam, k, hi, T{GTask{li, 0, first + (int)i, c}, conv * c * m * lg}, (size_t)pb * g.lds.stride * elem);
       }
-      p->phase_lds[0][k] = std::max(p->phase_lds[0][k], (size_t)pb * g.lds.stride * elem);
     } else {
+      if (g.col_ppt > kLo || g.row_ppt > kLo) return FYFFT_UNSUPPORTED;
       const int C = 1 << g.col.log2c, R = 1 << g.row.log2c;
       const double conv = g.kind == kGkFsPlain ? 1.0 : 2.0;
       const int kc = lp.col_cls, kr = lp.row_cls;
+      const int famA = g.kind == kGkFsPlain ? kGfColPlain : (g.kind == kGkFsRader ? kGfColRaderA : kGfColBlueA);
+      const int famB = g.kind == kGkFsPlain ? kGfRowPlain : kGfRowConv;
+      const int famC = g.kind == kGkFsRader ? kGfColRaderC : kGfColBlueC;
+      const size_t col_lds = (size_t)C * g.col.stride * elem, row_lds = (size_t)R * g.row.stride * elem;
       for (size_t i = 0; i < idx.size(); ++i) {
         for (int c0 = 0; c0 < g.col.n2; c0 += C) {
-          phase[0][kc].push_back({GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg});
+          add(famA, kc, 0, T{GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg}, col_lds);
           if (g.kind != kGkFsPlain)
-            phase[2][kc].push_back({GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg});
+            add(famC, kc, 0, T{GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg}, col_lds);
         }
         for (int r0 = 0; r0 < g.col.n1; r0 += R)
-          phase[1][kr].push_back({GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg});
+          add(famB, kr, 0, T{GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg}, row_lds);
       }
-      const size_t col_lds = (size_t)C * g.col.stride * elem, row_lds = (size_t)R * g.row.stride * elem;
-      p->phase_lds[0][kc] = std::max(p->phase_lds[0][kc], col_lds);
-      p->phase_lds[1][kr] = std::max(p->phase_lds[1][kr], row_lds);
-      if (g.kind != kGkFsPlain) p->phase_lds[2][kc] = std::max(p->phase_lds[2][kc], col_lds);
     }
     ++li;
   }
-  // Bin-pack each (phase, class) list onto the XCDs: all tasks of one
-  // transform (four-step groups) stay together, greedy by decreasing group
-  // cost onto the least loaded XCD; within an XCD, tasks by decreasing cost.
+  // Bin-pack each family list onto the XCDs: all tasks of one transform
+  // (four-step groups) stay together, greedy by decreasing group cost onto
+  // the least loaded XCD; within an XCD, tasks by decreasing cost.
   std::vector<GTask> tasks;
-  size_t class_lds[kGroupClasses] = {};
-  for (int ph = 0; ph < 3; ++ph)
-    for (int k = 0; k < kGroupClasses; ++k) {
-      const std::vector<T>& list = phase[ph][k];
-      std::map<std::pair<int, int>, std::vector<T>> groups;  // (len, transform or task) -> tasks
-      for (size_t i = 0; i < list.size(); ++i) {
-        const T& t = list[i];
-        const bool fs = lens[t.t.len].kind >= kGkFsPlain;
-        groups[{t.t.len, fs ? t.t.first : -1 - (int)i}].push_back(t);
-      }
-      std::vector<std::pair<double, std::vector<T>*>> order;
-      for (auto& kv : groups) {
-        double c = 0;
-        for (const T& t : kv.second) c += t.cost;
-        order.push_back({c, &kv.second});
-      }
-      std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-      std::vector<T> per_x[dev::kXcds];
-      double load[dev::kXcds] = {0};
-      for (auto& g : order) {
-        const int x = (int)(std::min_element(load, load + dev::kXcds) - load);
-        load[x] += g.first;
-        per_x[x].insert(per_x[x].end(), g.second->begin(), g.second->end());
-      }
-      p->task_base[ph][k] = (int)tasks.size();
-      p->task_count[ph][k] = (int)list.size();
-      for (int x = 0; x < dev::kXcds; ++x) {
-        std::stable_sort(per_x[x].begin(), per_x[x].end(), [](const T& a, const T& b) { return a.cost > b.cost; });
-        p->task_xbase[ph][k][x] = (int)tasks.size() - p->task_base[ph][k];
-        for (const T& t : per_x[x]) tasks.push_back(t.t);
-      }
-      p->task_xbase[ph][k][dev::kXcds] = (int)tasks.size() - p->task_base[ph][k];
-      p->phase_lds[ph][k] = std::max<size_t>(p->phase_lds[ph][k], 16);
-      class_lds[k] = std::max(class_lds[k], p->phase_lds[ph][k]);
-      // resident workgroups per CU: 1024 / class threads (LDS budget per
-      // class keeps them within the CU); the grid is a multiple of the
-      // XCD count
-      const size_t per_cu = 1024 / (size_t)kGroupClassThreads[k];
-      size_t grid = std::min<size_t>(list.size(), (size_t)ncu * per_cu);
-      grid = std::max<size_t>(dev::kXcds, (grid + dev::kXcds - 1) / dev::kXcds * dev::kXcds);
-      p->phase_grid[ph][k] = (unsigned)grid;
+  p->glists.clear();
+  for (const auto& kv : lists) {
+    const std::vector<T>& list = kv.second;
+    GList L;
+    L.fam = std::get<0>(kv.first);
+    L.cls = std::get<1>(kv.first);
+    L.hi = std::get<2>(kv.first);
+    std::map<std::pair<int, int>, std::vector<T>> groups;  // (len, transform or task) -> tasks
+    for (size_t i = 0; i < list.size(); ++i) {
+      const T& t = list[i];
+      const bool fs = lens[t.t.len].kind >= kGkFsPlain;
+      groups[{t.t.len, fs ? t.t.first : -1 - (int)i}].push_back(t);
     }
+    std::vector<std::pair<double, std::vector<T>*>> order;
+    for (auto& g : groups) {
+      double c = 0;
+      for (const T& t : g.second) c += t.cost;
+      order.push_back({c, &g.second});
+      L.cost += c;
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<T> per_x[dev::kXcds];
+    double load[dev::kXcds] = {0};
+    for (auto& g : order) {
+      const int x = (int)(std::min_element(load, load + dev::kXcds) - load);
+      load[x] += g.first;
+      per_x[x].insert(per_x[x].end(), g.second->begin(), g.second->end());
+    }
+    L.base = (int)tasks.size();
+    L.count = (int)list.size();
+    for (int x = 0; x < dev::kXcds; ++x) {
+      std::stable_sort(per_x[x].begin(), per_x[x].end(), [](const T& u, const T& v) { return u.cost > v.cost; });
+      L.xbase[x] = (int)tasks.size() - L.base;
+      for (const T& t : per_x[x]) tasks.push_back(t.t);
+    }
+    L.xbase[dev::kXcds] = (int)tasks.size() - L.base;
+    L.lds = std::max<size_t>(list_lds[kv.first], 16);
+    // resident workgroups per CU: 1024 / class threads (the class LDS budget
+    // keeps them within the CU); the grid is a multiple of the XCD count
+    const size_t per_cu = 1024 / (size_t)kGroupClassThreads[L.cls];
+    size_t grid = std::min<size_t>(list.size(), (size_t)ncu * per_cu);
+    grid = std::max<size_t>(dev::kXcds, (grid + dev::kXcds - 1) / dev::kXcds * dev::kXcds);
+    L.grid = (unsigned)grid;
+    p->glists.push_back(L);
+  }
+  std::stable_sort(p->glists.begin(), p->glists.end(), [](const GList& x, const GList& y) {
+    const int px = family_phase(x.fam), py = family_phase(y.fam);
+    return px != py ? px < py : x.cost > y.cost;
+  });
   fyfft_status st;
   if ((st = p->alloc(std::max<size_t>(1, tasks.size()) * sizeof(GTask), &p->d_tasks)) ||
       (st = p->alloc(lens.size() * sizeof(GLen<V>), &p->d_lens)) ||
@@ -1250,15 +1304,29 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
   HIP_TRY(hipMemcpy(p->d_lens, lens.data(), lens.size() * sizeof(GLen<V>), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_off, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_woff, woff.data(), woff.size() * sizeof(long long), hipMemcpyHostToDevice));
-  HIP_TRY(prepare_grouped(p->prec, class_lds));
-  auto counts = [&](int ph) {
-    int n = 0;
-    for (int k = 0; k < kGroupClasses; ++k) n += p->task_count[ph][k];
-    return std::to_string(n);
-  };
+  int phase_lists[3] = {0, 0, 0};
+  for (const GList& L : p->glists) {
+    ++phase_lists[family_phase(L.fam)];
+    for (int dir : {-1, +1})
+      HIP_TRY(group_family(p->prec, true, L.fam, dir, L.hi != 0, kGroupClassThreads[L.cls], nullptr, 0, L.lds,
+                           nullptr));
+  }
+  // branch streams and fork/join events for phases with several families
+  // (default priority: streams of other priorities have hung these GPUs)
+  if (std::max({phase_lists[0], phase_lists[1], phase_lists[2]}) > 1 && !(p->gflags & FYFFT_GROUPED_PER_TRANSFORM)) {
+    for (hipStream_t& b : p->gbranch) HIP_TRY(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
+    for (int ph = 0; ph < 3; ++ph) {
+      HIP_TRY(hipEventCreateWithFlags(&p->gfork[ph], hipEventDisableTiming));
+      for (hipEvent_t& e : p->gjoin[ph]) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+  }
+  size_t ntask[3] = {0, 0, 0};
+  for (const GList& L : p->glists) ntask[family_phase(L.fam)] += (size_t)L.count;
+  static const char* kStore[] = {"f32", "f64", "bf16", "fp16"};
   p->desc = "grouped count=" + std::to_string(p->count) + " lengths=" + std::to_string(lens.size()) +
-            " total=" + std::to_string(p->total) + " tasks=" + counts(0) + "/" + counts(1) + "/" + counts(2) +
-            " workspace=" + std::to_string(wsize);
+            " total=" + std::to_string(p->total) + " storage=" + kStore[p->prec] + " tasks=" +
+            std::to_string(ntask[0]) + "/" + std::to_string(ntask[1]) + "/" + std::to_string(ntask[2]) +
+            " family-launches=" + std::to_string(p->glists.size()) + " workspace=" + std::to_string(wsize);
   return FYFFT_SUCCESS;
 }
 
@@ -1269,31 +1337,67 @@ static fyfft_status launch_grouped_phases(const fyfft_plan* p, const void* in, v
   a.lens = (const GLen<V>*)p->d_lens;
   a.off = (const long long*)p->d_off;
   a.woff = (const long long*)p->d_woff;
-  a.in = (const V*)in;
-  a.out = (V*)out;
+  a.in = in;
+  a.out = out;
   a.ws = (V*)p->gws;
   a.x0buf = (V*)p->gx0;
   a.tables = direction != FYFFT_FORWARD;
   a.normalize = direction == FYFFT_INVERSE;
   const int dir = direction == FYFFT_FORWARD ? -1 : +1;
-  for (int ph = 0; ph < 3; ++ph)
-    for (int k = kGroupClasses - 1; k >= 0; --k) {
-      if (!p->task_count[ph][k]) continue;
-      a.tasks = (const GTask*)p->d_tasks + p->task_base[ph][k];
-      for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = p->task_xbase[ph][k][x];
-      HIP_TRY(launch_grouped(p->prec, dir, &a, p->phase_grid[ph][k], kGroupClassThreads[k],
-                             p->phase_lds[ph][k], s));
+  auto launch = [&](const GList& L, hipStream_t st) {
+    a.tasks = (const GTask*)p->d_tasks + L.base;
+    for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = L.xbase[x];
+    return group_family(p->prec, false, L.fam, dir, L.hi != 0, kGroupClassThreads[L.cls], &a, L.grid, L.lds, st);
+  };
+  size_t i = 0;
+  for (int ph = 0; ph < 3; ++ph) {
+    size_t j = i;
+    while (j < p->glists.size() && family_phase(p->glists[j].fam) == ph) ++j;
+    const int n = (int)(j - i);
+    if (n == 1 || (n > 1 && !p->gbranch[0])) {
+      for (size_t k = i; k < j; ++k) HIP_TRY(launch(p->glists[k], s));
+    } else if (n > 1) {
+      // fork: every family of the phase waits for the previous phase, runs on
+      // the least loaded branch (lists come by decreasing cost), and the
+      // stream waits for every branch before the next phase
+      const int nb = std::min(n, kGroupBranches);
+      HIP_TRY(hipEventRecord(p->gfork[ph], s));
+      for (int b = 0; b < nb; ++b) HIP_TRY(hipStreamWaitEvent(p->gbranch[b], p->gfork[ph], 0));
+      double load[kGroupBranches] = {0};
+      for (size_t k = i; k < j; ++k) {
+        const int b = (int)(std::min_element(load, load + nb) - load);
+        load[b] += p->glists[k].cost;
+        HIP_TRY(launch(p->glists[k], p->gbranch[b]));
+      }
+      for (int b = 0; b < nb; ++b) {
+        HIP_TRY(hipEventRecord(p->gjoin[ph][b], p->gbranch[b]));
+        HIP_TRY(hipStreamWaitEvent(s, p->gjoin[ph][b], 0));
+      }
     }
+    i = j;
+  }
   return FYFFT_SUCCESS;
 }
 
 static fyfft_status exec_grouped(const fyfft_plan* p, const void* in, void* out, int direction, hipStream_t s) {
   const size_t elem = elem_bytes(p->prec);
+  const bool low = p->prec == kBF16 || p->prec == kF16;
   if (p->gflags & FYFFT_GROUPED_PER_TRANSFORM) {
+    // the baseline: one launch of the plain kernels per transform; 16-bit
+    // data is widened into an FP32 copy around each one
     for (size_t i = 0; i < p->count; ++i) {
-      const fyfft_status st = fyfft_execute(p->subs[p->sub_of[i]], (const char*)in + p->h_off[i] * elem,
-                                            (char*)out + p->h_off[i] * elem, direction, s);
-      if (st) return st;
+      const fyfft_plan* sp = p->subs[p->sub_of[i]];
+      const char* x = (const char*)in + p->h_off[i] * elem;
+      char* y = (char*)out + p->h_off[i] * elem;
+      if (low) {
+        HIP_TRY(launch_convert16(p->prec == kBF16, true, x, p->gstage, (long long)sp->n, s));
+        fyfft_status st = fyfft_execute(sp, p->gstage, p->gstage, direction, s);
+        if (st) return st;
+        HIP_TRY(launch_convert16(p->prec == kBF16, false, p->gstage, y, (long long)sp->n, s));
+      } else {
+        const fyfft_status st = fyfft_execute(sp, x, y, direction, s);
+        if (st) return st;
+      }
     }
     return FYFFT_SUCCESS;
   }
@@ -1958,6 +2062,36 @@ fyfft_status fyfft_plan_create_ex(fyfft_plan** out, size_t n, size_t batch, fyff
   return create_plan_impl(out, n, batch, precision, form);
 }
 
+fyfft_status fyfft_plan_create_strategy(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
+                                       int strategy) {
+  if (strategy < FYFFT_KERNEL_DIRECT || strategy > FYFFT_DFT) return FYFFT_INVALID_ARGUMENT;
+  if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
+  if ((strategy == FYFFT_KERNEL_DIRECT && !in_reference_registry(n)) ||
+      (strategy == FYFFT_MIX_RADIX && !is_smooth7(n))) {
+    g_last_error = "fyfft_plan_create_strategy: N=" + std::to_string(n) + " is not " +
+                   (strategy == FYFFT_KERNEL_DIRECT ? "a directly supported length" : "2/3/5/7-smooth");
+    return FYFFT_UNSUPPORTED;
+  }
+  if (precision == FYFFT_BF16 || precision == FYFFT_F16) {
+    if (strategy == FYFFT_RADER || strategy == FYFFT_BLUESTEIN) return FYFFT_UNSUPPORTED;
+  }
+  return create_plan_impl(out, n, batch, precision, FYFFT_FORM_AUTO, strategy);
+}
+
+fyfft_status fyfft_strategy_processable(int strategy, size_t n, int* processable) {
+  if (!processable) return FYFFT_INVALID_ARGUMENT;
+  switch (strategy) {
+    case FYFFT_KERNEL_DIRECT: *processable = in_reference_registry(n); break;
+    case FYFFT_MIX_RADIX: *processable = n >= 1 && is_smooth7(n); break;
+    // a prime outside the registry whose 2N - 3 has a registry length above it
+    case FYFFT_RADER: *processable = !in_reference_registry(n) && is_prime(n) && 2 * n - 3 <= 78125; break;
+    case FYFFT_BLUESTEIN: *processable = n <= 39062; break;
+    case FYFFT_DFT: *processable = n >= 1; break;
+    default: return FYFFT_INVALID_ARGUMENT;
+  }
+  return FYFFT_SUCCESS;
+}
+
 fyfft_status fyfft_plan_create_real(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision) {
   return create_real_impl(out, n, batch, precision);
 }
@@ -1998,8 +2132,8 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
   *out = nullptr;
   if (!lengths || count == 0 || count > (size_t)INT32_MAX / 2) return FYFFT_INVALID_ARGUMENT;
   if (flags & ~(unsigned)(FYFFT_GROUPED_GRAPH | FYFFT_GROUPED_PER_TRANSFORM)) return FYFFT_INVALID_ARGUMENT;
-  if (precision != FYFFT_F32 && precision != FYFFT_F64)
-    return (precision == FYFFT_BF16 || precision == FYFFT_F16) ? FYFFT_UNSUPPORTED : FYFFT_INVALID_ARGUMENT;
+  if (precision != FYFFT_F32 && precision != FYFFT_F64 && precision != FYFFT_BF16 && precision != FYFFT_F16)
+    return FYFFT_INVALID_ARGUMENT;
   for (size_t i = 0; i < count; ++i)
     if (lengths[i] < 1 || lengths[i] > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
   int dev = 0;
@@ -2007,7 +2141,9 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
   fyfft_plan* p = new (std::nothrow) fyfft_plan();
   if (!p) return FYFFT_ALLOC_FAILED;
   p->kind = kGrouped;
-  p->prec = precision == FYFFT_F64 ? kF64 : kF32;
+  // storage precision; 16-bit plans compute in FP32 with FP32 sub-plans
+  p->prec = precision == FYFFT_F64 ? kF64 : (precision == FYFFT_BF16 ? kBF16 : (precision == FYFFT_F16 ? kF16 : kF32));
+  const fyfft_precision sub_prec = precision == FYFFT_F64 ? FYFFT_F64 : FYFFT_F32;
   p->device = dev;
   p->gflags = flags;
   p->count = count;
@@ -2024,12 +2160,14 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
   fyfft_status st = FYFFT_SUCCESS;
   for (const auto& kv : by_len) {
     fyfft_plan* sp = nullptr;
-    st = create_plan_impl(&sp, kv.first, 1, precision, FYFFT_FORM_PLAIN);
+    st = create_plan_impl(&sp, kv.first, 1, sub_prec, FYFFT_FORM_PLAIN);
     if (st) break;
     for (size_t i : kv.second) p->sub_of[i] = (int)p->subs.size();
     p->subs.push_back(sp);
   }
   if (!st) st = p->prec == kF64 ? build_grouped<double2>(p, by_len) : build_grouped<float2>(p, by_len);
+  if (!st && (p->prec == kBF16 || p->prec == kF16) && (flags & FYFFT_GROUPED_PER_TRANSFORM))
+    st = p->alloc(by_len.rbegin()->first * sizeof(float2), &p->gstage);
   if (st) {
     delete p;
     return st;
@@ -2153,19 +2291,40 @@ fyfft_status fyfft_execute_host(const fyfft_plan* p, const void* in, void* out, 
   const bool apart = p->kind == kReal || p->kind == kStrided;
   const size_t out_off = apart ? (in_bytes + 255) / 256 * 256 : 0;
   const size_t bytes = std::max(in_bytes, out_off + out_bytes);
+  if (!p->io_stream) HIP_TRY(hipStreamCreateWithFlags(&p->io_stream, hipStreamNonBlocking));
   if (p->d_io_bytes < bytes) {
     if (p->d_io) (void)hipFree(p->d_io);
-    p->d_io = nullptr;
+    if (p->h_io) (void)hipHostFree(p->h_io);
+    p->d_io = p->h_io = nullptr;
     p->d_io_bytes = 0;
     HIP_TRY(hipMalloc(&p->d_io, bytes));
+    // pinned staging: the copies run as DMA on the plan's stream and only
+    // the final synchronisation waits (pageable copies if it is refused)
+    if (hipHostMalloc(&p->h_io, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p->h_io = nullptr;
+    }
     p->d_io_bytes = bytes;
   }
+  const hipStream_t s = p->io_stream;
   void* dout = (char*)p->d_io + out_off;
-  HIP_TRY(hipMemcpy(p->d_io, in, in_bytes, hipMemcpyHostToDevice));
-  if (apart) HIP_TRY(hipMemcpy(dout, out, out_bytes, hipMemcpyHostToDevice));
-  fyfft_status st = fyfft_execute(p, p->d_io, dout, direction, nullptr);
-  if (st) return st;
-  HIP_TRY(hipMemcpy(out, dout, out_bytes, hipMemcpyDeviceToHost));
+  char* h = (char*)p->h_io;
+  if (h) {
+    std::memcpy(h, in, in_bytes);
+    if (apart) std::memcpy(h + out_off, out, out_bytes);
+    HIP_TRY(hipMemcpyAsync(p->d_io, h, apart ? out_off + out_bytes : in_bytes, hipMemcpyHostToDevice, s));
+  } else {
+    HIP_TRY(hipMemcpyAsync(p->d_io, in, in_bytes, hipMemcpyHostToDevice, s));
+    if (apart) HIP_TRY(hipMemcpyAsync(dout, out, out_bytes, hipMemcpyHostToDevice, s));
+  }
+  fyfft_status st = fyfft_execute(p, p->d_io, dout, direction, s);
+  if (st) {
+    (void)hipStreamSynchronize(s);
+    return st;
+  }
+  HIP_TRY(hipMemcpyAsync(h ? (void*)(h + out_off) : out, dout, out_bytes, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (h) std::memcpy(out, h + out_off, out_bytes);
   return FYFFT_SUCCESS;
 }
 

@@ -56,7 +56,7 @@ template <typename V, int PPT> struct GenericFft {
 template <typename V, int D, int N, int T, int Ls, int R, int... Rest>
 __device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, bool active, int ps) {
   constexpr int MAXB = cdiv(N / R, T);
-  stockham_pass<V, R, MAXB, D, false, false, tw_all(Ls, R)>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
+  stockham_pass<V, R, MAXB, D, false, false>(lds, lds, N, Ls, T, tid, tw, active, 1, ps);
   if constexpr (sizeof...(Rest) > 0) fixed_lds_passes<V, D, N, T, Ls * R, Rest...>(lds, tid, tw, active, ps);
 }
 
@@ -112,68 +112,91 @@ template <int TW, typename V> __device__ __forceinline__ V root_at(const V* w, i
   }
 }
 
+// ------------------------------------------------------------ storage
+// Element storage of a transform's input and output: E == V (complex float
+// or double), or a pair of 16-bit floats (cplx16<H>) widened to / rounded
+// from V = float2, so the 16-bit forms run the same bodies in FP32.  ld/st
+// go through the caches (gathers, scatters); ldnt/stnt are the streaming
+// accesses of contiguous reads and writes.
+template <typename V, typename E> struct Elem {
+  static_assert(sizeof(E) == sizeof(V), "E == V expected");
+  static __device__ __forceinline__ V ld(const E* p) { return *p; }
+  static __device__ __forceinline__ V ldnt(const E* p) { return gload(p); }
+  static __device__ __forceinline__ void st(E* p, V v) { *p = v; }
+  static __device__ __forceinline__ void stnt(E* p, V v) { gstore(p, v); }
+};
+template <typename H> struct Elem<float2, cplx16<H>> {
+  static __device__ __forceinline__ float2 ld(const cplx16<H>* p) {
+    const cplx16<H> c = *p;
+    return float2{(float)c.re, (float)c.im};
+  }
+  static __device__ __forceinline__ float2 ldnt(const cplx16<H>* p) { return load16(p); }
+  static __device__ __forceinline__ void st(cplx16<H>* p, float2 v) { *p = cplx16<H>{(H)v.x, (H)v.y}; }
+  static __device__ __forceinline__ void stnt(cplx16<H>* p, float2 v) { store16(p, v); }
+};
+
 // ------------------------------------------------------------ functors
 template <typename V> using gbl_ptr = __attribute__((address_space(1))) V*;
 
-template <typename V> struct GlobalSrc {  // x[i]
+template <typename V, typename E = V> struct GlobalSrc {  // x[i]
   static constexpr bool kFromLds = false;
-  gbl_cptr<V> x;
-  __device__ V operator()(int i) const { return gload((const V*)x + i); }
+  gbl_cptr<E> x;
+  __device__ V operator()(int i) const { return Elem<V, E>::ldnt((const E*)x + i); }
 };
-template <typename V> struct GlobalSink {  // y[i] = v * scale
+template <typename V, typename E = V> struct GlobalSink {  // y[i] = v * scale
   static constexpr bool kToLds = false;
-  gbl_ptr<V> y;
+  gbl_ptr<E> y;
   S<V> scale;
-  __device__ void operator()(int i, V v) const { gstore((V*)y + i, cscale(v, scale)); }
+  __device__ void operator()(int i, V v) const { Elem<V, E>::stnt((E*)y + i, cscale(v, scale)); }
 };
-template <typename V> struct RaderSrc {  // a[j] = x[g^-j mod N]
+template <typename V, typename E = V> struct RaderSrc {  // a[j] = x[g^-j mod N]
   static constexpr bool kFromLds = false;
-  gbl_cptr<V> x;
+  gbl_cptr<E> x;
   gbl_cptr<int> perm;
-  __device__ V operator()(int j) const { return ((const V*)x)[((const int*)perm)[j]]; }
+  __device__ V operator()(int j) const { return Elem<V, E>::ld((const E*)x + ((const int*)perm)[j]); }
 };
-template <typename V> struct RaderSink {  // y[g^i mod N] = (v + x0) * scale
+template <typename V, typename E = V> struct RaderSink {  // y[g^i mod N] = (v + x0) * scale
   static constexpr bool kToLds = false;
-  gbl_ptr<V> y;
+  gbl_ptr<E> y;
   gbl_cptr<int> perm;
   V x0;
   S<V> scale;
   __device__ void operator()(int i, V v) const {
-    ((V*)y)[((const int*)perm)[i]] = cscale(cadd(v, x0), scale);
+    Elem<V, E>::st((E*)y + ((const int*)perm)[i], cscale(cadd(v, x0), scale));
   }
 };
 // Spectrum times the kernel spectrum, back into LDS.  With y0 set, bin 0
 // (the sum of the Rader sequence) also gives y[0] = (x0 + bin 0) * scale.
-template <typename V> struct KernSink {
+template <typename V, typename E = V> struct KernSink {
   static constexpr bool kToLds = true;
   lds_ptr<V> p;
   int ps;
   gbl_cptr<V> kern;
-  gbl_ptr<V> y0;
+  gbl_ptr<E> y0;
   V x0;
   S<V> scale;
   __device__ void operator()(int i, V v) const {
-    if (y0 != nullptr && i == 0) *(V*)y0 = cscale(cadd(x0, v), scale);
+    if (y0 != nullptr && i == 0) Elem<V, E>::st((E*)y0, cscale(cadd(x0, v), scale));
     ((V*)p)[i + (i >> ps)] = cmul(v, ((const V*)kern)[i]);
   }
 };
-template <typename V> struct BlueSrc {  // x[n] * chirp[n], zero past N
+template <typename V, typename E = V> struct BlueSrc {  // x[n] * chirp[n], zero past N
   static constexpr bool kFromLds = false;
-  gbl_cptr<V> x;
+  gbl_cptr<E> x;
   gbl_cptr<V> chirp;
   int n;
   __device__ V operator()(int i) const {
-    return i < n ? cmul(((const V*)x)[i], ((const V*)chirp)[i]) : V{0, 0};
+    return i < n ? cmul(Elem<V, E>::ld((const E*)x + i), ((const V*)chirp)[i]) : V{0, 0};
   }
 };
-template <typename V> struct BlueSink {  // y[k] = v * chirp[k] * scale, k < N
+template <typename V, typename E = V> struct BlueSink {  // y[k] = v * chirp[k] * scale, k < N
   static constexpr bool kToLds = false;
-  gbl_ptr<V> y;
+  gbl_ptr<E> y;
   gbl_cptr<V> chirp;
   int n;
   S<V> scale;
   __device__ void operator()(int k, V v) const {
-    if (k < n) ((V*)y)[k] = cscale(cmul(v, ((const V*)chirp)[k]), scale);
+    if (k < n) Elem<V, E>::st((E*)y + k, cscale(cmul(v, ((const V*)chirp)[k]), scale));
   }
 };
 
@@ -193,10 +216,10 @@ template <typename V> struct TwSink {
 };
 
 // ------------------------------------------------------------ smooth
-template <typename V, int PPT, int DIR>
-__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const V* x, V* y, V* lds,
+template <typename V, int PPT, int DIR, typename E>
+__device__ __forceinline__ void body_lds_smooth(const LdsBody<V>& a, const E* x, E* y, V* lds,
                                                 int tid, bool active) {
-  lds_fft_io<V, PPT, DIR>(GlobalSrc<V>{(gbl_cptr<V>)x}, GlobalSink<V>{(gbl_ptr<V>)y, a.scale}, lds,
+  lds_fft_io<V, PPT, DIR>(GlobalSrc<V, E>{(gbl_cptr<E>)x}, GlobalSink<V, E>{(gbl_ptr<E>)y, a.scale}, lds,
                           a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
 }
 
@@ -213,18 +236,19 @@ template <typename F> constexpr bool kStageConv = F::kThreads >= 640;
 // ------------------------------------------------------------ Rader
 // X[0] = x0 + sum_{n>=1} x[n];  X[g^i] = x0 + (a (*) b)[i],  a[j] = x[g^-j]
 // F: the I/O policy of the LDS transforms (GenericIo or FixedIo).
-template <typename V, typename F>
-__device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
+template <typename V, typename F, typename E>
+__device__ __forceinline__ void body_rader(const LdsBody<V>& a, const E* x, E* y, V* lds_in, int tid,
                                            bool active) {
+  using El = Elem<V, E>;
   V* lds = as_lds(lds_in);
-  const V x0 = active ? x[0] : V{0, 0};
+  const V x0 = active ? El::ld(x) : V{0, 0};
   if constexpr (!kStageConv<F>) {
     __syncthreads();  // every read of x[0] precedes the write of y[0] (in place)
-    F::template run<-1>(RaderSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<int>)a.perm_in},
-                        KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
+    F::template run<-1>(RaderSrc<V, E>{(gbl_cptr<E>)x, (gbl_cptr<int>)a.perm_in},
+                        KernSink<V, E>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<E>)y, x0, a.scale},
                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
     F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                        RaderSink<V>{(gbl_ptr<V>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
+                        RaderSink<V, E>{(gbl_ptr<E>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
     return;
   }
@@ -238,7 +262,7 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int j = j0 + q * a.threads;
-        if (j < a.m) v[q] = x[perm[j]];
+        if (j < a.m) v[q] = El::ld(x + perm[j]);
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
@@ -249,7 +273,7 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
   }
   __syncthreads();
   F::template run<-1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                      KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<V>)y, x0, a.scale},
+                      KernSink<V, E>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<E>)y, x0, a.scale},
                       lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
   F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps}, LdsSink<V>{(lds_ptr<V>)lds, a.ps}, lds, a.m, a.threads,
                       tid, a.pd, a.tw, active, a.ps);
@@ -266,7 +290,7 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int i = i0 + q * a.threads;
-        if (i < a.m) y[perm[i]] = v[q];
+        if (i < a.m) El::st(y + perm[i], v[q]);
       }
     }
   }
@@ -274,16 +298,17 @@ __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const V* x, V* y
 
 // ------------------------------------------------------------ Bluestein
 // X[k] = chirp[k] * IFFT(FFT(x * chirp, zero-padded to M) * kernel)[k]
-template <typename V, typename F>
-__device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y, V* lds_in, int tid,
+template <typename V, typename F, typename E>
+__device__ __forceinline__ void body_blue(const LdsBody<V>& a, const E* x, E* y, V* lds_in, int tid,
                                           bool active) {
+  using El = Elem<V, E>;
   V* lds = as_lds(lds_in);
   if constexpr (!kStageConv<F>) {
-    F::template run<-1>(BlueSrc<V>{(gbl_cptr<V>)x, (gbl_cptr<V>)a.chirp, a.n},
+    F::template run<-1>(BlueSrc<V, E>{(gbl_cptr<E>)x, (gbl_cptr<V>)a.chirp, a.n},
                         KernSink<V>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, nullptr, V{0, 0}, 1},
                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
     F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                        BlueSink<V>{(gbl_ptr<V>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
+                        BlueSink<V, E>{(gbl_ptr<E>)y, (gbl_cptr<V>)a.chirp, a.n, a.scale},
                         lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
     return;
   }
@@ -295,7 +320,7 @@ __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y,
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int i = i0 + j * a.threads;
-        v[j] = i < a.n ? cmul(x[i], chirp[i]) : V{0, 0};
+        v[j] = i < a.n ? cmul(El::ld(x + i), chirp[i]) : V{0, 0};
       }
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
@@ -323,7 +348,7 @@ __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const V* x, V* y,
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int k = k0 + j * a.threads;
-        if (k < a.n) y[k] = cscale(v[j], a.scale);
+        if (k < a.n) El::st(y + k, cscale(v[j], a.scale));
       }
     }
   }
@@ -341,9 +366,10 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
 // Rader x[0] slot.  Element i of the workgroup's N1 x C tile is row
 // n1 = i >> log2c, column c = i & (C - 1), so a wavefront reads C-element
 // row segments.
-template <typename V, typename F, int DIR, int MODE>
-__device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* tmp, V* out, V* x0,
+template <typename V, typename F, int DIR, int MODE, typename E>
+__device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* tmp, E* out, V* x0,
                                             int c0, V* lds_in, V* zw = nullptr) {
+  using El = Elem<V, E>;
   constexpr int P = F::kPpt;
   V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c, lc = a.log2c;
@@ -357,9 +383,9 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
               const int n1 = i >> lc, c = i & (C - 1);
               const int n = n1 * N2 + c0 + (c < ncols ? c : 0);
               V v;
-              if constexpr (MODE == kFsPlainA) v = gload(x + n);
-              else if constexpr (MODE == kFsRaderA) v = x[a.perm[n]];
-              else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(x[n], a.chirp[n]) : V{0, 0};
+              if constexpr (MODE == kFsPlainA) v = El::ldnt(x + n);
+              else if constexpr (MODE == kFsRaderA) v = El::ld(x + a.perm[n]);
+              else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(El::ld(x + n), a.chirp[n]) : V{0, 0};
               else v = tmp[n];
               return v;
             },
@@ -368,7 +394,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
               if (c < ncols) lds_all[c * stride + n1 + (n1 >> ps)] = v;
             });
   if constexpr (MODE == kFsRaderA) {
-    if (c0 == 0 && threadIdx.x == 0) *x0 = x[0];
+    if (c0 == 0 && threadIdx.x == 0) *x0 = El::ld(x);
   }
   __syncthreads();
   const bool active = slot < C && slot < ncols;
@@ -399,7 +425,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
                  },
                  [&](int i, int d) {
                    const int k1 = i >> lc, c = i & (C - 1);
-                   if (c < ncols) out[d] = cscale(cadd(lds_all[c * stride + k1 + (k1 >> ps)], xz), a.scale);
+                   if (c < ncols) El::st(out + d, cscale(cadd(lds_all[c * stride + k1 + (k1 >> ps)], xz), a.scale));
                  });
   } else {  // kFsBlueC
     staged<P>(threadIdx.x, blockDim.x, total,
@@ -411,14 +437,15 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const V* x, V* t
                  [&](int i, V ch) {
                    const int k1 = i >> lc, c = i & (C - 1);
                    const int n = k1 * N2 + c0 + c;
-                   if (c < ncols && n < a.n_orig) out[n] = cscale(cmul(lds_all[c * stride + k1 + (k1 >> ps)], ch), a.scale);
+                   if (c < ncols && n < a.n_orig)
+                     El::st(out + n, cscale(cmul(lds_all[c * stride + k1 + (k1 >> ps)], ch), a.scale));
                  });
   }
 }
 
 // Row pass over rows [r0, r0 + C) of one transform's workspace.
-template <typename V, typename F, int DIR, int MODE>
-__device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, const V* x0,
+template <typename V, typename F, int DIR, int MODE, typename E>
+__device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, const V* x0,
                                             int r0, V* lds_in) {
   V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
@@ -437,8 +464,8 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
       if (c >= rows) continue;
-      gstore(out + (long long)(r0 + c) + (long long)N1 * k2,
-             cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale));
+      Elem<V, E>::stnt(out + (long long)(r0 + c) + (long long)N1 * k2,
+                       cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale));
     }
   } else {  // kFsConvB: one row per slot, read and written in place, contiguous
     // forward FFT straight from HBM, kernel multiply as the last pass's sink,
@@ -449,8 +476,8 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, V* out, 
     const bool bin0 = x0 != nullptr && r0 == 0 && slot == 0;
     const V xz = bin0 ? *x0 : V{0, 0};
     F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row},
-                           KernSink<V>{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)(a.kern + k1 * N2),
-                                       bin0 ? (gbl_ptr<V>)out : nullptr, xz, a.scale},
+                           KernSink<V, E>{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)(a.kern + k1 * N2),
+                                          bin0 ? (gbl_ptr<E>)out : nullptr, xz, a.scale},
                            lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     F::template run_io<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps},
                            TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1},

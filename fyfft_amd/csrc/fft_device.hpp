@@ -295,13 +295,14 @@ template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
 // w^2 times w^4), so every factor carries at most three roundings; chains
 // of two or three products measurably raised the transform's error
 // (N = 4096 forward: 1.36e-7 rel-L2 with w^15 = ((w^1 w^2) w^4) w^8, 1.195e-7
-// with the depth-one products).  R = 2..5, and every radix when ALL is set,
-// load every factor: compile-time passes set ALL where the pass's table
-// slice holds at most 256 entries (it stays in the L1 cache, so the extra
-// loads are nearly free: N = 4096, all factors of pass 1, 1.17e-7).
-template <int R, int DIR, bool ALL = false, typename V>
+// with the depth-one products and the split rotation constants of rotc).
+// R = 2..5 load every factor.  Loading all fifteen factors of a radix-16
+// butterfly instead was measured 3-4 % slower at N = 4096 (1.16e-7), even
+// for pass 1, whose 240-entry table slice stays in the L1 cache, and so was
+// loading the factors of passes 1 and 2 before pass 0.
+template <int R, int DIR, typename V>
 __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
-  if constexpr (ALL || R <= 5) {
+  if constexpr (R <= 5) {
 #pragma unroll
     for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], t[r - 1]);
   } else if constexpr (R == 16) {
@@ -328,10 +329,6 @@ __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
     for (int r = 1; r < 7; ++r) v[r] = twmul<DIR>(v[r], w[r]);
   }
 }
-// Compile-time passes of span Ls and radix R load every twiddle factor when
-// the pass's table slice (Ls * (R - 1) entries) is small.
-__host__ __device__ constexpr bool tw_all(int Ls, int R) { return Ls > 1 && Ls * (R - 1) <= 256; }
-
 // ---------------------------------------------------------------- uniform values
 // Copy of a trivially copyable descriptor with every 32-bit word taken from
 // the first active lane: the compiler then keeps it in scalar registers
@@ -414,7 +411,7 @@ template <typename V, int PPT> constexpr bool kNo16 = (sizeof(V) == 8 && PPT <= 
 // place: every read completes (barrier) before any write.  A barrier
 // follows LDS writes so the next pass sees them.  `scale` is applied only
 // on global stores.
-template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G, bool TWALL = false>
+template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G>
 __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int Ls, int T, int tid,
                                               const V* __restrict__ tw, bool active, S<V> scale,
                                               int ps) {
@@ -440,7 +437,7 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
           const int idx = j + r * nb;
           v[b][r] = SRC_G ? gload(src + idx) : src[idx + (idx >> ps)];
         }
-        if (Ls > 1) apply_twiddles<R, DIR, TWALL>(v[b], tw + (Ls - 1) + k * (R - 1));
+        if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
       }
       if (Ls > 1) {
@@ -534,7 +531,7 @@ template <typename V> struct LdsSink {
   __device__ void operator()(int i, V v) const { ((V*)p)[i + (i >> ps)] = v; }
 };
 
-template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst, bool TWALL = false>
+template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
 __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, int tid,
                                         const V* __restrict__ tw, bool active) {
   const int nb = N / R;
@@ -554,7 +551,7 @@ __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, 
       if (active && j < nb) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[b][r] = src(j + r * nb);
-        if (Ls > 1) apply_twiddles<R, DIR, TWALL>(v[b], tw + (Ls - 1) + k * (R - 1));
+        if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
       }
       if (Ls > 1) {
@@ -601,11 +598,10 @@ __device__ __forceinline__ void fixed_io_passes(Src src, Dst dst, V* lds, int ps
   constexpr int MAXB = cdiv(N / R, T);
   const LdsSrc<V> ls{(lds_ptr<V>)lds, ps};
   const LdsSink<V> lk{(lds_ptr<V>)lds, ps};
-  constexpr bool all = tw_all(Ls, R);
-  if constexpr (first && last) io_pass<V, R, MAXB, DIR, Src, Dst, all>(src, dst, N, Ls, T, tid, tw, active);
-  else if constexpr (first) io_pass<V, R, MAXB, DIR, Src, LdsSink<V>, all>(src, lk, N, Ls, T, tid, tw, active);
-  else if constexpr (last) io_pass<V, R, MAXB, DIR, LdsSrc<V>, Dst, all>(ls, dst, N, Ls, T, tid, tw, active);
-  else io_pass<V, R, MAXB, DIR, LdsSrc<V>, LdsSink<V>, all>(ls, lk, N, Ls, T, tid, tw, active);
+  if constexpr (first && last) io_pass<V, R, MAXB, DIR>(src, dst, N, Ls, T, tid, tw, active);
+  else if constexpr (first) io_pass<V, R, MAXB, DIR>(src, lk, N, Ls, T, tid, tw, active);
+  else if constexpr (last) io_pass<V, R, MAXB, DIR>(ls, dst, N, Ls, T, tid, tw, active);
+  else io_pass<V, R, MAXB, DIR>(ls, lk, N, Ls, T, tid, tw, active);
   if constexpr (!last) fixed_io_passes<V, DIR, N, T, Ls * R, Src, Dst, Rest...>(src, dst, lds, ps, tid, tw, active);
 }
 

@@ -298,21 +298,38 @@ template <typename V> struct GroupedArgs {
   const GLen<V>* lens = nullptr;
   const long long* off = nullptr;   // element offset of each transform (list order)
   const long long* woff = nullptr;  // workspace offset of each transform (list order)
-  const V* in = nullptr;
-  V* out = nullptr;
+  const void* in = nullptr;         // storage elements: V, or 16-bit pairs for BF16 / FP16 plans
+  void* out = nullptr;
   V* ws = nullptr;
   V* x0buf = nullptr;
   int tables = 0;     // 0 forward, 1 inverse/adjoint tables
   int normalize = 0;  // scale by 1/n
 };
+// Task families of the grouped kernels (grouped.hpp): the three LDS-resident
+// kinds and the four-step passes.  Phase 0: kGfLds .. kGfColBlueA, phase 1:
+// the row passes, phase 2: the closing column passes.
+enum GroupFamily {
+  kGfLds = 0, kGfRader = 1, kGfBlue = 2, kGfColPlain = 3, kGfColRaderA = 4, kGfColBlueA = 5,
+  kGfRowPlain = 6, kGfRowConv = 7, kGfColRaderC = 8, kGfColBlueC = 9, kGroupFamilies = 10
+};
+inline int family_phase(int fam) { return fam <= kGfColBlueA ? 0 : (fam <= kGfRowConv ? 1 : 2); }
 // Grouped launches come in workgroup-size classes of 256, 512 and 1024
 // threads with LDS budgets of 1/4, 1/2 and all of a CU's LDS.
 constexpr int kGroupClasses = 3;
 constexpr int kGroupClassThreads[kGroupClasses] = {256, 512, 1024};
-hipError_t launch_grouped(int prec, int dir, const void* args, unsigned grid, int block, size_t lds,
-                          hipStream_t s);
-// Sets the dynamic-LDS limits of the grouped kernels per class (call at plan time).
-hipError_t prepare_grouped(int prec, const size_t lds[kGroupClasses]);
+// One family kernel of a grouped plan: storage kF32 / kF64 (data in the
+// compute type) or kBF16 / kF16 (FP32 arithmetic, 16-bit data); hi: the
+// doubled points-per-thread bound (LDS families at 1024 threads only);
+// `args` points at GroupedArgs<float2> or GroupedArgs<double2>.  With
+// prepare, only sets the kernel's dynamic-LDS limit (plan time, so the
+// launch path holds nothing but launches and can be captured in a graph).
+hipError_t group_family(int storage, bool prepare, int fam, int dir, bool hi, int block, const void* args,
+                        unsigned grid, size_t lds, hipStream_t s);
+hipError_t group_family_lowp(bool bf16, bool prepare, int fam, int dir, bool hi, int block, const void* args,
+                             unsigned grid, size_t lds, hipStream_t s);
+// n complex values between 16-bit storage and complex float (to_f32: from
+// 16-bit); the per-transform baseline of 16-bit grouped plans.
+hipError_t launch_convert16(bool bf16, bool to_f32, const void* src, void* dst, long long n, hipStream_t s);
 
 // --- kernels_lowp.hip
 // BF16/FP16 storage with FP32 arithmetic for the lengths of the FP32

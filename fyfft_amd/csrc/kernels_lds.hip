@@ -74,7 +74,7 @@ __device__ __forceinline__ void run_fixed(const V* gin, V* lds, V* gout, int tid
   constexpr int MAXB = cdiv(N / R, T);
   const V* src = first ? gin : lds;
   V* dst = last ? gout : lds;
-  stockham_pass<V, R, MAXB, DIR, first, last, tw_all(Ls, R)>(src, dst, N, Ls, T, tid, tw, active, scale, PS);
+  stockham_pass<V, R, MAXB, DIR, first, last>(src, dst, N, Ls, T, tid, tw, active, scale, PS);
   if constexpr (!last) run_fixed<V, DIR, N, T, PS, Ls * R, Rest...>(gin, lds, gout, tid, tw, active, scale);
 }
 

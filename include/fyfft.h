@@ -125,6 +125,32 @@ typedef struct fyfft_plan fyfft_plan;
 fyfft_status fyfft_plan_create(fyfft_plan** plan, size_t n, size_t batch,
                                fyfft_precision precision);
 
+/* Replaces the reference's per-strategy invokers' constructors
+ * (mix_radix::FFT1D_mixed_radix_invoker, rader::FFT1D_C2C_rader_invoker,
+ * bluestein::FFT1D_bluestein_invoker, dft::FFT1D_C2C_dft_invoker:
+ * foye_FFT.hpp:81-84; mix_radix.hpp:645, rader.hpp:339, bluestein.hpp:217,
+ * dft.hpp:106): a plan of the given fyfft_strategy.
+ *   FYFFT_BLUESTEIN: a Bluestein convolution for any N > 1;
+ *   FYFFT_RADER: a Rader convolution when N is prime and N-1 is
+ *     2/3/5/7-smooth; other primes run the library's choice (Bluestein), as
+ *     this library has no zero-padded Rader (rader.hpp:166);
+ *   FYFFT_MIX_RADIX / FYFFT_KERNEL_DIRECT: N must be 2/3/5/7-smooth / one of
+ *     the reference's 38 direct lengths (FYFFT_UNSUPPORTED otherwise), then
+ *     as fyfft_plan_create;
+ *   FYFFT_DFT: the library's choice (the transform is the same DFT; no
+ *     O(N^2) tables are built).
+ * fyfft_plan_strategy reports what the plan runs.  The results equal those
+ * of fyfft_plan_create up to rounding. */
+fyfft_status fyfft_plan_create_strategy(fyfft_plan** plan, size_t n, size_t batch,
+                                        fyfft_precision precision, int strategy);
+
+/* The reference's eligibility predicate of each strategy invoker
+ * (is_processable: radix_wrap.hpp:93-98 for KERNEL_DIRECT,
+ * mix_radix.hpp:618-643, rader.hpp:341-345 + 403-423,
+ * bluestein.hpp:212-215; DFT accepts every N >= 1), so caller code that
+ * branches on it behaves as with the reference.  No GPU needed. */
+fyfft_status fyfft_strategy_processable(int strategy, size_t n, int* processable);
+
 /* Strided batches (no counterpart in the reference, which transforms one
  * contiguous array per call): `howmany` transforms of length n; element j
  * of transform b is read from in[b*idist + j*istride] and written to
@@ -174,13 +200,18 @@ fyfft_status fyfft_plan_create_real(fyfft_plan** plan, size_t n, size_t batch,
 /* Grouped ragged plans (no counterpart in the reference, which plans one
  * length at a time): `count` transforms of lengths lengths[0..count-1],
  * packed back to back (transform i starts at element sum_{j<i} lengths[j]),
- * executed by fyfft_execute as ONE persistent grouped launch per phase
- * (up to three phases) that reads a device-side task table.
+ * executed by fyfft_execute as persistent grouped launches that read a
+ * device-side task table: per phase (LDS-resident transforms and first
+ * four-step passes; row passes; closing passes of four-step Rader and
+ * Bluestein), one launch per task family present, the families of a phase
+ * running as parallel branches.  FYFFT_BF16 / FYFFT_F16 store the data as
+ * 16-bit pairs and compute in FP32 (one rounding at the final store).
  *   FYFFT_GROUPED_GRAPH: capture the phases into a hipGraph the first time
  *     a (in, out, direction, stream) combination runs on a non-null stream,
  *     and replay the graph afterwards;
  *   FYFFT_GROUPED_PER_TRANSFORM: the baseline form -- one launch of the
- *     plain (non-specialised) kernels per transform. */
+ *     plain (non-specialised) kernels per transform (16-bit plans widen
+ *     each transform into an FP32 copy around it). */
 #define FYFFT_GROUPED_GRAPH 1u
 #define FYFFT_GROUPED_PER_TRANSFORM 2u
 fyfft_status fyfft_plan_create_grouped(fyfft_plan** plan, const size_t* lengths, size_t count,

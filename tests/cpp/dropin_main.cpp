@@ -52,6 +52,43 @@ int main(int argc, char** argv) {
   try { FFT1D_C2C_invoker bad(1); } catch (const std::invalid_argument&) { threw = true; }
   if (!threw) { std::fprintf(stderr, "no exception for N=1\n"); return 1; }
   std::printf("strategy %d\n", (int)FFT1D_C2C_invoker::acquired_execution_strategy(n));
+  // the reference's per-strategy invokers (foye_FFT.hpp:81-84), each used
+  // through the base class where its is_processable accepts N: forward and
+  // backward must agree with dispatch<false> / dispatch<true>
+  auto close_to = [](const std::vector<float>& a, const std::vector<float>& b) {
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size(); ++i) {
+      num += (double)(a[i] - b[i]) * (a[i] - b[i]);
+      den += (double)b[i] * b[i];
+    }
+    return num <= 1e-10 * den;  // rel-L2 <= 1e-5
+  };
+  int used = 0;
+  auto agree = [&](fy::fft::basic_unsmooth_path_invoker& base, const char* name) {
+    std::vector<float> f(2 * n), b(2 * n);
+    base.forward(in.data(), f.data());
+    base.backward(in.data(), b.data());
+    if (!close_to(f, fwd) || !close_to(b, inv)) {
+      std::fprintf(stderr, "%s invoker disagrees with dispatch\n", name);
+      std::exit(1);
+    }
+    ++used;
+  };
+  if (fy::fft::FFT1D_mixed_radix_invoker::is_processable(n)) {
+    fy::fft::FFT1D_mixed_radix_invoker m(n);
+    agree(m, "mixed-radix");
+  }
+  if (fy::fft::FFT1D_C2C_rader_invoker::is_processable(n)) {
+    fy::fft::FFT1D_C2C_rader_invoker r(n);
+    agree(r, "rader");
+  }
+  if (fy::fft::FFT1D_bluestein_invoker::is_processable(n)) {
+    fy::fft::FFT1D_bluestein_invoker b(n);
+    agree(b, "bluestein");
+  }
+  fy::fft::FFT1D_C2C_dft_invoker d(n);
+  agree(d, "dft");
+  std::printf("invokers %d\n", used);
   if (!write_all(argv[3], fwd) || !write_all(argv[4], inv)) return 2;
   return 0;
 }

@@ -150,3 +150,25 @@ def test_mix_modes_argument_checks_without_gpu(fy):
     assert L.fyfft_mix_modes(0, 0, null, null, null, 1, 4, 4, 4, null) == 1      # null operands
     assert L.fyfft_mix_modes(0, 0, ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8),
                              70000, 4, 4, 4, null) == 1                          # k > 65535
+
+
+def test_strategy_processable_matches_reference_predicates(orc):
+    """fyfft_strategy_processable answers the reference's own is_processable
+    predicates, restated in the oracle (radix_wrap.hpp:93-98,
+    mix_radix.hpp:618-643, rader.hpp:403-423, bluestein.hpp:212-215)."""
+    import ctypes
+    import fyfft_amd as fy
+    L = fy.lib()
+    L.fyfft_strategy_processable.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+    ok = ctypes.c_int()
+
+    def q(strategy, n):
+        assert L.fyfft_strategy_processable(strategy, n, ctypes.byref(ok)) == 0
+        return bool(ok.value)
+    for n in list(range(2, 2000)) + [6720, 10006, 10007, 39041, 39061, 39062, 39063, 39064, 39067, 65537, 78125]:
+        assert q(0, n) == orc.is_directly_supported(n), n
+        assert q(1, n) == orc.mix_is_processable(n), n
+        assert q(2, n) == orc.rader_is_processable(n), n
+        assert q(3, n) == orc.bluestein_is_processable(n), n
+        assert q(4, n)
+    assert L.fyfft_strategy_processable(9, 16, ctypes.byref(ok)) == 1

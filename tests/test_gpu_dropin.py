@@ -14,9 +14,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "fyfft_amd", "lib", "dropin_main")
 
 
-@pytest.mark.parametrize("n,strategy", [(1024, 0), (4096, 0), (6720, 1), (17, 2), (65537, 2),
-                                        (10007, 3), (22, 3)])
-def test_dropin_header(orc, tmp_path, n, strategy):
+# invokers: how many of the reference's per-strategy invokers accept N
+# (is_processable, the reference's predicates): mixed-radix for 7-smooth N,
+# Rader for primes up to 39064 outside the direct table, Bluestein up to
+# 39062, DFT always
+@pytest.mark.parametrize("n,strategy,invokers", [(1024, 0, 3), (4096, 0, 3), (6720, 1, 3), (17, 2, 3),
+                                                 (65537, 2, 1), (10007, 3, 3), (22, 3, 2)])
+def test_dropin_header(orc, tmp_path, n, strategy, invokers):
     if n == 1024:
         x = np.load(os.path.join(ROOT, "tests", "golden", "config1_n1024.npz"))["x"]
     else:
@@ -28,6 +32,7 @@ def test_dropin_header(orc, tmp_path, n, strategy):
                        text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert f"strategy {strategy}" in r.stdout
+    assert f"invokers {invokers}" in r.stdout, r.stdout
     fwd = np.fromfile(ffwd, np.float32).view(np.complex64)
     inv = np.fromfile(finv, np.float32).view(np.complex64)
     x64 = x.astype(np.complex128)

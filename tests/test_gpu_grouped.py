@@ -23,8 +23,11 @@ def segments(arr, lengths):
         off += n
 
 
+# every kind of member: LDS smooth / Rader / Bluestein, four-step smooth,
+# Rader and Bluestein, up to the configuration's 2^20 and a 1000003 prime
 MIXED = [1, 2, 3, 16, 17, 100, 105, 243, 1024, 4096, 4096, 4096, 6720, 10007, 12289, 16384,
-         16807, 23, 65536, 65537, 97, 140000, 100003, 2, 2, 7, 121, 1031, 4096, 30011]
+         16807, 23, 65536, 65537, 97, 140000, 100003, 2, 2, 7, 121, 1031, 4096, 30011,
+         1 << 20, 1000003]
 
 
 @pytest.mark.parametrize("direction", ["forward", "inverse", "adjoint"])
@@ -102,3 +105,47 @@ def test_grouped_graph_replay(fy, cuda):
             graph_p.execute(y, z, fy.INVERSE, stream=s)
         s.synchronize()
     assert float((z - x).norm() / x.norm()) < 1e-6
+
+
+# 16-bit storage: the same members; the bar is the FP64 transform of the
+# 16-bit input values, rounded once to the storage type (the internal error
+# is at FP32 level, far below one 16-bit ulp)
+LOWP = [16, 100, 243, 4096, 6720, 97, 12289, 23, 1031, 65536, 140000, 65537, 100003, 1 << 20, 1000003]
+
+
+@pytest.mark.parametrize("dtype_name", ["bf16", "fp16"])
+@pytest.mark.parametrize("direction", ["forward", "inverse"])
+def test_grouped_lowp_at_rounding_floor(fy, orc, cuda, dtype_name, direction):
+    import torch
+    tdt = torch.bfloat16 if dtype_name == "bf16" else torch.float16
+    prec = fy.BF16 if dtype_name == "bf16" else fy.F16
+    d = fy.FORWARD if direction == "forward" else fy.INVERSE
+    tot = sum(LOWP)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    x = torch.randn(tot, 2, device="cuda", generator=g).to(tdt)
+    print(f"[grouped lowp] {dtype_name} {direction}", flush=True)
+    with fy.GroupedPlan(LOWP, prec) as p, fy.GroupedPlan(LOWP, prec, fy.GROUPED_PER_TRANSFORM) as b:
+        assert f"storage={dtype_name}" in p.describe(), p.describe()
+        y = p.execute(x, None, d)
+        yb = b.execute(x, None, d)
+    torch.cuda.synchronize()
+    xq = x.float().cpu().numpy().astype(np.float64)
+    xc = xq[:, 0] + 1j * xq[:, 1]
+    yh = y.cpu().double().numpy()
+    ybh = yb.cpu().double().numpy()
+    off, bad, total = 0, 0, 0
+    for n in LOWP:
+        ex = orc.fft_ext(xc[off:off + n], inverse=d != fy.FORWARD)
+        ext = np.stack([ex.real, ex.imag], -1)
+        rounded = torch.from_numpy(ext).to(tdt).double().numpy()
+        seg = yh[off:off + n]
+        err = np.linalg.norm(seg - ext) / np.linalg.norm(ext)
+        floor = np.linalg.norm(rounded - ext) / np.linalg.norm(ext)
+        bad += int((seg != rounded).sum())
+        total += seg.size
+        assert err < 1.02 * floor + 1e-6, (n, err, floor)
+        # the baseline (FP32 plain kernels between widening and rounding copies)
+        errb = np.linalg.norm(ybh[off:off + n] - ext) / np.linalg.norm(ext)
+        assert errb < 1.02 * floor + 1e-6, (n, errb, floor)
+        off += n
+    assert bad / total < (2e-3 if dtype_name == "bf16" else 1e-2), bad / total

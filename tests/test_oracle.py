@@ -216,3 +216,32 @@ def test_reference_input_stream_pinned(orc):
     # first values of std::mt19937(0) (known-answer): 2357136044, 2546248239
     raw = np.random.RandomState(0)._bit_generator.random_raw(2)
     assert list(raw) == [2357136044, 2546248239]
+
+
+def test_root_cache_survives_many_lengths(orc):
+    """Root tables fetched early in one transform stay valid to its end,
+    whatever the per-thread table cache holds.  Each case runs in a fresh
+    thread (fresh thread-local cache): N forward, then k transforms of other
+    lengths (one new table each), then N inverse, for every ring position k.
+    A cache that evicts the least recently inserted entry freed tables the
+    inverse had just fetched (history-dependent wrong results, first seen as
+    N = 140000 inverse after the grouped tests)."""
+    import threading
+    rng = np.random.default_rng(5)
+    fillers = [2 * k + 1 for k in range(60, 120) if all((2 * k + 1) % p for p in (3, 5, 7))]
+    for n in (10007, 4099):
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        ref = np.fft.ifft(x)
+        errs = {}
+
+        def case(k):
+            orc.fft_ext(x)
+            for m in fillers[:k]:
+                orc.fft_ext(np.ones(m, np.complex128))
+            errs[k] = orc.rel_l2(orc.fft_ext(x, inverse=True), ref)
+        for k in range(0, 40):
+            t = threading.Thread(target=case, args=(k,))
+            t.start()
+            t.join()
+        bad = {k: e for k, e in errs.items() if not e < 1e-13}
+        assert not bad and len(errs) == 40, (n, bad)
