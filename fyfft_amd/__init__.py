@@ -599,6 +599,7 @@ def irfft(X, n: int, stream=None):
 
 GROUPED_GRAPH = 1
 GROUPED_PER_TRANSFORM = 2
+GROUPED_TUNED = 4
 
 
 class GroupedPlan(Plan):

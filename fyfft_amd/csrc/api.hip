@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <map>
 #include <tuple>
 #include <cstdio>
@@ -66,6 +68,13 @@ constexpr int kGroupBranches = 4;
 // Forms of a real (R2C / C2R) plan.
 enum RealForm { kRealFusedHalf = 0, kRealFusedFull = 1, kRealHalfPasses = 2, kRealFullPasses = 3 };
 
+// Tuning time budget of FYFFT_GROUPED_TUNED plans (seconds of plan time).
+constexpr double kGroupedTuneSeconds = 20.0;
+
+// Internal plan form: TUNED restricted to what the runtime-pass kernels run
+// (sub-plans of tuned grouped plans, which never launch compile-time kernels).
+constexpr int kFormTunedGeneric = 16;
+
 // Workspace of multi-pass plans is capped; larger batches run in chunks.
 constexpr size_t kWorkspaceCap = (size_t)1 << 30;
 // Longest factor of a four-step split (one column/row FFT in LDS).
@@ -88,10 +97,21 @@ fyfft_status from_hip(hipError_t e, const char* what) {
 
 }  // namespace
 
+// A four-step structure choice: the split n = n1 x (n / n1) and the number
+// of sub-transforms per workgroup of the column and row passes.
+struct FsChoice {
+  size_t n1 = 0;
+  int ccol = 0, crow = 0;
+};
+
 struct fyfft_plan {
   uint32_t magic = kMagic;
   size_t n = 0, batch = 0;
   int prec = kF32;
+  int form = FYFFT_FORM_AUTO;
+  // tuning overrides for the structure being built (autotuner candidates)
+  const FsChoice* force_fs = nullptr;
+  size_t force_m = 0;
   int strategy = kDirect;
   int device = 0;
   Kind kind = kTrivial;
@@ -234,7 +254,7 @@ static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_f
 // Geometry of a four-step pass whose LDS FFT has length len: up to 16
 // sub-transforms (columns or rows) per workgroup, and at least two
 // workgroups per CU where the LDS allows.
-static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g) {
+static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int force_c = 0) {
   if (!lds_geometry(len, elem, cap, g)) return false;
   const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
   // 16 sub-transforms per workgroup make 128-byte row segments for FP32
@@ -243,6 +263,10 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g) {
   // against 16 at one 1.36 ms for 128 transforms, FP64 2^18 1.76 vs 2.06.
   int c = 16;
   while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
+  if (force_c > 0) {  // an autotuner candidate: any power of two that fits
+    if ((force_c & (force_c - 1)) || (size_t)force_c * g->threads > 1024) return false;
+    c = force_c;
+  }
   if ((size_t)c * stride * elem > cap) return false;
   g->per_block = c;
   g->lds_bytes = (size_t)c * stride * elem;
@@ -269,8 +293,17 @@ static bool fs_possible(size_t n, size_t elem, size_t cap) {
 // Four-step structure for smooth n (tables only; workspace separately).
 static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
   const size_t cap = lds_cap_bytes(p->device), elem = elem_bytes(p->prec);
-  if (!fs_split(n, elem, conv, &p->n1, &p->n2)) return FYFFT_UNSUPPORTED;
-  if (!fs_geometry(p->n1, elem, cap, &p->gcol) || !fs_geometry(p->n2, elem, cap, &p->grow))
+  const FsChoice* f = p->force_fs;
+  if (f) {
+    if (f->n1 < 2 || n % f->n1 || n / f->n1 < 2 || !is_smooth7(f->n1) || !is_smooth7(n / f->n1))
+      return FYFFT_UNSUPPORTED;
+    p->n1 = f->n1;
+    p->n2 = n / f->n1;
+  } else if (!fs_split(n, elem, conv, &p->n1, &p->n2)) {
+    return FYFFT_UNSUPPORTED;
+  }
+  if (!fs_geometry(p->n1, elem, cap, &p->gcol, f ? f->ccol : 0) ||
+      !fs_geometry(p->n2, elem, cap, &p->grow, f ? f->crow : 0))
     return FYFFT_UNSUPPORTED;
   fyfft_status st = make_pass_table(p, p->n1, &p->pdcol, &p->twcol);
   if (!st) st = make_pass_table(p, p->n2, &p->pdrow, &p->twrow);
@@ -299,20 +332,51 @@ static fyfft_status alloc_workspace(fyfft_plan* p, size_t len, bool x0) {
 // plan of this library (used to build kernel spectra from FP64 tables).
 
 // ------------------------------------------------------------ autotuner
-// FYFFT_FORM_TUNED plans of LDS-resident smooth lengths time a set of
-// candidate configurations on the device at plan time and keep the fastest:
-// the compile-time kernel when one exists, and the generic kernel over
-// several radix orders (power-of-two split and placement of the odd
-// radices), LDS padding on/off and two points-per-thread classes, each with
-// its own FP64 twiddle table.  Choices are remembered per (N, precision) as
-// "wisdom", which can be exported and imported as text.
+// FYFFT_FORM_TUNED plans time candidate configurations on the device at plan
+// time and keep the fastest:
+//   * LDS-resident smooth lengths: the compile-time kernel when one exists,
+//     and the generic kernel over several radix orders (power-of-two split
+//     and placement of the odd radices), LDS padding on/off and two
+//     points-per-thread classes, each with its own FP64 twiddle table;
+//   * four-step lengths (smooth N beyond one workgroup's LDS, and the
+//     convolution length of four-step Rader plans): the split N = n1 x n2
+//     over the plannable divisor pairs nearest the default one, and the
+//     sub-transforms per workgroup of the column and row passes;
+//   * Bluestein: the convolution length M among the cheapest candidates of
+//     the length cost model.
+// Choices are remembered as wisdom per (device, N, precision): the device
+// is its architecture and compute-unit count, so wisdom tuned on one GPU
+// model is never applied on another.
 struct LdsChoice {
   bool fixed = false;
   int ppt = 16, ps = 4, tpb = 1;
   std::vector<int> rad;
 };
+enum WisdomFs { kWfSmooth = 0, kWfRader = 1 };
 static std::mutex g_wisdom_mu;
-static std::map<std::pair<size_t, int>, LdsChoice> g_wisdom;
+// (device, N, prec, generic-only)
+static std::map<std::tuple<std::string, size_t, int, int>, LdsChoice> g_wisdom;
+static std::map<std::tuple<std::string, size_t, int, int>, FsChoice> g_wisdom_fs;      // (device, N, prec, kind)
+static std::map<std::tuple<std::string, size_t, int>, size_t> g_wisdom_m;             // Bluestein M
+
+// "gfx950 256": the architecture (without target features) and CU count of
+// a device, the identity wisdom is keyed by.
+static std::string device_identity(int dev) {
+  static std::mutex mu;
+  static std::map<int, std::string> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  hipDeviceProp_t prop;
+  std::string id = "unknown 0";
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    id = arch + " " + std::to_string(prop.multiProcessorCount);
+  }
+  cache[dev] = id;
+  return id;
+}
 
 static std::vector<std::vector<int>> radix_orders(size_t n) {
   const std::vector<int> base = radices_for(n);
@@ -421,8 +485,56 @@ static std::vector<LdsChoice> lds_candidates(fyfft_plan* p, size_t n) {
   return out;
 }
 
-static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n) {
-  const std::pair<size_t, int> key{n, p->prec};
+// Device buffers, stream and events of one tuning session: in/out of
+// `elems` elements (the input zeroed: timing does not depend on the values).
+struct TuneBench {
+  void *in = nullptr, *out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t err = hipSuccess;
+  TuneBench(size_t bytes) {
+    err = hipMalloc(&in, bytes);
+    if (err == hipSuccess) err = hipMalloc(&out, bytes);
+    if (err == hipSuccess) err = hipMemset(in, 0, bytes);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+  }
+  ~TuneBench() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
+  }
+  // ms per call of f(stream): the best of three groups of three calls,
+  // after two warm-up calls
+  template <typename F> float time(F f) {
+    for (int i = 0; i < 2 && err == hipSuccess; ++i) err = f(st);
+    float best = 1e30f;
+    for (int g = 0; g < 3 && err == hipSuccess; ++g) {
+      err = hipEventRecord(e0, st);
+      for (int i = 0; i < 3 && err == hipSuccess; ++i) err = f(st);
+      if (err == hipSuccess) err = hipEventRecord(e1, st);
+      if (err == hipSuccess) err = hipEventSynchronize(e1);
+      float ms = 1e30f;
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+      if (err == hipSuccess) best = std::min(best, ms / 3);
+    }
+    return err == hipSuccess ? best : 1e30f;
+  }
+};
+
+// Transforms per tuning launch: the plan's batch, up to 256 MB of data (long
+// enough that launch gaps do not decide between candidates).
+static size_t tune_batch(size_t batch, size_t n, size_t elem) {
+  return std::max<size_t>(1, std::min<size_t>(batch, ((size_t)256 << 20) / (n * elem)));
+}
+
+// generic_only: candidates the runtime-pass kernels can run (sub-plans of
+// grouped plans, which never launch the compile-time kernels)
+static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n, bool generic_only = false) {
+  const auto key = std::make_tuple(device_identity(p->device), n, p->prec, generic_only ? 1 : 0);
   LdsChoice best;
   bool have = false;
   {
@@ -432,30 +544,18 @@ static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n) {
   }
   if (!have) {
     const size_t elem = elem_bytes(p->prec);
-    const size_t b = std::max<size_t>(1, std::min<size_t>(p->batch, ((size_t)64 << 20) / (n * elem)));
-    void *in = nullptr, *out = nullptr;
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIP_TRY(hipMalloc(&in, b * n * elem));
-    hipError_t e = hipMalloc(&out, b * n * elem);
-    if (e == hipSuccess) e = hipMemset(in, 0, b * n * elem);
-    if (e == hipSuccess) e = hipStreamCreate(&st);
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
+    const size_t b = tune_batch(p->batch, n, elem);
+    TuneBench tb(b * n * elem);
     float best_ms = 1e30f;
     for (const LdsChoice& c : lds_candidates(p, n)) {
-      if (e != hipSuccess) break;
+      if (tb.err != hipSuccess) break;
+      if (generic_only && c.fixed) continue;
       LdsLaunch L;
       void* tw = nullptr;
       if (make_candidate(p, n, c, &L, &tw) == FYFFT_SUCCESS) {
-        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = launch_lds(p->prec, kForward, L, in, out, b, n, 1.0, st);
-        if (e == hipSuccess) e = hipEventRecord(e0, st);
-        for (int i = 0; i < 5 && e == hipSuccess; ++i) e = launch_lds(p->prec, kForward, L, in, out, b, n, 1.0, st);
-        if (e == hipSuccess) e = hipEventRecord(e1, st);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e == hipSuccess && ms < best_ms) {
+        const float ms =
+            tb.time([&](hipStream_t st) { return launch_lds(p->prec, kForward, L, tb.in, tb.out, b, n, 1.0, st); });
+        if (ms < best_ms) {
           best_ms = ms;
           best = c;
           have = true;
@@ -463,12 +563,7 @@ static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n) {
       }
       if (tw) (void)hipFree(tw);
     }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    if (st) (void)hipStreamDestroy(st);
-    (void)hipFree(in);
-    if (out) (void)hipFree(out);
-    if (e != hipSuccess) return from_hip(e, "autotune");
+    if (tb.err != hipSuccess) return from_hip(tb.err, "autotune");
     if (!have) return FYFFT_UNSUPPORTED;
     std::lock_guard<std::mutex> lock(g_wisdom_mu);
     g_wisdom[key] = best;
@@ -479,28 +574,197 @@ static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n) {
     p->allocs.push_back(tw);
     p->dev_bytes += std::max<size_t>(n, 2) * elem_bytes(p->prec);
   }
+  if (!st2 && generic_only) p->lds.fixed = -1;
   return st2;
+}
+
+// Four-step structure candidates for length L: the default split, the
+// plannable divisor pairs nearest to it, and for each split the default and
+// halved sub-transforms per workgroup of both passes.
+static std::vector<FsChoice> fs_candidates(const fyfft_plan* p, size_t L, bool conv) {
+  const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
+  std::vector<FsChoice> out;
+  size_t d1 = 0, d2 = 0;
+  if (!fs_split(L, elem, conv, &d1, &d2)) return out;
+  std::vector<size_t> splits = {d1};
+  std::vector<std::pair<double, size_t>> others;
+  for (size_t d = 2; d * 2 <= L; ++d) {
+    if (L % d || d == d1) continue;
+    LdsGeom a, b;
+    if (!is_smooth7(d) || !is_smooth7(L / d) || !fs_geometry(d, elem, cap, &a) || !fs_geometry(L / d, elem, cap, &b))
+      continue;
+    others.push_back({std::fabs(std::log((double)d / (double)d1)), d});
+  }
+  std::sort(others.begin(), others.end());
+  for (size_t i = 0; i < others.size() && splits.size() < 5; ++i) splits.push_back(others[i].second);
+  for (size_t n1 : splits) {
+    LdsGeom a, b;
+    if (!fs_geometry(n1, elem, cap, &a) || !fs_geometry(L / n1, elem, cap, &b)) continue;
+    for (int ca : {a.per_block, a.per_block / 2})
+      for (int cb : {b.per_block, b.per_block / 2}) {
+        if (ca < 1 || cb < 1) continue;
+        FsChoice c;
+        c.n1 = n1;
+        c.ccol = ca;
+        c.crow = cb;
+        out.push_back(c);
+      }
+  }
+  return out;
+}
+
+static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, int, int force = -1,
+                                     const FsChoice* force_fs = nullptr, size_t force_m = 0);
+
+static fyfft_precision api_precision(int prec) {
+  return prec == kF64 ? FYFFT_F64 : (prec == kBF16 ? FYFFT_BF16 : (prec == kF16 ? FYFFT_F16 : FYFFT_F32));
+}
+
+// The fastest four-step structure for the plan's length (kind: a smooth
+// length, or the convolution of a Rader plan), from wisdom or by timing a
+// candidate plan per structure on a batch of up to 256 MB.
+static fyfft_status tune_fourstep(fyfft_plan* p, size_t L, int kind, FsChoice* best) {
+  const auto key = std::make_tuple(device_identity(p->device), p->n, p->prec, kind);
+  {
+    std::lock_guard<std::mutex> lock(g_wisdom_mu);
+    auto it = g_wisdom_fs.find(key);
+    if (it != g_wisdom_fs.end()) {
+      *best = it->second;
+      return FYFFT_SUCCESS;
+    }
+  }
+  const std::vector<FsChoice> cands = fs_candidates(p, L, kind == kWfRader);
+  if (cands.empty()) return FYFFT_UNSUPPORTED;
+  const size_t elem = elem_bytes(p->prec), b = tune_batch(p->batch, p->n, elem);
+  TuneBench tb(b * p->n * elem);
+  float best_ms = 1e30f;
+  bool have = false;
+  for (const FsChoice& c : cands) {
+    if (tb.err != hipSuccess) break;
+    fyfft_plan* q = nullptr;
+    if (create_plan_impl(&q, p->n, b, api_precision(p->prec), FYFFT_FORM_AUTO, kind == kWfRader ? kRader : -1, &c) !=
+        FYFFT_SUCCESS)
+      continue;
+    const float ms = tb.time([&](hipStream_t st) {
+      return fyfft_execute(q, tb.in, tb.out, FYFFT_FORWARD, st) == FYFFT_SUCCESS ? hipSuccess : hipErrorUnknown;
+    });
+    fyfft_plan_destroy(q);
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = c;
+      have = true;
+    }
+  }
+  if (tb.err != hipSuccess) return from_hip(tb.err, "autotune four-step");
+  if (!have) return FYFFT_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_wisdom_mu);
+  g_wisdom_fs[key] = *best;
+  return FYFFT_SUCCESS;
+}
+
+static double conv_length_cost(size_t c, size_t elem, size_t cap, int prec);
+
+// Bluestein convolution lengths worth timing: the cheapest few of the
+// length cost model in [2N-1, 1.25 (2N-1)] (the model's own choice first).
+static std::vector<size_t> bluestein_candidates(const fyfft_plan* p, size_t n, size_t count) {
+  const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
+  std::vector<std::pair<double, size_t>> c;
+  const size_t lo = 2 * n - 1, hi = lo + lo / 4;
+  for (size_t m = lo; m <= hi; ++m) {
+    const double cost = conv_length_cost(m, elem, cap, p->prec);
+    if (cost > 0) c.push_back({cost, m});
+  }
+  std::sort(c.begin(), c.end());
+  std::vector<size_t> out;
+  for (size_t i = 0; i < c.size() && out.size() < count; ++i) out.push_back(c[i].second);
+  return out;
+}
+
+static fyfft_status tune_bluestein(fyfft_plan* p, size_t* best_m) {
+  const auto key = std::make_tuple(device_identity(p->device), p->n, p->prec);
+  {
+    std::lock_guard<std::mutex> lock(g_wisdom_mu);
+    auto it = g_wisdom_m.find(key);
+    if (it != g_wisdom_m.end()) {
+      *best_m = it->second;
+      return FYFFT_SUCCESS;
+    }
+  }
+  const std::vector<size_t> cands = bluestein_candidates(p, p->n, 4);
+  if (cands.size() < 2) {
+    *best_m = 0;  // nothing to choose: the cost model's length
+    return FYFFT_SUCCESS;
+  }
+  const size_t elem = elem_bytes(p->prec), b = tune_batch(p->batch, p->n, elem);
+  TuneBench tb(b * p->n * elem);
+  float best_ms = 1e30f;
+  for (size_t m : cands) {
+    if (tb.err != hipSuccess) break;
+    fyfft_plan* q = nullptr;
+    if (create_plan_impl(&q, p->n, b, api_precision(p->prec), FYFFT_FORM_AUTO, kBluestein, nullptr, m) !=
+        FYFFT_SUCCESS)
+      continue;
+    const float ms = tb.time([&](hipStream_t st) {
+      return fyfft_execute(q, tb.in, tb.out, FYFFT_FORWARD, st) == FYFFT_SUCCESS ? hipSuccess : hipErrorUnknown;
+    });
+    fyfft_plan_destroy(q);
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best_m = m;
+    }
+  }
+  if (tb.err != hipSuccess) return from_hip(tb.err, "autotune bluestein");
+  std::lock_guard<std::mutex> lock(g_wisdom_mu);
+  g_wisdom_m[key] = *best_m;
+  return FYFFT_SUCCESS;
 }
 
 static std::string wisdom_text() {
   std::lock_guard<std::mutex> lock(g_wisdom_mu);
-  std::string s = "fyfft-wisdom 1\n";
+  std::map<std::string, std::string> per_dev;
   for (const auto& kv : g_wisdom) {
     const LdsChoice& c = kv.second;
-    s += "lds " + std::to_string(kv.first.first) + " " + std::to_string(kv.first.second) + " " +
-         std::to_string(c.fixed ? 1 : 0) + " " + std::to_string(c.ppt) + " " + std::to_string(c.ps) +
-         " " + std::to_string(c.tpb) + " ";
-    for (size_t i = 0; i < c.rad.size(); ++i) s += (i ? "x" : "") + std::to_string(c.rad[i]);
-    s += "\n";
+    std::string ln = std::string(std::get<3>(kv.first) ? "ldsg " : "lds ") + std::to_string(std::get<1>(kv.first)) +
+                     " " + std::to_string(std::get<2>(kv.first)) +
+                     " " + std::to_string(c.fixed ? 1 : 0) + " " + std::to_string(c.ppt) + " " +
+                     std::to_string(c.ps) + " " + std::to_string(c.tpb) + " ";
+    for (size_t i = 0; i < c.rad.size(); ++i) ln += (i ? "x" : "") + std::to_string(c.rad[i]);
+    per_dev[std::get<0>(kv.first)] += ln + "\n";
   }
+  for (const auto& kv : g_wisdom_fs)
+    per_dev[std::get<0>(kv.first)] += "fs " + std::to_string(std::get<1>(kv.first)) + " " +
+                                      std::to_string(std::get<2>(kv.first)) + " " +
+                                      std::to_string(std::get<3>(kv.first)) + " " + std::to_string(kv.second.n1) +
+                                      " " + std::to_string(kv.second.ccol) + " " + std::to_string(kv.second.crow) +
+                                      "\n";
+  for (const auto& kv : g_wisdom_m)
+    per_dev[std::get<0>(kv.first)] += "bluestein " + std::to_string(std::get<1>(kv.first)) + " " +
+                                      std::to_string(std::get<2>(kv.first)) + " " + std::to_string(kv.second) +
+                                      "\n";
+  std::string s = "fyfft-wisdom 2\n";
+  for (const auto& kv : per_dev) s += "device " + kv.first + "\n" + kv.second;
   return s;
+}
+
+static fyfft_status wisdom_bad(const std::string& why) {
+  g_last_error = "fyfft_wisdom_import: " + why;
+  return FYFFT_INVALID_ARGUMENT;
 }
 
 static fyfft_status wisdom_parse(const char* text) {
   std::string t(text);
   size_t pos = t.find('\n');
-  if (t.compare(0, 15, "fyfft-wisdom 1\n") != 0 || pos == std::string::npos) return FYFFT_INVALID_ARGUMENT;
-  std::map<std::pair<size_t, int>, LdsChoice> parsed;
+  if (t.compare(0, 15, "fyfft-wisdom 2\n") != 0 || pos == std::string::npos)
+    return wisdom_bad("expected the header line \"fyfft-wisdom 2\"");
+  // wisdom is applied only on the GPU model it was tuned on: a device line
+  // that does not match the current device (when one is visible) is refused
+  int cur = -1;
+  const std::string here = hipGetDevice(&cur) == hipSuccess ? device_identity(cur) : "";
+  (void)hipGetLastError();
+  decltype(g_wisdom) lds;
+  decltype(g_wisdom_fs) fs;
+  decltype(g_wisdom_m) bm;
+  std::string dev;
   size_t start = pos + 1;
   while (start < t.size()) {
     size_t end = t.find('\n', start);
@@ -508,11 +772,52 @@ static fyfft_status wisdom_parse(const char* text) {
     const std::string line = t.substr(start, end - start);
     start = end + 1;
     if (line.empty()) continue;
+    char arch[64] = {0};
+    int cus = 0;
+    if (line.compare(0, 7, "device ") == 0) {
+      if (std::sscanf(line.c_str(), "device %63s %d", arch, &cus) != 2 || cus <= 0)
+        return wisdom_bad("malformed device line: " + line);
+      dev = std::string(arch) + " " + std::to_string(cus);
+      if (!here.empty() && dev != here)
+        return wisdom_bad("tuned on " + dev + ", the current device is " + here);
+      continue;
+    }
+    if (dev.empty()) return wisdom_bad("entry before any device line: " + line);
     unsigned long long n = 0;
-    int prec = 0, fixed = 0, ppt = 0, ps = 0, tpb = 0;
+    int prec = 0;
+    if (line.compare(0, 3, "fs ") == 0) {
+      unsigned long long n1 = 0;
+      int kind = 0, ccol = 0, crow = 0;
+      if (std::sscanf(line.c_str(), "fs %llu %d %d %llu %d %d", &n, &prec, &kind, &n1, &ccol, &crow) != 6 ||
+          (prec != kF32 && prec != kF64) || (kind != kWfSmooth && kind != kWfRader) || n1 < 2 ||
+          n > FYFFT_MAX_LENGTH || ccol < 1 || crow < 1 || ccol > 1024 || crow > 1024 || (ccol & (ccol - 1)) ||
+          (crow & (crow - 1)))
+        return wisdom_bad("malformed four-step entry: " + line);
+      const unsigned long long L = kind == kWfRader ? n - 1 : n;
+      if (L % n1 || !is_smooth7((size_t)n1) || !is_smooth7((size_t)(L / n1)))
+        return wisdom_bad("split does not divide the length: " + line);
+      FsChoice c;
+      c.n1 = (size_t)n1;
+      c.ccol = ccol;
+      c.crow = crow;
+      fs[std::make_tuple(dev, (size_t)n, prec, kind)] = c;
+      continue;
+    }
+    if (line.compare(0, 10, "bluestein ") == 0) {
+      unsigned long long m = 0;
+      if (std::sscanf(line.c_str(), "bluestein %llu %d %llu", &n, &prec, &m) != 3 || (prec != kF32 && prec != kF64) ||
+          n < 2 || n > FYFFT_MAX_LENGTH || m < 2 * n - 1 || m > 8 * n || !is_smooth7((size_t)m))
+        return wisdom_bad("malformed Bluestein entry: " + line);
+      bm[std::make_tuple(dev, (size_t)n, prec)] = (size_t)m;
+      continue;
+    }
+    int fixed = 0, ppt = 0, ps = 0, tpb = 0;
     char rad[256] = {0};
-    if (std::sscanf(line.c_str(), "lds %llu %d %d %d %d %d %255s", &n, &prec, &fixed, &ppt, &ps, &tpb, rad) != 7)
-      return FYFFT_INVALID_ARGUMENT;
+    const bool gen = line.compare(0, 5, "ldsg ") == 0;
+    if (std::sscanf(line.c_str() + (gen ? 5 : 4), "%llu %d %d %d %d %d %255s", &n, &prec, &fixed, &ppt, &ps, &tpb,
+                    rad) != 7 ||
+        (!gen && line.compare(0, 4, "lds ") != 0) || (gen && fixed))
+      return wisdom_bad("malformed entry: " + line);
     LdsChoice c;
     c.fixed = fixed != 0;
     c.ppt = ppt;
@@ -521,7 +826,8 @@ static fyfft_status wisdom_parse(const char* text) {
     size_t prod = 1;
     for (char* tok = std::strtok(rad, "x"); tok; tok = std::strtok(nullptr, "x")) {
       const int r = std::atoi(tok);
-      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 9 && r != 16) return FYFFT_INVALID_ARGUMENT;
+      if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 9 && r != 16)
+        return wisdom_bad("radix outside the codelets: " + line);
       c.rad.push_back(r);
       prod *= (size_t)r;
     }
@@ -534,15 +840,16 @@ static fyfft_status wisdom_parse(const char* text) {
                          lds_geometry((size_t)n, prec == kF64 ? 16 : 8, (size_t)1 << 40, &geo, true) && geo.ppt == ppt);
     if (prod != n || (prec != kF32 && prec != kF64) || !ppt_ok || (ps != 30 && (ps < 2 || ps > 8)) ||
         tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses)
-      return FYFFT_INVALID_ARGUMENT;
-    parsed[{(size_t)n, prec}] = c;
+      return wisdom_bad("invalid LDS entry: " + line);
+    lds[std::make_tuple(dev, (size_t)n, prec, gen ? 1 : 0)] = c;
   }
   std::lock_guard<std::mutex> lock(g_wisdom_mu);
-  for (auto& kv : parsed) g_wisdom[kv.first] = kv.second;
+  for (auto& kv : lds) g_wisdom[kv.first] = kv.second;
+  for (auto& kv : fs) g_wisdom_fs[kv.first] = kv.second;
+  for (auto& kv : bm) g_wisdom_m[kv.first] = kv.second;
   return FYFFT_SUCCESS;
 }
 
-static fyfft_status create_plan_impl(fyfft_plan**, size_t, size_t, fyfft_precision, int, int force = -1);
 
 static fyfft_status fft_f64_inplace(double2* buf, size_t len) {
   fyfft_plan* sub = nullptr;
@@ -569,7 +876,14 @@ static fyfft_status plan_conv_structure(fyfft_plan* p, size_t m, bool rader) {
     return st;
   }
   p->kind = rader ? kRaderFs : kBlueFs;
-  fyfft_status st = plan_fourstep(p, m, true);
+  FsChoice tuned_fs;
+  fyfft_status st = FYFFT_SUCCESS;
+  if (rader && !p->force_fs && (p->form == FYFFT_FORM_TUNED || p->form == kFormTunedGeneric)) {
+    st = tune_fourstep(p, m, kWfRader, &tuned_fs);
+    if (!st) p->force_fs = &tuned_fs;
+  }
+  if (!st) st = plan_fourstep(p, m, true);
+  if (p->force_fs == &tuned_fs) p->force_fs = nullptr;
   if (!st) st = alloc_workspace(p, m, rader);
   return st;
 }
@@ -664,7 +978,9 @@ static fyfft_status plan_rader(fyfft_plan* p) {
   (void)hipFree(bb);
   if (!st && p->kind == kRaderFs) st = plan_rader_buckets(p, pout);
   if (!st)
-    p->desc = std::string(p->kind == kRaderLds ? "rader-lds" : "rader-fourstep") + " N=" +
+    p->desc = std::string(p->kind == kRaderLds ? "rader-lds" : "rader-fourstep") +
+              (p->kind == kRaderFs && (p->form == FYFFT_FORM_TUNED || p->form == kFormTunedGeneric) ? "(tuned)" : "") +
+              " N=" +
               std::to_string(n) + " L=" + std::to_string(L) + " g=" + std::to_string(g) +
               (p->rb_q ? " bucketed-permutation Q=" + std::to_string(p->rb_q) : "") +
               (p->kind == kRaderLds && p->lds.fixed >= 0 ? " fixed" : "");
@@ -711,12 +1027,21 @@ static fyfft_status plan_bluestein(fyfft_plan* p) {
   // the cheapest plannable smooth length in [2N-1, 1.25 (2N-1)], else the
   // first plannable one above
   size_t m = 0;
+  if (p->force_m) {
+    if (p->force_m < 2 * n - 1 || conv_length_cost(p->force_m, elem, cap, p->prec) <= 0) return FYFFT_UNSUPPORTED;
+    m = p->force_m;
+  } else if (p->form == FYFFT_FORM_TUNED || p->form == kFormTunedGeneric) {
+    fyfft_status st = tune_bluestein(p, &m);
+    if (st) return st;
+  }
   double best = 0;
   const size_t lo = 2 * n - 1, hi = lo + lo / 4;
-  for (size_t c = lo; c <= hi; ++c) {
+  for (size_t c = lo; c <= hi && !m; ++c) {
     const double cost = conv_length_cost(c, elem, cap, p->prec);
-    if (cost > 0 && (m == 0 || cost < best)) { m = c; best = cost; }
+    if (cost > 0 && (best == 0 || cost < best)) best = cost;
   }
+  for (size_t c = lo; c <= hi && !m; ++c)
+    if (best > 0 && conv_length_cost(c, elem, cap, p->prec) == best) m = c;
   for (size_t c = hi + 1; !m && c < 4 * lo + 64; ++c)
     if (conv_length_cost(c, elem, cap, p->prec) > 0) m = c;
   if (!m) return FYFFT_UNSUPPORTED;
@@ -742,7 +1067,8 @@ static fyfft_status plan_bluestein(fyfft_plan* p) {
   (void)hipFree(c);
   (void)hipFree(bb);
   if (!st)
-    p->desc = std::string(p->kind == kBlueLds ? "bluestein-lds" : "bluestein-fourstep") + " N=" +
+    p->desc = std::string(p->kind == kBlueLds ? "bluestein-lds" : "bluestein-fourstep") +
+              (p->form == FYFFT_FORM_TUNED || p->form == kFormTunedGeneric ? "(tuned)" : "") + " N=" +
               std::to_string(n) + " M=" + std::to_string(m) +
               (p->kind == kBlueLds && p->lds.fixed >= 0 ? " fixed" : "");
   return st;
@@ -964,7 +1290,7 @@ static fyfft_status spectral(const fyfft_plan* p, bool inverse, const void* src,
 // 7-smooth (other N keep the library's choice); the other strategies keep
 // the library's choice (the caller checks eligibility).
 static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, fyfft_precision precision,
-                                     int form, int force) {
+                                     int form, int force, const FsChoice* force_fs, size_t force_m) {
   if (!out) return FYFFT_INVALID_ARGUMENT;
   *out = nullptr;
   if (batch == 0) return FYFFT_INVALID_ARGUMENT;
@@ -972,9 +1298,11 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
       precision != FYFFT_F16)
     return FYFFT_INVALID_ARGUMENT;
   if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
-  if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED && form != FYFFT_FORM_MFMA)
+  if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED && form != FYFFT_FORM_MFMA &&
+      form != kFormTunedGeneric)
     return FYFFT_INVALID_ARGUMENT;
   const bool low = precision == FYFFT_BF16 || precision == FYFFT_F16;
+  const bool tuned = form == FYFFT_FORM_TUNED || form == kFormTunedGeneric;
   if (form == FYFFT_FORM_MFMA && !low) return FYFFT_INVALID_ARGUMENT;
   int r0 = 1;
   dev::PassDesc lowp_pd{};
@@ -990,6 +1318,10 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
   p->batch = batch;
   p->prec = precision == FYFFT_F64 ? kF64 : (precision == FYFFT_BF16 ? kBF16 : (precision == FYFFT_F16 ? kF16 : kF32));
   p->device = dev;
+  p->form = form;
+  p->force_fs = force_fs;
+  p->force_m = force_m;
+  FsChoice tuned_fs;
   p->strategy = strategy_for(n);
   fyfft_status st = FYFFT_SUCCESS;
   if (lowp) {
@@ -1035,21 +1367,26 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     LdsGeom g;
     if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
       p->kind = kLds;
-      st = form == FYFFT_FORM_TUNED ? plan_lds_tuned(p, n) : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO, true, true);
+      st = tuned ? plan_lds_tuned(p, n, form == kFormTunedGeneric)
+                 : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO, true, true);
       if (!st) {
         const LdsGeom& q = p->lds.geom;
         p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") +
-                  (form == FYFFT_FORM_TUNED ? "(tuned)" : "") + " N=" +
+                  (tuned ? "(tuned)" : "") + " N=" +
                   std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
                   " T=" + std::to_string(q.threads) + " TPB=" + std::to_string(q.per_block) +
                   " pad=" + std::to_string(q.pad_shift);
       }
     } else {
       p->kind = kFourStep;
-      st = plan_fourstep(p, n, false);
+      if (tuned && !p->force_fs) {
+        st = tune_fourstep(p, n, kWfSmooth, &tuned_fs);
+        if (!st) p->force_fs = &tuned_fs;
+      }
+      if (!st) st = plan_fourstep(p, n, false);
       if (!st) st = alloc_workspace(p, n, false);
       if (!st)
-        p->desc = "fourstep N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
+        p->desc = std::string(tuned ? "fourstep(tuned)" : "fourstep") + " N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
                   radix_string(p->pdcol) + ") x " + std::to_string(p->n2) + "(" +
                   radix_string(p->pdrow) + ") C=" + std::to_string(p->gcol.per_block) + "/" +
                   std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk);
@@ -1059,6 +1396,7 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
   } else {
     st = plan_bluestein(p);
   }
+  p->force_fs = nullptr;  // pointed at this function's candidate
   if (!st) {
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) st = from_hip(e, "plan table generation");
@@ -2058,6 +2396,7 @@ fyfft_status fyfft_plan_create_ex(fyfft_plan** out, size_t n, size_t batch, fyff
     for (int r : options->reserved)
       if (r != 0) return FYFFT_INVALID_ARGUMENT;
     form = options->form;
+    if (form < FYFFT_FORM_AUTO || form > FYFFT_FORM_MFMA) return FYFFT_INVALID_ARGUMENT;
   }
   return create_plan_impl(out, n, batch, precision, form);
 }
@@ -2123,6 +2462,8 @@ fyfft_status fyfft_wisdom_import(const char* text) {
 fyfft_status fyfft_wisdom_forget(void) {
   std::lock_guard<std::mutex> lock(g_wisdom_mu);
   g_wisdom.clear();
+  g_wisdom_fs.clear();
+  g_wisdom_m.clear();
   return FYFFT_SUCCESS;
 }
 
@@ -2131,7 +2472,8 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
   if (!out) return FYFFT_INVALID_ARGUMENT;
   *out = nullptr;
   if (!lengths || count == 0 || count > (size_t)INT32_MAX / 2) return FYFFT_INVALID_ARGUMENT;
-  if (flags & ~(unsigned)(FYFFT_GROUPED_GRAPH | FYFFT_GROUPED_PER_TRANSFORM)) return FYFFT_INVALID_ARGUMENT;
+  if (flags & ~(unsigned)(FYFFT_GROUPED_GRAPH | FYFFT_GROUPED_PER_TRANSFORM | FYFFT_GROUPED_TUNED))
+    return FYFFT_INVALID_ARGUMENT;
   if (precision != FYFFT_F32 && precision != FYFFT_F64 && precision != FYFFT_BF16 && precision != FYFFT_F16)
     return FYFFT_INVALID_ARGUMENT;
   for (size_t i = 0; i < count; ++i)
@@ -2158,9 +2500,35 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
   }
   p->n = p->total;
   fyfft_status st = FYFFT_SUCCESS;
+  // FYFFT_GROUPED_TUNED: the lengths that carry the most elements are tuned
+  // first (radix order, padding and points per thread of LDS lengths; split
+  // and tile of four-step and Rader convolution lengths; Bluestein's
+  // convolution length), each timed on a batch of its own length, until the
+  // tuning time budget is spent; wisdom makes later plans of the same
+  // lengths tune instantly.  Other lengths keep the planner's defaults.
+  std::vector<size_t> tune_order;
+  for (const auto& kv : by_len) tune_order.push_back(kv.first);
+  std::stable_sort(tune_order.begin(), tune_order.end(), [&](size_t a, size_t b) {
+    return a * by_len.at(a).size() > b * by_len.at(b).size();
+  });
+  std::map<size_t, int> tune_form;
+  if (flags & FYFFT_GROUPED_TUNED) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (size_t len : tune_order) {
+      if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > kGroupedTuneSeconds) break;
+      tune_form[len] = kFormTunedGeneric;
+      fyfft_plan* probe = nullptr;  // tunes (or finds wisdom for) this length once
+      const size_t cnt = by_len.at(len).size();
+      if (create_plan_impl(&probe, len, std::max<size_t>(cnt, 64), sub_prec, kFormTunedGeneric) == FYFFT_SUCCESS)
+        fyfft_plan_destroy(probe);
+      else
+        tune_form[len] = FYFFT_FORM_PLAIN;
+    }
+  }
   for (const auto& kv : by_len) {
     fyfft_plan* sp = nullptr;
-    st = create_plan_impl(&sp, kv.first, 1, sub_prec, FYFFT_FORM_PLAIN);
+    auto tf = tune_form.find(kv.first);
+    st = create_plan_impl(&sp, kv.first, 1, sub_prec, tf == tune_form.end() ? FYFFT_FORM_PLAIN : tf->second);
     if (st) break;
     for (size_t i : kv.second) p->sub_of[i] = (int)p->subs.size();
     p->subs.push_back(sp);

@@ -85,11 +85,13 @@ typedef enum fyfft_strategy {
  *           length, else the generic kernels (what fyfft_plan_create does);
  *   PLAIN - always the generic ("plain GPU form") kernels, the baseline the
  *           other forms are measured against;
- *   TUNED - time candidate configurations (radix order, LDS padding, points
- *           per thread, specialised vs generic) on the device at plan time
- *           and keep the fastest; the choice is remembered as wisdom per
- *           (N, precision).  Applies to lengths held in one workgroup's LDS;
- *           other lengths plan as AUTO;
+ *   TUNED - time candidate configurations on the device at plan time and
+ *           keep the fastest: radix order, LDS padding, points per thread
+ *           and specialised vs generic kernels for lengths held in one
+ *           workgroup's LDS; the split and tiles of four-step lengths and
+ *           of four-step Rader convolutions; Bluestein's convolution
+ *           length.  The choice is remembered as wisdom per (GPU model, N,
+ *           precision);
  *   MFMA  - BF16/F16 storage only: radix-16 stages on MFMA with split hi/lo
  *           fp16 operands (for BF16/F16 the other forms run FP32 arithmetic
  *           in the compile-time kernels, rounding once at the store). */
@@ -214,6 +216,13 @@ fyfft_status fyfft_plan_create_real(fyfft_plan** plan, size_t n, size_t batch,
  *     each transform into an FP32 copy around it). */
 #define FYFFT_GROUPED_GRAPH 1u
 #define FYFFT_GROUPED_PER_TRANSFORM 2u
+/*   FYFFT_GROUPED_TUNED: autotune the sub-plans per length (radix order,
+ *     LDS padding and points per thread; four-step split and tiles;
+ *     Bluestein convolution length), lengths carrying the most elements
+ *     first, within about 20 s of plan time; choices are recorded as wisdom
+ *     (fyfft_wisdom_export), so a later plan of the same lengths tunes
+ *     instantly. */
+#define FYFFT_GROUPED_TUNED 4u
 fyfft_status fyfft_plan_create_grouped(fyfft_plan** plan, const size_t* lengths, size_t count,
                                        fyfft_precision precision, unsigned flags);
 
@@ -226,11 +235,21 @@ fyfft_status fyfft_plan_elements(const fyfft_plan* plan, size_t* elements);
 fyfft_status fyfft_plan_create_ex(fyfft_plan** plan, size_t n, size_t batch,
                                   fyfft_precision precision, const fyfft_options* options);
 
-/* Autotuner wisdom as text ("fyfft-wisdom 1" and one line per (N,
- * precision)): export writes it to buf (NUL-terminated; *needed receives the
- * size including the NUL, buf may be NULL to query), import merges lines
- * into the process's wisdom (FYFFT_INVALID_ARGUMENT on malformed text),
- * forget clears it.  The reference has no plan serialisation. */
+/* Autotuner wisdom as text: the line "fyfft-wisdom 2", then per GPU model a
+ * line "device <arch> <compute units>" (e.g. "device gfx950 256") followed
+ * by its choices, one line each:
+ *   lds N prec fixed ppt pad tpb radices   (LDS-resident lengths; "ldsg":
+ *                                            runtime-pass kernels only, the
+ *                                            sub-plans of grouped plans)
+ *   fs N prec kind n1 ccol crow             (four-step split and tiles; kind
+ *                                            0 smooth N, 1 Rader convolution)
+ *   bluestein N prec M                      (Bluestein convolution length).
+ * export writes it to buf (NUL-terminated; *needed receives the size
+ * including the NUL, buf may be NULL to query), import merges it into the
+ * process's wisdom -- FYFFT_INVALID_ARGUMENT on malformed text, or when a
+ * device line does not match the current GPU (wisdom is only applied on the
+ * model it was tuned on) -- and forget clears it.  Plans look up the entries
+ * of their own device.  The reference has no plan serialisation. */
 fyfft_status fyfft_wisdom_export(char* buf, size_t len, size_t* needed);
 fyfft_status fyfft_wisdom_import(const char* text);
 fyfft_status fyfft_wisdom_forget(void);

@@ -88,17 +88,36 @@ def test_invoker_mirror_strategy(fy):
 
 
 def test_wisdom_text_roundtrip_and_validation(fy):
+    """Wisdom text, version 2: entries follow the device line of the GPU model
+    they were tuned on; malformed text is refused and leaves the wisdom as it
+    was.  (Without a GPU any device line is accepted; on a GPU a device line
+    of another model is refused: test_gpu_forms.)"""
     fy.wisdom_forget()
-    assert fy.wisdom_export() == "fyfft-wisdom 1\n"
-    line = "lds 6720 0 0 16 4 1 8x8x7x5x3\n"
-    fy.wisdom_import("fyfft-wisdom 1\n" + line)
-    assert fy.wisdom_export() == "fyfft-wisdom 1\n" + line
-    for bad in ["nonsense\n", "fyfft-wisdom 1\nlds 6720 0 0 16 4 1 8x8x7x5\n",   # product != N
-                "fyfft-wisdom 1\nlds 64 0 0 12 4 1 8x8\n",                      # bad points/thread
-                "fyfft-wisdom 1\nlds 64 0 0 16 4 1 6x8\n"]:                     # radix 6 not a codelet
+    assert fy.wisdom_export() == "fyfft-wisdom 2\n"
+    text = ("fyfft-wisdom 2\ndevice gfx950 256\n"
+            "lds 6720 0 0 16 4 1 8x8x7x5x3\n"
+            "ldsg 1000 0 0 16 4 4 8x5x5x5\n"
+            "fs 100000 0 0 250 16 16\n"
+            "fs 65537 0 1 256 16 8\n"
+            "bluestein 10007 0 20480\n")
+    fy.wisdom_import(text)
+    out = fy.wisdom_export()
+    assert out.startswith("fyfft-wisdom 2\ndevice gfx950 256\n"), out
+    for line in text.splitlines()[2:]:
+        assert line in out.splitlines(), (line, out)
+    for bad in ["nonsense\n",
+                "fyfft-wisdom 1\nlds 6720 0 0 16 4 1 8x8x7x5x3\n",              # version 1: no device identity
+                "fyfft-wisdom 2\nlds 6720 0 0 16 4 1 8x8x7x5x3\n",              # entry before a device line
+                "fyfft-wisdom 2\ndevice gfx950 256\nlds 6720 0 0 16 4 1 8x8x7x5\n",   # product != N
+                "fyfft-wisdom 2\ndevice gfx950 256\nlds 64 0 0 12 4 1 8x8\n",         # bad points/thread
+                "fyfft-wisdom 2\ndevice gfx950 256\nlds 64 0 0 16 4 1 6x8\n",         # radix 6 not a codelet
+                "fyfft-wisdom 2\ndevice gfx950 256\nfs 100000 0 0 300 16 16\n",       # 300 does not divide
+                "fyfft-wisdom 2\ndevice gfx950 256\nfs 65537 0 1 256 12 8\n",         # tile not a power of two
+                "fyfft-wisdom 2\ndevice gfx950 256\nbluestein 10007 0 11000\n",       # M < 2N - 1
+                "fyfft-wisdom 2\ndevice gfx950\nlds 64 0 0 16 4 1 8x8\n"]:            # no CU count
         with pytest.raises(fy.FyfftError):
             fy.wisdom_import(bad)
-    assert fy.wisdom_export() == "fyfft-wisdom 1\n" + line
+    assert fy.wisdom_export() == out
     fy.wisdom_forget()
 
 
