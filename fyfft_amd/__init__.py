@@ -341,14 +341,14 @@ MIX_FORWARD, MIX_ADJOINT_X, MIX_GRAD_W = 0, 1, 2
 
 def mix_modes(kind: int, p, q, stream=None):
     """Per-mode complex GEMMs of the spectral convolution on the matrix cores
-    (complex64 device tensors, mode-major): with modes [K, rows, C] and
+    (complex64 or complex128 device tensors, mode-major): with modes [K, rows, C] and
     weights [K, Ci, Co],
       MIX_FORWARD:   p modes [K, R, Ci], q weights -> [K, R, Co]  (p @ q)
       MIX_ADJOINT_X: p modes [K, R, Co], q weights -> [K, R, Ci]  (p @ q^H)
       MIX_GRAD_W:    p modes [K, R, Ci], q modes [K, R, Co] -> [K, Ci, Co]  (p^H @ q)."""
     import torch
-    if p.dtype != torch.complex64 or q.dtype != torch.complex64 or not p.is_cuda or not q.is_cuda:
-        raise TypeError("mix_modes: complex64 device tensors expected")
+    if p.dtype not in (torch.complex64, torch.complex128) or q.dtype != p.dtype or not p.is_cuda or not q.is_cuda:
+        raise TypeError("mix_modes: complex64 or complex128 device tensors of one dtype expected")
     p, q = p.contiguous(), q.contiguous()
     if p.dim() != 3 or q.dim() != 3 or p.shape[0] != q.shape[0]:
         raise ValueError("mix_modes: [K, ., .] operands with equal K expected")
@@ -370,7 +370,8 @@ def mix_modes(kind: int, p, q, stream=None):
         out = torch.empty(K, ci, co, dtype=p.dtype, device=p.device)
     else:
         raise ValueError("mix_modes: unknown kind")
-    _check(_load().fyfft_mix_modes(kind, F32, ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(q.data_ptr()),
+    prec = F32 if p.dtype == torch.complex64 else F64
+    _check(_load().fyfft_mix_modes(kind, prec, ctypes.c_void_p(p.data_ptr()), ctypes.c_void_p(q.data_ptr()),
                                    ctypes.c_void_p(out.data_ptr()), K, R, ci, co,
                                    ctypes.c_void_p(_stream_handle(stream, p.device))), "mix_modes")
     return out

@@ -58,9 +58,9 @@ class SpectralConv1dFunction(torch.autograd.Function):
 
     # The modes live mode-major ([K, B, C]) between the transforms, so each
     # mode's channel mixing is one GEMM with no layout copies of the
-    # [B, C, K] mode tensors: complex64 runs the MFMA kernel of
-    # fyfft_mix_modes (forward, p @ q^H and p^H @ q forms), complex128 runs
-    # torch.matmul with the conjugate transposes as views.  W [Ci, Co, K] is
+    # [B, C, K] mode tensors: the MFMA kernel of fyfft_mix_modes (forward,
+    # p @ q^H and p^H @ q forms; FP32 MFMA for complex64, FP64 MFMA for
+    # complex128).  W [Ci, Co, K] is
     # used through its [K, Ci, Co] permutation, a view without a copy when W
     # is stored mode-major (as SpectralConv1d stores it), and dW is returned
     # as the same permuted view of a [K, Ci, Co] result.
@@ -75,7 +75,7 @@ class SpectralConv1dFunction(torch.autograd.Function):
         wt = w.permute(2, 0, 1)                               # [K, Ci, Co]
         if not wt.is_contiguous():
             wt = wt.contiguous()
-        ym = mix_modes(MIX_FORWARD, xm, wt) if xm.dtype == torch.complex64 else torch.matmul(xm, wt)
+        ym = mix_modes(MIX_FORWARD, xm, wt)
         y = irfft_modes(ym, L, 1.0 / L, 2.0 / L, mode_major=True)   # [B, Co, L]
         ctx.save_for_backward(xm, wt)
         ctx.L = L
@@ -87,12 +87,8 @@ class SpectralConv1dFunction(torch.autograd.Function):
         L, K = ctx.L, wt.shape[0]
         # dL/dY = S * (1/L) * FFT(g) restricted to the kept modes (S = 1, 2, 2, ...)
         gy = rfft_modes(g.contiguous(), K, 1.0 / L, 2.0 / L, mode_major=True)   # [K, B, Co]
-        if gy.dtype == torch.complex64:
-            gxm = mix_modes(MIX_ADJOINT_X, gy, wt)                              # [K, B, Ci]
-            gw = mix_modes(MIX_GRAD_W, xm, gy).permute(1, 2, 0)                 # [Ci, Co, K] view
-        else:
-            gxm = torch.matmul(gy, wt.mH)
-            gw = torch.matmul(xm.mH, gy).permute(1, 2, 0)
+        gxm = mix_modes(MIX_ADJOINT_X, gy, wt)                                  # [K, B, Ci]
+        gw = mix_modes(MIX_GRAD_W, xm, gy).permute(1, 2, 0)                     # [Ci, Co, K] view
         # dL/dx = Re(F^H pad(gX)): the real part of the adjoint over K modes
         gx = irfft_modes(gxm, L, 1.0, 1.0, mode_major=True)
         return gx, gw

@@ -2623,8 +2623,8 @@ fyfft_status fyfft_mix_modes(int kind, fyfft_precision precision, const void* p,
     g_last_error = "fyfft_mix_modes: unknown kind";
     return FYFFT_INVALID_ARGUMENT;
   }
-  if (precision != FYFFT_F32) {
-    g_last_error = "fyfft_mix_modes: only FYFFT_F32 (complex float) operands";
+  if (precision != FYFFT_F32 && precision != FYFFT_F64) {
+    g_last_error = "fyfft_mix_modes: FYFFT_F32 (complex float) or FYFFT_F64 (complex double) operands only";
     return FYFFT_UNSUPPORTED;
   }
   if (k == 0 || rows == 0 || cin == 0 || cout == 0) return FYFFT_SUCCESS;
@@ -2633,8 +2633,8 @@ fyfft_status fyfft_mix_modes(int kind, fyfft_precision precision, const void* p,
     g_last_error = "fyfft_mix_modes: k must be <= 65535 and rows, cin, cout < 2^30";
     return FYFFT_INVALID_ARGUMENT;
   }
-  HIP_TRY(launch_mix_modes(kind, p, q, out, (long long)k, (long long)rows, (long long)cin, (long long)cout,
-                           (hipStream_t)stream));
+  HIP_TRY(launch_mix_modes(precision == FYFFT_F64 ? kF64 : kF32, kind, p, q, out, (long long)k, (long long)rows,
+                           (long long)cin, (long long)cout, (hipStream_t)stream));
   return FYFFT_SUCCESS;
 }
 

@@ -139,11 +139,11 @@ template <typename V> struct ConvArgs {
 };
 // Fused single-kernel Rader (rader=true) or Bluestein, whole convolution in LDS.
 // --- kernels_mix.hip
-// Mode mixing of the spectral convolution (complex64, mode-major operands:
-// modes [K][rows][C], weights [K][cin][cout]).
+// Mode mixing of the spectral convolution (complex64 or complex128 by prec,
+// mode-major operands: modes [K][rows][C], weights [K][cin][cout]).
 enum MixKind { kMixForward = 0, kMixAdjointX = 1, kMixGradW = 2 };
-hipError_t launch_mix_modes(int kind, const void* p, const void* q, void* out, long long K, long long rows,
-                            long long cin, long long cout, hipStream_t s);
+hipError_t launch_mix_modes(int prec, int kind, const void* p, const void* q, void* out, long long K,
+                            long long rows, long long cin, long long cout, hipStream_t s);
 
 // fixed: index from find_conv_fixed, or -1 for the runtime passes
 hipError_t launch_conv_lds(int prec, bool rader, const void* args, const LdsGeom& g, int fixed, hipStream_t s);

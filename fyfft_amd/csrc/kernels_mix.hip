@@ -1,5 +1,7 @@
 // Mode mixing of the spectral convolution: one complex GEMM per kept mode,
-// on the FP32 matrix cores (v_mfma_f32_16x16x4f32).
+// on the matrix cores: v_mfma_f32_16x16x4f32 for complex float,
+// v_mfma_f64_16x16x4f64 for complex double (same 16x16x4 block shape and
+// operand layout; the result rows are interleaved differently, below).
 //
 //   C_k = op(A_k) * op(B_k),  k < K,   op = identity or conjugate, any strides
 //
@@ -10,9 +12,10 @@
 // as 2x2 blocks of 16x16; the L dimension goes through LDS in chunks of 16,
 // staged with coalesced loads whichever of the operand's strides is 1 and
 // the next chunk's loads in flight while the current one is multiplied.
-// Lane layout of the 16x16x4 instruction (checked on the device): lane l
+// Lane layout of the 16x16x4 instructions (probed on the device): lane l
 // holds A[l%16][l/16] and B[l/16][l%16]; result element i of lane l is
-// C[4*(l/16) + i][l%16].
+// C[4*(l/16) + i][l%16] for the FP32 instruction and C[l/16 + 4*i][l%16]
+// for the FP64 one.
 #include "bodies.hpp"
 
 namespace fyfft {
@@ -25,13 +28,21 @@ constexpr int kAStride = kChunk + 1;  // A tile rows padded (16 lanes read 16 ro
 constexpr int kBStride = kTile + 16;  // B tile rows padded by 32 banks
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc;
+template <> struct Acc<float> { using type = v4f; };
+template <> struct Acc<double> { using type = v4d; };
 
 __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
 
-struct Op {
-  const float2* p;
+template <typename V> struct Op {
+  const V* p;
   long long sk, s0, s1;  // element (i0, i1) of batch k at p[k*sk + i0*s0 + i1*s1]
   int n0, n1;            // extents
   bool conj;
@@ -54,31 +65,34 @@ __device__ __forceinline__ void tile_coord(int e, bool fast1, int* i0, int* i1) 
 }  // namespace
 
 // A: (m, l), B: (l, n), C: (m, n).  grid: x = tiles of M x N, y = K.
-__global__ __launch_bounds__(256) void k_cgemm_modes(Op A, Op B, float2* C, long long csk, long long csm,
+template <typename V>
+__global__ __launch_bounds__(256) void k_cgemm_modes(Op<V> A, Op<V> B, V* C, long long csk, long long csm,
                                                      long long csn, int M, int N, int L) {
-  __shared__ float2 As[kTile * kAStride];   // [m][l]
-  __shared__ float2 Bs[kChunk * kBStride];  // [l][n]
+  using T = S<V>;
+  using VA = typename Acc<T>::type;
+  __shared__ V As[kTile * kAStride];   // [m][l]
+  __shared__ V Bs[kChunk * kBStride];  // [l][n]
   const int tiles_n = (N + kTile - 1) / kTile;
   const int m0 = (int)(blockIdx.x / tiles_n) * kTile, n0 = (int)(blockIdx.x % tiles_n) * kTile;
   const long long k = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const bool a_fast_l = A.s1 == 1, b_fast_n = B.s1 == 1;
-  const float2* Ap = A.p + k * A.sk;
-  const float2* Bp = B.p + k * B.sk;
+  const V* Ap = A.p + k * A.sk;
+  const V* Bp = B.p + k * B.sk;
 
   // one chunk of both tiles: 4 + 4 elements per thread (zero outside)
-  float2 ra[4], rb[4];
+  V ra[4], rb[4];
   auto load = [&](int l0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       int i, j;
       tile_coord<kTile, kChunk>(tid + q * 256, a_fast_l, &i, &j);  // (m, l)
       const int m = m0 + i, l = l0 + j;
-      ra[q] = (m < M && l < L) ? Ap[m * A.s0 + l * A.s1] : float2{0.f, 0.f};
+      ra[q] = (m < M && l < L) ? Ap[m * A.s0 + l * A.s1] : V{0, 0};
       tile_coord<kChunk, kTile>(tid + q * 256, b_fast_n, &i, &j);  // (l, n)
       const int lb = l0 + i, n = n0 + j;
-      rb[q] = (lb < L && n < N) ? Bp[lb * B.s0 + n * B.s1] : float2{0.f, 0.f};
+      rb[q] = (lb < L && n < N) ? Bp[lb * B.s0 + n * B.s1] : V{0, 0};
     }
   };
   auto store = [&]() {
@@ -92,12 +106,12 @@ __global__ __launch_bounds__(256) void k_cgemm_modes(Op A, Op B, float2* C, long
     }
   };
 
-  v4f cr[2][2], ci[2][2];
+  VA cr[2][2], ci[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) cr[i][j] = ci[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-  const float sa = A.conj ? -1.f : 1.f, sb = B.conj ? -1.f : 1.f;
+    for (int j = 0; j < 2; ++j) cr[i][j] = ci[i][j] = VA{0, 0, 0, 0};
+  const T sa = A.conj ? -1 : 1, sb = B.conj ? -1 : 1;
 
   load(0);
   for (int l0 = 0; l0 < L; l0 += kChunk) {
@@ -108,7 +122,7 @@ __global__ __launch_bounds__(256) void k_cgemm_modes(Op A, Op B, float2* C, long
 #pragma unroll
     for (int s = 0; s < kChunk / 4; ++s) {
       const int kk = s * 4 + (lane >> 4), r = lane & 15;
-      float2 a[2], b[2];
+      V a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         a[i] = As[(wm + i * 16 + r) * kAStride + kk];
@@ -130,39 +144,41 @@ __global__ __launch_bounds__(256) void k_cgemm_modes(Op A, Op B, float2* C, long
         }
     }
   }
-  float2* Cp = C + k * csk;
+  V* Cp = C + k * csk;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q, n = n0 + wn + j * 16 + (lane & 15);
-        if (m < M && n < N) Cp[m * csm + n * csn] = float2{cr[i][j][q], ci[i][j][q]};
+        const int row = sizeof(T) == 8 ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
+        const int m = m0 + wm + i * 16 + row, n = n0 + wn + j * 16 + (lane & 15);
+        if (m < M && n < N) Cp[m * csm + n * csn] = V{cr[i][j][q], ci[i][j][q]};
       }
 }
 
 // Mode-major operands: modes [K][rows][C], weights [K][cin][cout].
-hipError_t launch_mix_modes(int kind, const void* p, const void* q, void* out, long long K, long long rows,
-                            long long cin, long long cout, hipStream_t s) {
-  Op A, B;
+template <typename V>
+static hipError_t go_mix(int kind, const void* p, const void* q, void* out, long long K, long long rows,
+                         long long cin, long long cout, hipStream_t s) {
+  Op<V> A, B;
   long long csk, csm, csn;
   int M, N, L;
-  const float2* P = static_cast<const float2*>(p);
-  const float2* Q = static_cast<const float2*>(q);
+  const V* P = static_cast<const V*>(p);
+  const V* Q = static_cast<const V*>(q);
   if (kind == kMixForward) {  // out[k][r][o] = sum_i p[k][r][i] q[k][i][o]
-    A = Op{P, rows * cin, cin, 1, (int)rows, (int)cin, false};
-    B = Op{Q, cin * cout, cout, 1, (int)cin, (int)cout, false};
+    A = Op<V>{P, rows * cin, cin, 1, (int)rows, (int)cin, false};
+    B = Op<V>{Q, cin * cout, cout, 1, (int)cin, (int)cout, false};
     M = (int)rows, N = (int)cout, L = (int)cin;
     csk = rows * cout, csm = cout, csn = 1;
   } else if (kind == kMixAdjointX) {  // out[k][r][i] = sum_o p[k][r][o] conj(q[k][i][o])
-    A = Op{P, rows * cout, cout, 1, (int)rows, (int)cout, false};
-    B = Op{Q, cin * cout, 1, cout, (int)cout, (int)cin, true};
+    A = Op<V>{P, rows * cout, cout, 1, (int)rows, (int)cout, false};
+    B = Op<V>{Q, cin * cout, 1, cout, (int)cout, (int)cin, true};
     M = (int)rows, N = (int)cin, L = (int)cout;
     csk = rows * cin, csm = cin, csn = 1;
   } else if (kind == kMixGradW) {  // out[k][i][o] = sum_r conj(p[k][r][i]) q[k][r][o]
-    A = Op{P, rows * cin, 1, cin, (int)cin, (int)rows, true};
-    B = Op{Q, rows * cout, cout, 1, (int)rows, (int)cout, false};
+    A = Op<V>{P, rows * cin, 1, cin, (int)cin, (int)rows, true};
+    B = Op<V>{Q, rows * cout, cout, 1, (int)rows, (int)cout, false};
     M = (int)cin, N = (int)cout, L = (int)rows;
     csk = cin * cout, csm = cout, csn = 1;
   } else {
@@ -171,9 +187,16 @@ hipError_t launch_mix_modes(int kind, const void* p, const void* q, void* out, l
   if (K == 0 || M == 0 || N == 0) return hipSuccess;
   const long long tiles = (long long)((M + kTile - 1) / kTile) * ((N + kTile - 1) / kTile);
   if (tiles > 0x7fffffffLL || K > 65535) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_cgemm_modes, dim3((unsigned)tiles, (unsigned)K), dim3(256), 0, s, A, B, (float2*)out, csk,
+  hipLaunchKernelGGL(k_cgemm_modes<V>, dim3((unsigned)tiles, (unsigned)K), dim3(256), 0, s, A, B, (V*)out, csk,
                      csm, csn, M, N, L);
   return hipGetLastError();
+}
+
+hipError_t launch_mix_modes(int prec, int kind, const void* p, const void* q, void* out, long long K,
+                            long long rows, long long cin, long long cout, hipStream_t s) {
+  if (prec == kF64) return go_mix<double2>(kind, p, q, out, K, rows, cin, cout, s);
+  if (prec == kF32) return go_mix<float2>(kind, p, q, out, K, rows, cin, cout, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace fyfft

@@ -341,8 +341,9 @@ fyfft_status fyfft_irfft_modes_ex(const fyfft_plan* plan, const void* modes, voi
 /* Mode mixing of the spectral convolution (no counterpart in the reference):
  * one complex GEMM per kept mode on the matrix cores, with the mode-major
  * layouts of fyfft_rfft_modes_ex: modes [k][rows][channels], weights
- * [k][cin][cout] (complex, interleaved re/im).  Only FYFFT_F32 (complex
- * float) is supported; k <= 65535.  Asynchronous on `stream`.
+ * [k][cin][cout] (complex, interleaved re/im).  FYFFT_F32 (complex float,
+ * FP32 MFMA) or FYFFT_F64 (complex double, FP64 MFMA); k <= 65535.
+ * Asynchronous on `stream`.
  *   FYFFT_MIX_FORWARD:   out[j][r][o] = sum_i p[j][r][i] * q[j][i][o]
  *                        (p: input modes, q: weights, out: output modes)
  *   FYFFT_MIX_ADJOINT_X: out[j][r][i] = sum_o p[j][r][o] * conj(q[j][i][o])
