@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-baseline", action="store_true",
                     help="ragged: skip the one-launch-per-transform baselines (kernel-trace runs)")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="ragged: only the FP32 grouped leg (no tuned and BF16 legs; profiling runs)")
     ap.add_argument("--no-configs", action="store_true",
                     help="headline only: skip the other configurations (profiling runs)")
     ap.add_argument("--cpu-sample", type=int, default=65536,
@@ -371,6 +373,8 @@ def bench_ragged(a, compact_out=False):
                per_transform_baseline={"ms_per_step": round(ms_base, 3),
                                        "speedup_grouped": round(ms_base / ms, 2)})
     gp.close()
+    if getattr(a, "fp32_only", False):
+        return out
     # the same plan with every sub-plan autotuned (FYFFT_GROUPED_TUNED): the
     # lengths carrying the most elements first, within the tuning budget
     t0 = time.perf_counter()
