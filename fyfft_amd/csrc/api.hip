@@ -525,10 +525,12 @@ struct TuneBench {
   }
 };
 
-// Transforms per tuning launch: the plan's batch, up to 256 MB of data (long
-// enough that launch gaps do not decide between candidates).
+// Transforms per tuning launch: the plan's own batch, up to 1 GiB of data.
+// Smaller samples misled: 256 MB fits the 256 MB Infinity Cache, and the
+// structures fastest there were 5-10 % slower at the full batch (N = 2^16 x
+// 1024, 2^20 x 64).
 static size_t tune_batch(size_t batch, size_t n, size_t elem) {
-  return std::max<size_t>(1, std::min<size_t>(batch, ((size_t)256 << 20) / (n * elem)));
+  return std::max<size_t>(1, std::min<size_t>(batch, ((size_t)1 << 30) / (n * elem)));
 }
 
 // generic_only: candidates the runtime-pass kernels can run (sub-plans of
@@ -579,8 +581,11 @@ static fyfft_status plan_lds_tuned(fyfft_plan* p, size_t n, bool generic_only = 
 }
 
 // Four-step structure candidates for length L: the default split, the
-// plannable divisor pairs nearest to it, and for each split the default and
-// halved sub-transforms per workgroup of both passes.
+// plannable divisor pairs nearest to it, and for each split the default,
+// halved and (where threads and LDS allow) doubled sub-transforms per
+// workgroup of both passes (doubled column tiles read 256-byte row segments,
+// which straddle fewer cache lines when the row length is not a multiple of
+// the line).
 static std::vector<FsChoice> fs_candidates(const fyfft_plan* p, size_t L, bool conv) {
   const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
   std::vector<FsChoice> out;
@@ -596,13 +601,17 @@ static std::vector<FsChoice> fs_candidates(const fyfft_plan* p, size_t L, bool c
     others.push_back({std::fabs(std::log((double)d / (double)d1)), d});
   }
   std::sort(others.begin(), others.end());
-  for (size_t i = 0; i < others.size() && splits.size() < 5; ++i) splits.push_back(others[i].second);
+  for (size_t i = 0; i < others.size() && splits.size() < 4; ++i) splits.push_back(others[i].second);
   for (size_t n1 : splits) {
     LdsGeom a, b;
     if (!fs_geometry(n1, elem, cap, &a) || !fs_geometry(L / n1, elem, cap, &b)) continue;
-    for (int ca : {a.per_block, a.per_block / 2})
-      for (int cb : {b.per_block, b.per_block / 2}) {
-        if (ca < 1 || cb < 1) continue;
+    auto fits = [&](const LdsGeom& q, size_t len, int c) {
+      return c >= 1 && (size_t)c * q.threads <= 1024 &&
+             (size_t)c * (size_t)dev::lds_len((int)len, q.pad_shift) * elem <= cap;
+    };
+    for (int ca : {a.per_block, a.per_block / 2, a.per_block * 2})
+      for (int cb : {b.per_block, b.per_block / 2, b.per_block * 2}) {
+        if (!fits(a, n1, ca) || !fits(b, L / n1, cb)) continue;
         FsChoice c;
         c.n1 = n1;
         c.ccol = ca;
@@ -622,7 +631,7 @@ static fyfft_precision api_precision(int prec) {
 
 // The fastest four-step structure for the plan's length (kind: a smooth
 // length, or the convolution of a Rader plan), from wisdom or by timing a
-// candidate plan per structure on a batch of up to 256 MB.
+// candidate plan per structure on the plan's batch (up to 1 GiB).
 static fyfft_status tune_fourstep(fyfft_plan* p, size_t L, int kind, FsChoice* best) {
   const auto key = std::make_tuple(device_identity(p->device), p->n, p->prec, kind);
   {
@@ -1075,6 +1084,10 @@ static fyfft_status plan_bluestein(fyfft_plan* p) {
 }
 
 // ------------------------------------------------------------ execution
+// Rows of n elements that are not a whole number of 128-byte lines: the
+// four-step row passes stage them through LDS.
+static bool rows_misaligned(size_t n, size_t elem) { return (n * elem) % 128 != 0; }
+
 template <typename V>
 static void fill_pass(FsArgs<V>& a, const LdsGeom& g, size_t len, const dev::PassDesc& pd,
                       const void* tw) {
@@ -1150,7 +1163,9 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     a.perm = p->perm_in;
     HIP_TRY(launch_fs_pass(p->prec, true, colA, dir, &a, cb, p->gcol, p->fixed_col, s));
     fill_pass(a, p->grow, p->n2, p->pdrow, p->twrow);
+    a.stage_rows = rows_misaligned(p->n2, elem_bytes(p->prec));
     HIP_TRY(launch_fs_pass(p->prec, false, rowB, dir, &a, cb, p->grow, p->fixed_row, s));
+    a.stage_rows = 0;
     if (colC >= 0) {
       fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
       a.perm = p->perm_out;
@@ -1506,6 +1521,7 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap) 
       g.kind = s->kind == kFourStep ? kGkFsPlain : (s->kind == kRaderFs ? kGkFsRader : kGkFsBlue);
       fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol, &r.col_cls);
       fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow, &r.row_cls);
+      g.row.stage_rows = rows_misaligned(s->n2, sizeof(V));
       g.col_ppt = s->gcol.ppt;
       g.row_ppt = s->grow.ppt;
       break;

@@ -443,7 +443,39 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
   }
 }
 
-// Row pass over rows [r0, r0 + C) of one transform's workspace.
+// The workgroup's rows [r0, r0 + rows) of the workspace (one contiguous
+// span) into LDS row slots with coalesced loads, up to P per thread.
+template <typename V, int P>
+__device__ __forceinline__ void stage_rows(const V* span, int rows, int N2, V* lds_all, int stride, int ps) {
+  const int total = rows * N2, nt = blockDim.x;
+  V buf[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int i = threadIdx.x + j * nt;
+    if (i < total) buf[j] = gload(span + i);
+  }
+  int r = threadIdx.x / N2, c = threadIdx.x - (threadIdx.x / N2) * N2;
+  const int dr = nt / N2, dc = nt - dr * N2;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int i = threadIdx.x + j * nt;
+    if (i < total) lds_all[r * stride + c + (c >> ps)] = buf[j];
+    r += dr;
+    c += dc;
+    if (c >= N2) {
+      c -= N2;
+      ++r;
+    }
+  }
+  __syncthreads();
+}
+
+// Row pass over rows [r0, r0 + C) of one transform's workspace.  Rows whose
+// length is a whole number of 128-byte lines are read by each slot's first
+// pass straight from HBM (5-11 % faster than staging them); other rows start
+// mid-line, so a slot's loads touch two lines per segment and the pass read
+// 1.7x its bytes (N = 40500 = 162 x 250): those are staged (a.stage_rows)
+// as the workgroup's one contiguous span with coalesced loads.
 template <typename V, typename F, int DIR, int MODE, typename E>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, const V* x0,
                                             int r0, V* lds_in) {
@@ -455,12 +487,17 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
   const int rows = min(C, N1 - r0);
   const bool active = slot < rows;
   V* lds = lds_all + (slot < C ? slot : 0) * stride;
+  if (a.stage_rows) stage_rows<V, F::kPpt>(tmp + (long long)r0 * N2, rows, N2, lds_all, stride, ps);
   if constexpr (MODE == kFsPlainB) {
-    // each slot's first pass reads its (contiguous) row straight from HBM;
-    // the last pass leaves the row in LDS for the transposed store
+    // each slot's first pass reads its (contiguous) row straight from HBM
+    // (or LDS, staged); the last pass leaves the row in LDS for the
+    // transposed store
     const V* src = tmp + (long long)(r0 + (active ? slot : 0)) * N2;
-    F::template run_io<DIR>(GlobalSrc<V>{(gbl_cptr<V>)src}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, N2, a.threads,
-                            tid, a.pd, a.tw, active, ps);
+    if (a.stage_rows)
+      F::template run<DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    else
+      F::template run_io<DIR>(GlobalSrc<V>{(gbl_cptr<V>)src}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, N2,
+                              a.threads, tid, a.pd, a.tw, active, ps);
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
       if (c >= rows) continue;
@@ -475,10 +512,12 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     V* row = tmp + k1 * N2;
     const bool bin0 = x0 != nullptr && r0 == 0 && slot == 0;
     const V xz = bin0 ? *x0 : V{0, 0};
-    F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row},
-                           KernSink<V, E>{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)(a.kern + k1 * N2),
-                                          bin0 ? (gbl_ptr<E>)out : nullptr, xz, a.scale},
-                           lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    const KernSink<V, E> ks{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)(a.kern + k1 * N2), bin0 ? (gbl_ptr<E>)out : nullptr,
+                            xz, a.scale};
+    if (a.stage_rows)
+      F::template run_io<-1>(LdsSrc<V>{(lds_ptr<V>)lds, ps}, ks, lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+    else
+      F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row}, ks, lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     F::template run_io<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps},
                            TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1},
                            lds, N2, a.threads, tid, a.pd, a.tw, active, ps);

@@ -102,6 +102,9 @@ template <typename V> struct FsArgs {
   V* zw = nullptr;
   long long zw_dist = 0;
   const int* lidx = nullptr;
+  // row passes: stage the workgroup's rows through LDS with coalesced loads
+  // (rows that are not a whole number of 128-byte lines)
+  int stage_rows = 0;
 };
 
 // One four-step pass; `args` points at FsArgs<float2> or FsArgs<double2>.
