@@ -52,14 +52,16 @@ struct StridedPass {
 
 // One family launch of a grouped plan: the tasks [base, base + count) of the
 // device task table, split per XCD at xbase, with its workgroup size class,
-// dynamic LDS, grid and estimated cost (for ordering and branch balance).
+// dynamic LDS, grid, estimated cost, time measured at plan time (ms, 0 if
+// not measured), and the phase and branch it is scheduled on.
 struct GList {
   int fam = 0, cls = 0, hi = 0;
   int base = 0, count = 0;
   int xbase[dev::kXcds + 1] = {};
   size_t lds = 0;
   unsigned grid = 0;
-  double cost = 0;
+  double cost = 0, ms = 0;
+  int phase = 0, branch = 0;
 };
 // Parallel branches the family launches of one phase are spread over
 // (the box's hardware queues per process).
@@ -164,6 +166,7 @@ struct fyfft_plan {
   std::vector<GList> glists;                 // family launches, by phase then decreasing cost
   hipStream_t gbranch[kGroupBranches] = {};  // branch streams of multi-family phases
   hipEvent_t gfork[3] = {}, gjoin[3][kGroupBranches] = {};
+  double gspan_ms = 0;  // sum over phases of the longest branch (measured weights; 0: estimates)
   void* gstage = nullptr;  // 16-bit per-transform baseline: FP32 copy of the longest transform
   mutable std::mutex graph_mu;
   mutable std::map<std::tuple<const void*, void*, int, hipStream_t>, hipGraphExec_t> graphs;
@@ -1085,7 +1088,10 @@ static fyfft_status plan_bluestein(fyfft_plan* p) {
 
 // ------------------------------------------------------------ execution
 // Rows of n elements that are not a whole number of 128-byte lines: the
-// four-step row passes stage them through LDS.
+// four-step row passes stage them through LDS, and strided segments of such
+// rows (column-pass reads, transposed row-pass stores) go through the caches
+// instead of streaming, since they share their first and last lines with
+// the neighbouring segments (ragged grouped step 6.81 -> 6.70 ms).
 static bool rows_misaligned(size_t n, size_t elem) { return (n * elem) % 128 != 0; }
 
 template <typename V>
@@ -1161,6 +1167,8 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
 
     fill_pass(a, p->gcol, p->n1, p->pdcol, p->twcol);
     a.perm = p->perm_in;
+    a.col_cached = rows_misaligned(p->n2, elem_bytes(p->prec));
+    a.out_cached = rows_misaligned(p->n1, elem_bytes(p->prec));
     HIP_TRY(launch_fs_pass(p->prec, true, colA, dir, &a, cb, p->gcol, p->fixed_col, s));
     fill_pass(a, p->grow, p->n2, p->pdrow, p->twrow);
     a.stage_rows = rows_misaligned(p->n2, elem_bytes(p->prec));
@@ -1430,23 +1438,30 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
 // count and LDS budget (cap * threads / 1024) hold it.
 static size_t class_lds_cap(size_t cap, int k) { return cap * (size_t)kGroupClassThreads[k] / 1024; }
 
-// Class and column count of a four-step sub-pass: 8 to 16 sub-transforms
-// per workgroup keep row segments at 64 bytes or more; below that, the
-// largest class takes whatever fits.
+// Class and column count of a four-step sub-pass.  A workgroup's row
+// segments are C elements wide; a strided FP32 copy with 128-byte segments
+// ran at 60-65 % of HBM against 33-47 % with 64-byte ones, so 16 columns are
+// taken in the smallest class that holds them (the two smaller classes
+// only: a 1024-thread class holds one workgroup per CU), then 8; below
+// that, the largest class takes whatever fits.
 static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t cap, int* cls, int* log2c) {
   const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem;
-  for (int k = 0; k < kGroupClasses; ++k) {
-    int c = 16;
-    while (c > 1 && ((size_t)c * q.threads > (size_t)kGroupClassThreads[k] || c * per > class_lds_cap(cap, k))) c >>= 1;
-    if ((size_t)c * q.threads > (size_t)kGroupClassThreads[k] || c * per > class_lds_cap(cap, k)) continue;
-    if (c >= 8 || k == kGroupClasses - 1) {
-      *cls = k;
-      *log2c = ilog2_exact(c);
-      return;
+  auto fits = [&](int c, int k) {
+    return (size_t)c * q.threads <= (size_t)kGroupClassThreads[k] && c * per <= class_lds_cap(cap, k);
+  };
+  for (int want : {16, 8})
+    for (int k = 0; k < kGroupClasses; ++k) {
+      if (want == 16 && k == kGroupClasses - 1) continue;
+      if (fits(want, k)) {
+        *cls = k;
+        *log2c = ilog2_exact(want);
+        return;
+      }
     }
-  }
+  int c = 16;
+  while (c > 1 && !fits(c, kGroupClasses - 1)) c >>= 1;
   *cls = kGroupClasses - 1;
-  *log2c = 0;
+  *log2c = ilog2_exact(c);
 }
 
 template <typename V>
@@ -1457,7 +1472,7 @@ struct GLenPlan {
 };
 
 template <typename V>
-static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap) {
+static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, size_t io_elem) {
   GLenPlan<V> r;
   GLen<V>& g = r.g;
   g.n = (int)s->n;
@@ -1522,11 +1537,131 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap) 
       fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol, &r.col_cls);
       fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow, &r.row_cls);
       g.row.stage_rows = rows_misaligned(s->n2, sizeof(V));
+      g.col.col_cached = rows_misaligned(s->n2, io_elem);
+      g.row.out_cached = rows_misaligned(s->n1, io_elem);
       g.col_ppt = s->gcol.ppt;
       g.row_ppt = s->grow.ppt;
       break;
   }
   return r;
+}
+
+template <typename V>
+static GroupedArgs<V> grouped_args(const fyfft_plan* p, const void* in, void* out, int direction) {
+  GroupedArgs<V> a;
+  a.lens = (const GLen<V>*)p->d_lens;
+  a.off = (const long long*)p->d_off;
+  a.woff = (const long long*)p->d_woff;
+  a.in = in;
+  a.out = out;
+  a.ws = (V*)p->gws;
+  a.x0buf = (V*)p->gx0;
+  a.tables = direction != FYFFT_FORWARD;
+  a.normalize = direction == FYFFT_INVERSE;
+  return a;
+}
+
+template <typename V>
+static hipError_t launch_list(const fyfft_plan* p, const GList& L, GroupedArgs<V>& a, int dir, hipStream_t st) {
+  a.tasks = (const GTask*)p->d_tasks + L.base;
+  for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = L.xbase[x];
+  return group_family(p->prec, false, L.fam, dir, L.hi != 0, kGroupClassThreads[L.cls], &a, L.grid, L.lds, st);
+}
+
+// Times every family launch of a grouped plan on its own (forward, on a
+// zeroed scratch copy of the data, one warm-up round then the best of two),
+// in phase order so each reads what the earlier phases wrote.  Leaves
+// GList::ms at 0 when anything fails (scheduling then uses the estimates).
+template <typename V>
+static void time_grouped_lists(fyfft_plan* p) {
+  const size_t bytes = std::max<size_t>(1, p->total) * elem_bytes(p->prec);
+  void* buf = nullptr;
+  if (hipMalloc(&buf, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return;
+  }
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+            hipMemsetAsync(buf, 0, bytes, st) == hipSuccess;
+  GroupedArgs<V> a = grouped_args<V>(p, buf, buf, FYFFT_FORWARD);
+  std::vector<size_t> order(p->glists.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    return family_phase(p->glists[x].fam) < family_phase(p->glists[y].fam);
+  });
+  std::vector<double> best(order.size(), 1e30);
+  for (int rep = 0; ok && rep < 3; ++rep)
+    for (size_t i : order) {
+      float ms = 0;
+      ok = ok && hipEventRecord(e0, st) == hipSuccess && launch_list(p, p->glists[i], a, -1, st) == hipSuccess &&
+           hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+           hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+      if (ok && rep > 0) best[i] = std::min(best[i], (double)ms);
+    }
+  if (ok)
+    for (size_t i = 0; i < order.size(); ++i) p->glists[i].ms = std::max(best[i], 1e-3);
+  else
+    (void)hipGetLastError();
+  if (st) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(buf);
+}
+
+// Places the family launches on phases and branches: four-step passes keep
+// their phase (column passes, row passes, closing passes), longest first
+// onto the least loaded branch; single-pass families (LDS-resident plain,
+// Rader and Bluestein transforms) depend on no other launch, so each goes,
+// longest first, to the phase and branch where it lengthens the sum of the
+// phase spans least (the least loaded such branch on a tie).  Weights are
+// the plan-time measurements when every launch has one, else the estimates.
+// The launches are then ordered by phase, longest first.
+static void schedule_branches(fyfft_plan* p) {
+  bool measured = !p->glists.empty();
+  for (const GList& L : p->glists) measured = measured && L.ms > 0;
+  auto w = [&](const GList& L) { return measured ? L.ms : L.cost; };
+  const int nb = p->gbranch[0] ? kGroupBranches : 1;
+  double load[3][kGroupBranches] = {};
+  bool bound_phase[3] = {false, false, false};
+  std::vector<GList*> bound, free_lists;
+  for (GList& L : p->glists) {
+    L.phase = family_phase(L.fam);
+    (L.fam <= kGfBlue ? free_lists : bound).push_back(&L);
+  }
+  auto heavier = [&](const GList* x, const GList* y) { return w(*x) > w(*y); };
+  std::stable_sort(bound.begin(), bound.end(), heavier);
+  std::stable_sort(free_lists.begin(), free_lists.end(), heavier);
+  for (GList* L : bound) {
+    const int b = (int)(std::min_element(load[L->phase], load[L->phase] + nb) - load[L->phase]);
+    load[L->phase][b] += w(*L);
+    L->branch = b;
+    bound_phase[L->phase] = true;
+  }
+  if (!bound_phase[1] && !bound_phase[2]) bound_phase[0] = true;
+  for (GList* L : free_lists) {
+    int bp = 0, bb = 0;
+    double best_d = 1e300, best_l = 1e300;
+    for (int ph = 0; ph < 3; ++ph) {
+      if (!bound_phase[ph]) continue;
+      const double span = *std::max_element(load[ph], load[ph] + nb);
+      for (int b = 0; b < nb; ++b) {
+        const double l = load[ph][b] + w(*L), d = std::max(span, l) - span;
+        if (d < best_d || (d == best_d && l < best_l)) best_d = d, best_l = l, bp = ph, bb = b;
+      }
+    }
+    load[bp][bb] += w(*L);
+    L->phase = bp;
+    L->branch = bb;
+  }
+  std::stable_sort(p->glists.begin(), p->glists.end(), [&](const GList& x, const GList& y) {
+    return x.phase != y.phase ? x.phase < y.phase : w(x) > w(y);
+  });
+  p->gspan_ms = 0;
+  if (measured)
+    for (int ph = 0; ph < 3; ++ph) p->gspan_ms += *std::max_element(load[ph], load[ph] + nb);
 }
 
 template <typename V>
@@ -1551,7 +1686,7 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
   for (const auto& kv : by_len) {
     const fyfft_plan* s = p->subs[li];
     const std::vector<size_t>& idx = kv.second;
-    const GLenPlan<V> lp = make_glen<V>(s, idx.size(), cap);
+    const GLenPlan<V> lp = make_glen<V>(s, idx.size(), cap, elem_bytes(p->prec));
     lens.push_back(lp.g);
     const GLen<V>& g = lens.back();
     const int first = (int)off.size();
@@ -1658,22 +1793,21 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
   HIP_TRY(hipMemcpy(p->d_lens, lens.data(), lens.size() * sizeof(GLen<V>), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_off, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_woff, woff.data(), woff.size() * sizeof(long long), hipMemcpyHostToDevice));
-  int phase_lists[3] = {0, 0, 0};
-  for (const GList& L : p->glists) {
-    ++phase_lists[family_phase(L.fam)];
+  for (const GList& L : p->glists)
     for (int dir : {-1, +1})
       HIP_TRY(group_family(p->prec, true, L.fam, dir, L.hi != 0, kGroupClassThreads[L.cls], nullptr, 0, L.lds,
                            nullptr));
-  }
-  // branch streams and fork/join events for phases with several families
+  // branch streams and fork/join events when there are several launches
   // (default priority: streams of other priorities have hung these GPUs)
-  if (std::max({phase_lists[0], phase_lists[1], phase_lists[2]}) > 1 && !(p->gflags & FYFFT_GROUPED_PER_TRANSFORM)) {
+  if (p->glists.size() > 1 && !(p->gflags & FYFFT_GROUPED_PER_TRANSFORM)) {
     for (hipStream_t& b : p->gbranch) HIP_TRY(hipStreamCreateWithFlags(&b, hipStreamNonBlocking));
     for (int ph = 0; ph < 3; ++ph) {
       HIP_TRY(hipEventCreateWithFlags(&p->gfork[ph], hipEventDisableTiming));
       for (hipEvent_t& e : p->gjoin[ph]) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
   }
+  if (p->gbranch[0]) time_grouped_lists<V>(p);
+  schedule_branches(p);
   size_t ntask[3] = {0, 0, 0};
   for (const GList& L : p->glists) ntask[family_phase(L.fam)] += (size_t)L.count;
   static const char* kStore[] = {"f32", "f64", "bf16", "fp16"};
@@ -1681,54 +1815,46 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
             " total=" + std::to_string(p->total) + " storage=" + kStore[p->prec] + " tasks=" +
             std::to_string(ntask[0]) + "/" + std::to_string(ntask[1]) + "/" + std::to_string(ntask[2]) +
             " family-launches=" + std::to_string(p->glists.size()) + " workspace=" + std::to_string(wsize);
+  if (p->gspan_ms > 0) {
+    int per_phase[3] = {0, 0, 0};
+    for (const GList& L : p->glists) ++per_phase[L.phase];
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), " schedule=measured phases=%d/%d/%d est-ms=%.3f", per_phase[0], per_phase[1],
+                  per_phase[2], p->gspan_ms);
+    p->desc += buf;
+  }
   return FYFFT_SUCCESS;
 }
 
 template <typename V>
 static fyfft_status launch_grouped_phases(const fyfft_plan* p, const void* in, void* out, int direction,
                                           hipStream_t s) {
-  GroupedArgs<V> a;
-  a.lens = (const GLen<V>*)p->d_lens;
-  a.off = (const long long*)p->d_off;
-  a.woff = (const long long*)p->d_woff;
-  a.in = in;
-  a.out = out;
-  a.ws = (V*)p->gws;
-  a.x0buf = (V*)p->gx0;
-  a.tables = direction != FYFFT_FORWARD;
-  a.normalize = direction == FYFFT_INVERSE;
+  GroupedArgs<V> a = grouped_args<V>(p, in, out, direction);
   const int dir = direction == FYFFT_FORWARD ? -1 : +1;
-  auto launch = [&](const GList& L, hipStream_t st) {
-    a.tasks = (const GTask*)p->d_tasks + L.base;
-    for (int x = 0; x <= dev::kXcds; ++x) a.xbase[x] = L.xbase[x];
-    return group_family(p->prec, false, L.fam, dir, L.hi != 0, kGroupClassThreads[L.cls], &a, L.grid, L.lds, st);
-  };
-  size_t i = 0;
   for (int ph = 0; ph < 3; ++ph) {
-    size_t j = i;
-    while (j < p->glists.size() && family_phase(p->glists[j].fam) == ph) ++j;
-    const int n = (int)(j - i);
-    if (n == 1 || (n > 1 && !p->gbranch[0])) {
-      for (size_t k = i; k < j; ++k) HIP_TRY(launch(p->glists[k], s));
-    } else if (n > 1) {
-      // fork: every family of the phase waits for the previous phase, runs on
-      // the least loaded branch (lists come by decreasing cost), and the
-      // stream waits for every branch before the next phase
-      const int nb = std::min(n, kGroupBranches);
-      HIP_TRY(hipEventRecord(p->gfork[ph], s));
-      for (int b = 0; b < nb; ++b) HIP_TRY(hipStreamWaitEvent(p->gbranch[b], p->gfork[ph], 0));
-      double load[kGroupBranches] = {0};
-      for (size_t k = i; k < j; ++k) {
-        const int b = (int)(std::min_element(load, load + nb) - load);
-        load[b] += p->glists[k].cost;
-        HIP_TRY(launch(p->glists[k], p->gbranch[b]));
-      }
-      for (int b = 0; b < nb; ++b) {
+    bool used[kGroupBranches] = {};
+    int nused = 0;
+    for (const GList& L : p->glists)
+      if (L.phase == ph && !used[L.branch]) used[L.branch] = true, ++nused;
+    if (nused == 0) continue;
+    if (nused == 1 || !p->gbranch[0]) {
+      for (const GList& L : p->glists)
+        if (L.phase == ph) HIP_TRY(launch_list(p, L, a, dir, s));
+      continue;
+    }
+    // fork: the phase's launches wait for the previous phase and run on
+    // their scheduled branches (longest first within a branch); the stream
+    // waits for every branch before the next phase
+    HIP_TRY(hipEventRecord(p->gfork[ph], s));
+    for (int b = 0; b < kGroupBranches; ++b)
+      if (used[b]) HIP_TRY(hipStreamWaitEvent(p->gbranch[b], p->gfork[ph], 0));
+    for (const GList& L : p->glists)
+      if (L.phase == ph) HIP_TRY(launch_list(p, L, a, dir, p->gbranch[L.branch]));
+    for (int b = 0; b < kGroupBranches; ++b)
+      if (used[b]) {
         HIP_TRY(hipEventRecord(p->gjoin[ph][b], p->gbranch[b]));
         HIP_TRY(hipStreamWaitEvent(s, p->gjoin[ph][b], 0));
       }
-    }
-    i = j;
   }
   return FYFFT_SUCCESS;
 }

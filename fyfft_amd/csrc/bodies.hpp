@@ -383,7 +383,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
               const int n1 = i >> lc, c = i & (C - 1);
               const int n = n1 * N2 + c0 + (c < ncols ? c : 0);
               V v;
-              if constexpr (MODE == kFsPlainA) v = El::ldnt(x + n);
+              if constexpr (MODE == kFsPlainA) v = a.col_cached ? El::ld(x + n) : El::ldnt(x + n);
               else if constexpr (MODE == kFsRaderA) v = El::ld(x + a.perm[n]);
               else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(El::ld(x + n), a.chirp[n]) : V{0, 0};
               else v = tmp[n];
@@ -501,8 +501,12 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
       if (c >= rows) continue;
-      Elem<V, E>::stnt(out + (long long)(r0 + c) + (long long)N1 * k2,
-                       cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale));
+      E* dst = out + (long long)(r0 + c) + (long long)N1 * k2;
+      const V v = cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale);
+      if (a.out_cached)
+        Elem<V, E>::st(dst, v);
+      else
+        Elem<V, E>::stnt(dst, v);
     }
   } else {  // kFsConvB: one row per slot, read and written in place, contiguous
     // forward FFT straight from HBM, kernel multiply as the last pass's sink,

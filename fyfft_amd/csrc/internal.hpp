@@ -105,6 +105,11 @@ template <typename V> struct FsArgs {
   // row passes: stage the workgroup's rows through LDS with coalesced loads
   // (rows that are not a whole number of 128-byte lines)
   int stage_rows = 0;
+  // column pass: input rows that are not a whole number of 128-byte lines
+  // are read through the caches (a workgroup's row segments then share
+  // their first and last lines with the neighbouring column groups);
+  // row pass: the same for the transposed output when N1 is not
+  int col_cached = 0, out_cached = 0;
 };
 
 // One four-step pass; `args` points at FsArgs<float2> or FsArgs<double2>.

@@ -165,7 +165,8 @@ def test_mix_modes_argument_checks_without_gpu(fy):
     null = ctypes.c_void_p(0)
     assert L.fyfft_mix_modes(0, 0, null, null, null, 0, 4, 4, 4, null) == 0      # empty: nothing to do
     assert L.fyfft_mix_modes(9, 0, null, null, null, 1, 4, 4, 4, null) == 1      # unknown kind
-    assert L.fyfft_mix_modes(0, 1, null, null, null, 1, 4, 4, 4, null) == 3      # FP64: unsupported
+    assert L.fyfft_mix_modes(0, 2, null, null, null, 1, 4, 4, 4, null) == 3      # BF16: unsupported
+    assert L.fyfft_mix_modes(0, 1, null, null, null, 1, 4, 4, 4, null) == 1      # FP64, null operands
     assert L.fyfft_mix_modes(0, 0, null, null, null, 1, 4, 4, 4, null) == 1      # null operands
     assert L.fyfft_mix_modes(0, 0, ctypes.c_void_p(8), ctypes.c_void_p(8), ctypes.c_void_p(8),
                              70000, 4, 4, 4, null) == 1                          # k > 65535
