@@ -7,6 +7,7 @@
 #include <map>
 #include <tuple>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -61,6 +62,7 @@ struct GList {
   size_t lds = 0;
   unsigned grid = 0;
   double cost = 0, ms = 0;
+  double bytes = 0;  // HBM bytes the launch moves (data, workspace, kernel spectra)
   int phase = 0, branch = 0;
 };
 // Parallel branches the family launches of one phase are spread over
@@ -257,6 +259,11 @@ static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_f
 // Geometry of a four-step pass whose LDS FFT has length len: up to 16
 // sub-transforms (columns or rows) per workgroup, and at least two
 // workgroups per CU where the LDS allows.
+// FsArgs::inv_n of a four-step length: 1/n rounded to FP32 where the FP32
+// kernels compute their twiddles (n <= 2^24: every remainder of the
+// octant reduction is exact in FP32), 0 (table lookups) beyond.
+static float fs_inv_n(size_t n) { return n <= ((size_t)1 << 24) ? (float)(1.0 / (double)n) : 0.0f; }
+
 static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int force_c = 0) {
   if (!lds_geometry(len, elem, cap, g)) return false;
   const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
@@ -1143,6 +1150,7 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     a.tw_hi = (const V*)p->tw_hi;
     a.tw_lo = (const V*)p->tw_lo;
     a.lo_bits = p->lo_bits;
+    a.inv_n = fs_inv_n(p->n1 * p->n2);
     a.n_orig = (int)n;
     a.scale = (dev::S<V>)scale;
     const int slot = inv_tables ? 1 : 0;
@@ -1507,6 +1515,7 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
     a.tw_hi = (const V*)s->tw_hi;
     a.tw_lo = (const V*)s->tw_lo;
     a.lo_bits = s->lo_bits;
+    a.inv_n = fs_inv_n(s->n1 * s->n2);
     a.n_orig = (int)s->n;
     a.perm = s->perm_in;
     a.chirp = (const V*)s->chirp[0];
@@ -1667,12 +1676,13 @@ static void schedule_branches(fyfft_plan* p) {
 template <typename V>
 static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vector<size_t>>& by_len) {
   const size_t cap = lds_cap_bytes(p->device), elem = sizeof(V);
+  const double eio = (double)elem_bytes(p->prec);  // storage bytes per element of the data
   constexpr int kLo = sizeof(V) == 8 ? 16 : 8;  // points-per-thread bound of the lo family kernels
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->device) != hipSuccess || ncu <= 0) ncu = 256;
   std::vector<GLen<V>> lens;
   std::vector<long long> off, woff;
-  struct T { GTask t; double cost; };
+  struct T { GTask t; double cost; double bytes; };
   // (family, class, hi) -> tasks, LDS
   std::map<std::tuple<int, int, int>, std::vector<T>> lists;
   std::map<std::tuple<int, int, int>, size_t> list_lds;
@@ -1705,7 +1715,8 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
       if (hi && kGroupClassThreads[k] != 1024) return FYFFT_UNSUPPORTED;
       for (size_t i = 0; i < idx.size(); i += pb) {
         const int c = (int)std::min<size_t>(pb, idx.size() - i);
-        add(fam, k, hi, T{GTask{li, 0, first + (int)i, c}, conv * c * m * lg}, (size_t)pb * g.lds.stride * elem);
+        const double by = (double)c * (2.0 * g.n * eio + (g.kind == kGkLds ? 0.0 : (double)m * elem));
+        add(fam, k, hi, T{GTask{li, 0, first + (int)i, c}, conv * c * m * lg, by}, (size_t)pb * g.lds.stride * elem);
       }
     } else {
       if (g.col_ppt > kLo || g.row_ppt > kLo) return FYFFT_UNSUPPORTED;
@@ -1718,12 +1729,16 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
       const size_t col_lds = (size_t)C * g.col.stride * elem, row_lds = (size_t)R * g.row.stride * elem;
       for (size_t i = 0; i < idx.size(); ++i) {
         for (int c0 = 0; c0 < g.col.n2; c0 += C) {
-          add(famA, kc, 0, T{GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg}, col_lds);
+          const double frac = (double)g.n / ((double)g.col.n1 * g.col.n2);  // data share of the conv length
+          add(famA, kc, 0, T{GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg,
+                             (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);
           if (g.kind != kGkFsPlain)
-            add(famC, kc, 0, T{GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg}, col_lds);
+            add(famC, kc, 0, T{GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg,
+                               (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);
         }
         for (int r0 = 0; r0 < g.col.n1; r0 += R)
-          add(famB, kr, 0, T{GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg}, row_lds);
+          add(famB, kr, 0, T{GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg,
+                             (double)R * g.col.n2 * (g.kind == kGkFsPlain ? elem + eio : 3.0 * elem)}, row_lds);
       }
     }
     ++li;
@@ -1748,7 +1763,7 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
     std::vector<std::pair<double, std::vector<T>*>> order;
     for (auto& g : groups) {
       double c = 0;
-      for (const T& t : g.second) c += t.cost;
+      for (const T& t : g.second) c += t.cost, L.bytes += t.bytes;
       order.push_back({c, &g.second});
       L.cost += c;
     }
@@ -1808,6 +1823,13 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
   }
   if (p->gbranch[0]) time_grouped_lists<V>(p);
   schedule_branches(p);
+  if (const char* rep = std::getenv("FYFFT_GROUPED_REPORT"); rep && *rep == '1') {
+    // one line per family launch: its plan-time measurement alone on the GPU
+    for (const GList& L : p->glists)
+      std::fprintf(stderr, "grouped-list fam=%d cls=%d hi=%d tasks=%d grid=%u lds=%zu phase=%d branch=%d "
+                   "ms=%.4f MB=%.1f GB/s=%.0f\n", L.fam, kGroupClassThreads[L.cls], L.hi, L.count, L.grid, L.lds,
+                   L.phase, L.branch, L.ms, L.bytes / 1e6, L.ms > 0 ? L.bytes / L.ms / 1e6 : 0.0);
+  }
   size_t ntask[3] = {0, 0, 0};
   for (const GList& L : p->glists) ntask[family_phase(L.fam)] += (size_t)L.count;
   static const char* kStore[] = {"f32", "f64", "bf16", "fp16"};

@@ -208,8 +208,16 @@ template <typename V> struct TwSink {
   gbl_cptr<V> hi, lo;
   int lo_bits;
   long long k1;
+  int n;        // n1 * n2
+  float inv_n;  // nonzero: FP32 root computed (big_twiddle)
   __device__ void operator()(int k2, V v) const {
     const long long m = (long long)k2 * k1;
+    if constexpr (sizeof(V) == 8) {
+      if (inv_n != 0.0f) {
+        ((V*)y)[k2] = cmul(v, root_f32<+1>((int)m, n, inv_n));
+        return;
+      }
+    }
     const V w = cmul(((const V*)hi)[m >> lo_bits], ((const V*)lo)[m & ((1LL << lo_bits) - 1)]);
     ((V*)y)[k2] = cmulc(v, w);
   }
@@ -355,8 +363,15 @@ __device__ __forceinline__ void body_blue(const LdsBody<V>& a, const E* x, E* y,
 }
 
 // ------------------------------------------------------------ four-step
+// The four-step twiddle w_N^m (N = n1 * n2): computed in FP32 (one
+// element's worth of VALU work instead of two scattered cache lookups; as
+// accurate as the product of two rounded table entries), from the hi/lo
+// tables in FP64 and beyond 2^24 points.
 template <int DIR, typename V>
 __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
+  if constexpr (sizeof(V) == 8) {
+    if (a.inv_n != 0.0f) return root_f32<DIR>((int)m, a.n1 * a.n2, a.inv_n);
+  }
   const V w = cmul(a.tw_hi[m >> a.lo_bits], a.tw_lo[m & ((1LL << a.lo_bits) - 1)]);
   return DIR < 0 ? w : V{w.x, -w.y};
 }
@@ -523,7 +538,7 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     else
       F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row}, ks, lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
     F::template run_io<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps},
-                           TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1},
+                           TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1, N1 * N2, a.inv_n},
                            lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
   }
 }

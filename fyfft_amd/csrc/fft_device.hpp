@@ -663,6 +663,45 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   }
 }
 
+// w_n^m = exp(-2*pi*i*m/n) for 0 <= m < n <= 2^24 in FP32 (conjugated when
+// DIR > 0), computed instead of looked up: an exact integer reduction of 8m
+// to an octant q and remainder r (odd octants reflected, r -> n - r, so the
+// angle left is x * pi/4 with x = r/n in [0, 1]), x formed with one
+// residual correction (about half an ulp), then the Taylor polynomials of
+// sin(pi*x/4) and cos(pi*x/4) to degree 9 and 10 (truncation below 2e-9)
+// and the octant's symmetry.  The four-step twiddle w_N^(n2*k1) is needed
+// once per element at scattered exponents; two table lookups per element
+// (hi * lo) made the passes that apply it L2-request bound.
+template <int DIR>
+__device__ __forceinline__ float2 root_f32(int m, int n, float inv_n) {
+  const int t = m << 3;                                  // 8m < 2^27
+  int q = (int)((float)t * inv_n);                       // octant estimate, off by at most 1
+  int r = t - q * n;
+  if (r < 0) { r += n; --q; }
+  if (r >= n) { r -= n; ++q; }
+  const bool odd = q & 1;
+  const int rr = odd ? n - r : r;                        // reflect odd octants
+  const float rf = (float)rr, nf = (float)n;
+  float x = rf * inv_n;
+  x = __builtin_fmaf(__builtin_fmaf(-x, nf, rf), inv_n, x);
+  const float x2 = x * x;
+  // sin(pi x / 4), cos(pi x / 4): Taylor coefficients (pi/4)^k / k! with alternating signs
+  const float sn = x * __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2,
+                   (float)3.13361689037812e-07, (float)-3.657620418217724e-05), (float)0.00249039457019272),
+                   (float)-0.08074551218828077), (float)0.7853981633974483);
+  const float cs = __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2, __builtin_fmaf(x2,
+                   (float)-2.461136950494199e-08, (float)3.590860448591509e-06), (float)-0.00032599188692738996),
+                   (float)0.0158543442438155), (float)-0.30842513753404244), 1.0f);
+  const float c = odd ? sn : cs, s = odd ? cs : sn;      // cos / sin of the angle within its quadrant
+  const int qd = (q >> 1) & 3;
+  float re, im;
+  if (qd == 0) { re = c; im = s; }
+  else if (qd == 1) { re = -s; im = c; }
+  else if (qd == 2) { re = -c; im = -s; }
+  else { re = s; im = -c; }
+  return DIR < 0 ? float2{re, -im} : float2{re, im};
+}
+
 // ---------------------------------------------------------------- roots
 // exp(sign * 2*pi*i * m / D) in FP64 with an exact integer reduction of m to
 // a quadrant, so the only rounding is in r/D and in sincospi (|angle| <=

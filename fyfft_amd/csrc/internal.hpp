@@ -91,6 +91,7 @@ template <typename V> struct FsArgs {
   const V* tw_hi = nullptr;    // w_N^(i << lo_bits)
   const V* tw_lo = nullptr;    // w_N^i, i < 2^lo_bits
   int lo_bits = 0;
+  float inv_n = 0;             // 1 / (n1 * n2): FP32 twiddles are computed (dev::root_f32) for n1 * n2 <= 2^24
   int n_orig = 0;              // Rader/Bluestein: the user length
   const int* perm = nullptr;   // Rader gather (A) or scatter (C) permutation
   const V* chirp = nullptr;    // Bluestein pre/post chirp
