@@ -259,6 +259,9 @@ static fyfft_status plan_lds(fyfft_plan* p, size_t n, LdsLaunch* L, bool allow_f
 // Geometry of a four-step pass whose LDS FFT has length len: up to 16
 // sub-transforms (columns or rows) per workgroup, and at least two
 // workgroups per CU where the LDS allows.
+// LDS bytes of a copy of the Stockham table of a length (n - 1 entries).
+static size_t table_lds_bytes(size_t n, size_t elem) { return (std::max<size_t>(n, 2) - 1) * elem; }
+
 // FsArgs::inv_n of a four-step length: 1/n rounded to FP32 where the FP32
 // kernels compute their twiddles (n <= 2^24: every remainder of the
 // octant reduction is exact in FP32), 0 (table lookups) beyond.
@@ -280,6 +283,10 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int for
   if ((size_t)c * stride * elem > cap) return false;
   g->per_block = c;
   g->lds_bytes = (size_t)c * stride * elem;
+  // the runtime-pass sub-transform reads its Stockham table from a copy in
+  // LDS after the tile where that fits (compile-time sub-transforms ignore it)
+  const size_t tw = table_lds_bytes(len, elem);
+  g->tw_bytes = g->lds_bytes + tw <= cap ? tw : 0;
   return true;
 }
 
@@ -1452,10 +1459,17 @@ static size_t class_lds_cap(size_t cap, int k) { return cap * (size_t)kGroupClas
 // taken in the smallest class that holds them (the two smaller classes
 // only: a 1024-thread class holds one workgroup per CU), then 8; below
 // that, the largest class takes whatever fits.
-static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t cap, int* cls, int* log2c) {
+// The tile is followed by the LDS copy of the sub-length's Stockham table
+// (*tl) when some class holds a one-column tile and the table; otherwise
+// the table is read from global memory.
+static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t cap, int* cls, int* log2c,
+                             bool* tl) {
   const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem;
+  const size_t twb = table_lds_bytes(len, elem);
+  *tl = per + twb <= class_lds_cap(cap, kGroupClasses - 1);
+  const size_t extra = *tl ? twb : 0;
   auto fits = [&](int c, int k) {
-    return (size_t)c * q.threads <= (size_t)kGroupClassThreads[k] && c * per <= class_lds_cap(cap, k);
+    return (size_t)c * q.threads <= (size_t)kGroupClassThreads[k] && c * per + extra <= class_lds_cap(cap, k);
   };
   for (int want : {16, 8})
     for (int k = 0; k < kGroupClasses; ++k) {
@@ -1477,6 +1491,7 @@ struct GLenPlan {
   GLen<V> g;
   int per_block = 1;                   // LDS kinds: transforms per task
   int lds_cls = 0, col_cls = 0, row_cls = 0;
+  bool col_tl = false, row_tl = false;  // four-step: Stockham table in LDS after the tile
 };
 
 template <typename V>
@@ -1503,10 +1518,10 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
     r.per_block = (int)std::max<size_t>(1, std::min(pb, count_len));
   };
   auto fill_fs = [&](FsArgs<V>& a, const LdsGeom& q, size_t len, const dev::PassDesc& pd, const void* tw,
-                     int* cls) {
+                     int* cls, bool* tl) {
     a.n1 = (int)s->n1;
     a.n2 = (int)s->n2;
-    grouped_fs_class(q, len, sizeof(V), cap, cls, &a.log2c);
+    grouped_fs_class(q, len, sizeof(V), cap, cls, &a.log2c, tl);
     a.threads = q.threads;
     a.ps = q.pad_shift;
     a.stride = dev::lds_len((int)len, q.pad_shift);
@@ -1543,8 +1558,8 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
     case kBlueLds: g.kind = kGkBlue; fill_lds(s->m); break;
     default:
       g.kind = s->kind == kFourStep ? kGkFsPlain : (s->kind == kRaderFs ? kGkFsRader : kGkFsBlue);
-      fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol, &r.col_cls);
-      fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow, &r.row_cls);
+      fill_fs(g.col, s->gcol, s->n1, s->pdcol, s->twcol, &r.col_cls, &r.col_tl);
+      fill_fs(g.row, s->grow, s->n2, s->pdrow, s->twrow, &r.row_cls, &r.row_tl);
       g.row.stage_rows = rows_misaligned(s->n2, sizeof(V));
       g.col.col_cached = rows_misaligned(s->n2, io_elem);
       g.row.out_cached = rows_misaligned(s->n1, io_elem);
@@ -1726,18 +1741,21 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
       const int famA = g.kind == kGkFsPlain ? kGfColPlain : (g.kind == kGkFsRader ? kGfColRaderA : kGfColBlueA);
       const int famB = g.kind == kGkFsPlain ? kGfRowPlain : kGfRowConv;
       const int famC = g.kind == kGkFsRader ? kGfColRaderC : kGfColBlueC;
-      const size_t col_lds = (size_t)C * g.col.stride * elem, row_lds = (size_t)R * g.row.stride * elem;
+      // hi = 1: the global-table form of the family (no room for the table)
+      const int chi = lp.col_tl ? 0 : 1, rhi = lp.row_tl ? 0 : 1;
+      const size_t col_lds = (size_t)C * g.col.stride * elem + (lp.col_tl ? table_lds_bytes(g.col.n1, elem) : 0);
+      const size_t row_lds = (size_t)R * g.row.stride * elem + (lp.row_tl ? table_lds_bytes(g.col.n2, elem) : 0);
       for (size_t i = 0; i < idx.size(); ++i) {
         for (int c0 = 0; c0 < g.col.n2; c0 += C) {
           const double frac = (double)g.n / ((double)g.col.n1 * g.col.n2);  // data share of the conv length
-          add(famA, kc, 0, T{GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg,
-                             (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);
-          if (g.kind != kGkFsPlain)
-            add(famC, kc, 0, T{GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg,
+          add(famA, kc, chi, T{GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg,
                                (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);
+          if (g.kind != kGkFsPlain)
+            add(famC, kc, chi, T{GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg,
+                                 (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);
         }
         for (int r0 = 0; r0 < g.col.n1; r0 += R)
-          add(famB, kr, 0, T{GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg,
+          add(famB, kr, rhi, T{GTask{li, 1, first + (int)i, r0}, conv * R * g.col.n2 * lg,
                              (double)R * g.col.n2 * (g.kind == kGkFsPlain ? elem + eio : 3.0 * elem)}, row_lds);
       }
     }

@@ -38,20 +38,39 @@ __device__ __forceinline__ void staged(int start, int step, int total, LoadF ld,
 
 // FFT policies for the bodies: how the LDS-resident sub-transform is run.
 // GenericFft: runtime pass list (any smooth length), passes out of line.
-template <typename V, int PPT> struct GenericFft {
+// TL: the four-step bodies stage the sub-length's Stockham table in LDS
+// after their tile (n - 1 entries) and the passes read it there: the
+// runtime passes' twiddle loads from global memory sat on the critical path
+// of every pass (N = 96000 = 300 x 320 column pass 643 us with them, 444 us
+// with none at all).
+template <typename V, int PPT, bool TL = false> struct GenericFft {
   static constexpr int kPpt = PPT;
+  static constexpr bool kTwLds = TL;
   template <int D>
   static __device__ __forceinline__ void run(V* lds, int n, int t, int tid, const PassDesc& pd,
                                              const V* tw, bool active, int ps) {
-    lds_fft<V, PPT, D>(lds, n, t, tid, pd, tw, active, ps);
+    lds_fft<V, PPT, D, TL>(lds, n, t, tid, pd, tw, active, ps);
   }
   // the same transform with the first pass reading src and the last writing dst
   template <int D, typename Src, typename Dst>
   static __device__ __forceinline__ void run_io(Src src, Dst dst, V* lds, int n, int t, int tid,
                                                 const PassDesc& pd, const V* tw, bool active, int ps) {
-    lds_fft_io<V, PPT, D>(src, dst, lds, n, t, tid, pd, tw, active, ps);
+    lds_fft_io<V, PPT, D, TL>(src, dst, lds, n, t, tid, pd, tw, active, ps);
   }
 };
+
+// Copy of the n-entry (n - 1 used) Stockham table into LDS at `dst` when the
+// policy reads it there; the caller's next barrier publishes it (the first
+// pass of a transform uses no twiddles).  Returns the table the passes read.
+template <typename F, typename V>
+__device__ __forceinline__ const V* stage_table(const V* tw, int n, V* dst) {
+  if constexpr (F::kTwLds) {
+    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) dst[i] = tw[i];
+    return dst;
+  } else {
+    return tw;
+  }
+}
 
 template <typename V, int D, int N, int T, int Ls, int R, int... Rest>
 __device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, bool active, int ps) {
@@ -64,6 +83,7 @@ __device__ __forceinline__ void fixed_lds_passes(V* lds, int tid, const V* tw, b
 // passes inlined.  The plan must use exactly this geometry and pass list.
 template <typename V, int N, int T, int... Rs> struct FixedFft {
   static constexpr int kPpt = (N + T - 1) / T;
+  static constexpr bool kTwLds = false;
   template <int D>
   static __device__ __forceinline__ void run(V* lds, int, int, int tid, const PassDesc&,
                                              const V* tw, bool active, int ps) {
@@ -411,9 +431,10 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
   if constexpr (MODE == kFsRaderA) {
     if (c0 == 0 && threadIdx.x == 0) *x0 = El::ld(x);
   }
+  const V* tw = stage_table<F>(a.tw, N1, lds_all + C * stride);
   __syncthreads();
   const bool active = slot < C && slot < ncols;
-  F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, a.tw, active, ps);
+  F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, tw, active, ps);
   V xz = V{0, 0};
   if constexpr (MODE == kFsRaderC || MODE == kFsRaderC2) xz = *x0;
   if constexpr (MODE == kFsRaderC2) {
@@ -502,6 +523,9 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
   const int rows = min(C, N1 - r0);
   const bool active = slot < rows;
   V* lds = lds_all + (slot < C ? slot : 0) * stride;
+  // the table copy is published by the barrier of stage_rows or of the
+  // first pass, which reads no twiddles
+  const V* tw = stage_table<F>(a.tw, N2, lds_all + C * stride);
   if (a.stage_rows) stage_rows<V, F::kPpt>(tmp + (long long)r0 * N2, rows, N2, lds_all, stride, ps);
   if constexpr (MODE == kFsPlainB) {
     // each slot's first pass reads its (contiguous) row straight from HBM
@@ -509,10 +533,10 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     // transposed store
     const V* src = tmp + (long long)(r0 + (active ? slot : 0)) * N2;
     if (a.stage_rows)
-      F::template run<DIR>(lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+      F::template run<DIR>(lds, N2, a.threads, tid, a.pd, tw, active, ps);
     else
       F::template run_io<DIR>(GlobalSrc<V>{(gbl_cptr<V>)src}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, N2,
-                              a.threads, tid, a.pd, a.tw, active, ps);
+                              a.threads, tid, a.pd, tw, active, ps);
     for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
       const int k2 = i >> a.log2c, c = i & (C - 1);
       if (c >= rows) continue;
@@ -534,12 +558,12 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     const KernSink<V, E> ks{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)(a.kern + k1 * N2), bin0 ? (gbl_ptr<E>)out : nullptr,
                             xz, a.scale};
     if (a.stage_rows)
-      F::template run_io<-1>(LdsSrc<V>{(lds_ptr<V>)lds, ps}, ks, lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+      F::template run_io<-1>(LdsSrc<V>{(lds_ptr<V>)lds, ps}, ks, lds, N2, a.threads, tid, a.pd, tw, active, ps);
     else
-      F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row}, ks, lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+      F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row}, ks, lds, N2, a.threads, tid, a.pd, tw, active, ps);
     F::template run_io<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps},
                            TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1, N1 * N2, a.inv_n},
-                           lds, N2, a.threads, tid, a.pd, a.tw, active, ps);
+                           lds, N2, a.threads, tid, a.pd, tw, active, ps);
   }
 }
 

@@ -371,7 +371,13 @@ __device__ __forceinline__ int xcd_remap(int b, int total) {
 // as address_space(3) and tables as address_space(1) keeps ds_* / global_*
 // instructions instead of flat ones in out-of-line code.
 template <typename V> using lds_ptr = __attribute__((address_space(3))) V*;
+template <typename V> using lds_cptr = const __attribute__((address_space(3))) V*;
 template <typename V> using gbl_cptr = const __attribute__((address_space(1))) V*;
+// The Stockham twiddle table of a runtime pass loop: global memory, or (TL)
+// a copy in LDS that the workgroup staged once for all its transforms.
+template <bool TL, typename V> struct TwTable { using type = gbl_cptr<V>; };
+template <typename V> struct TwTable<true, V> { using type = lds_cptr<V>; };
+template <bool TL, typename V> using tw_ptr = typename TwTable<TL, V>::type;
 template <typename V> __device__ __forceinline__ V* as_lds(V* p) { return (V*)(lds_ptr<V>)p; }
 
 // ---------------------------------------------------------------- LDS layout
@@ -474,35 +480,36 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
 // One LDS -> LDS pass kept out of line: inlining every radix case into the
 // pass loop of lds_fft makes the register allocator keep all cases' arrays
 // live at once and spill; as separate functions each gets its own budget.
-template <typename V, int R, int MAXB, int DIR>
+template <typename V, int R, int MAXB, int DIR, bool TL>
 __device__ __noinline__ void lds_pass_ool(lds_ptr<V> lds_s, int N, int Ls, int T, int tid,
-                                          gbl_cptr<V> tw_g, bool active, int ps) {
+                                          tw_ptr<TL, V> tw_g, bool active, int ps) {
   V* lds = (V*)lds_s;
   stockham_pass<V, R, MAXB, DIR, false, false>(lds, lds, N, Ls, T, tid, (const V*)tw_g, active, 1, ps);
 }
 
 // Run every pass of `pd` on an LDS-resident transform (LDS -> LDS).
-// PPT bounds the points per thread (N <= PPT * T).
-template <typename V, int PPT, int DIR>
+// PPT bounds the points per thread (N <= PPT * T).  TL: tw_gen points at an
+// LDS copy of the table.
+template <typename V, int PPT, int DIR, bool TL = false>
 __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const PassDesc& pd,
                                         const V* __restrict__ tw_gen, bool active, int ps) {
   const lds_ptr<V> lds = (lds_ptr<V>)lds_gen;
-  const gbl_cptr<V> tw = (gbl_cptr<V>)tw_gen;
+  const tw_ptr<TL, V> tw = (tw_ptr<TL, V>)tw_gen;
   const unsigned long long code = pack_radices(pd);
   int Ls = 1;
   for (int p = 0; p < pd.npass; ++p) {
     const int R = radix_at(code, p);
     switch (R) {
-      case 2: lds_pass_ool<V, 2, cdiv(PPT, 2), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 3: lds_pass_ool<V, 3, cdiv(PPT, 3), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 4: lds_pass_ool<V, 4, cdiv(PPT, 4), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 5: lds_pass_ool<V, 5, cdiv(PPT, 5), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 7: lds_pass_ool<V, 7, cdiv(PPT, 7), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 8: lds_pass_ool<V, 8, cdiv(PPT, 8), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 9: lds_pass_ool<V, 9, cdiv(PPT, 9), DIR>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 2: lds_pass_ool<V, 2, cdiv(PPT, 2), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 3: lds_pass_ool<V, 3, cdiv(PPT, 3), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 4: lds_pass_ool<V, 4, cdiv(PPT, 4), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 5: lds_pass_ool<V, 5, cdiv(PPT, 5), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 7: lds_pass_ool<V, 7, cdiv(PPT, 7), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 8: lds_pass_ool<V, 8, cdiv(PPT, 8), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
+      case 9: lds_pass_ool<V, 9, cdiv(PPT, 9), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
       default:
-        if constexpr (kNo16<V, PPT>) lds_pass_ool<V, 8, cdiv(PPT, 8), DIR>(lds, N, Ls, T, tid, tw, active, ps);
-        else lds_pass_ool<V, 16, cdiv(PPT, 16), DIR>(lds, N, Ls, T, tid, tw, active, ps);
+        if constexpr (kNo16<V, PPT>) lds_pass_ool<V, 8, cdiv(PPT, 8), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps);
+        else lds_pass_ool<V, 16, cdiv(PPT, 16), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps);
         break;
     }
     Ls *= R;
@@ -582,9 +589,9 @@ __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, 
 }
 
 // Out-of-line form for the runtime pass loops.
-template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
+template <typename V, int R, int MAXB, int DIR, bool TL, typename Src, typename Dst>
 __device__ __noinline__ void io_pass_ool(Src src, Dst dst, int N, int Ls, int T, int tid,
-                                         gbl_cptr<V> tw_g, bool active) {
+                                         tw_ptr<TL, V> tw_g, bool active) {
   io_pass<V, R, MAXB, DIR>(src, dst, N, Ls, T, tid, (const V*)tw_g, active);
 }
 
@@ -627,12 +634,12 @@ template <bool NO16 = false, typename F> __device__ __forceinline__ void radix_s
 // Every pass of `pd` with the first reading `src` and the last writing
 // `dst`; `lds` (padded by ps) holds the transform between passes.  Every
 // thread of the workgroup must call it (barriers).
-template <typename V, int PPT, int DIR, typename Src, typename Dst>
+template <typename V, int PPT, int DIR, bool TL = false, typename Src, typename Dst>
 __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, int T, int tid,
                                            const PassDesc& pd, const V* __restrict__ tw_gen,
                                            bool active, int ps) {
   const lds_ptr<V> lds = (lds_ptr<V>)lds_gen;
-  const gbl_cptr<V> tw = (gbl_cptr<V>)tw_gen;
+  const tw_ptr<TL, V> tw = (tw_ptr<TL, V>)tw_gen;
   const LdsSrc<V> ls{lds, ps};
   const LdsSink<V> lk{lds, ps};
   const int P = pd.npass;
@@ -640,7 +647,7 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
     if (P == 1)
       radix_switch<kNo16<V, PPT>>(radix_at(pack_radices(pd), 0), [&](auto rc) {
         constexpr int R = decltype(rc)::value;
-        io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, 1, T, tid, tw, active);
+        io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(src, lk, N, 1, T, tid, tw, active);
       });
     else if (active)
       for (int i = tid; i < N; i += T) lk(i, src(i));
@@ -655,9 +662,9 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   for (int p = 0; p < P; ++p) {
     radix_switch<kNo16<V, PPT>>(radix_at(code, p), [&](auto rc) {
       constexpr int R = decltype(rc)::value;
-      if (p == 0) io_pass_ool<V, R, cdiv(PPT, R), DIR>(src, lk, N, Ls, T, tid, tw, active);
-      else if (p == P - 1) io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, dst, N, Ls, T, tid, tw, active);
-      else io_pass_ool<V, R, cdiv(PPT, R), DIR>(ls, lk, N, Ls, T, tid, tw, active);
+      if (p == 0) io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(src, lk, N, Ls, T, tid, tw, active);
+      else if (p == P - 1) io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(ls, dst, N, Ls, T, tid, tw, active);
+      else io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(ls, lk, N, Ls, T, tid, tw, active);
     });
     Ls *= radix_at(code, p);
   }

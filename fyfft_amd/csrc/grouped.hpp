@@ -44,7 +44,7 @@ template <int FAM> constexpr int family_mode() {
        : kFsBlueC;
 }
 
-template <typename V, typename E, int FAM, int DIR, int PPT>
+template <typename V, typename E, int FAM, int DIR, int PPT, bool TL>
 __device__ __forceinline__ void run_family(const GLen<V>& gl, const GTask& t, const TaskCtx<V>& c, const E* in,
                                            E* out) {
   if constexpr (FAM == kGfLds || FAM == kGfRader || FAM == kGfBlue) {
@@ -74,9 +74,9 @@ __device__ __forceinline__ void run_family(const GLen<V>& gl, const GTask& t, co
     const long long o = c.off[t.first], w = c.woff[t.first];
     V* x0 = c.x0buf + t.first;
     if constexpr (col)
-      body_fs_col<V, GenericFft<V, PPT>, DIR, MODE>(a, in + o, c.ws + w, out + o, x0, t.count, c.lds);
+      body_fs_col<V, GenericFft<V, PPT, TL>, DIR, MODE>(a, in + o, c.ws + w, out + o, x0, t.count, c.lds);
     else
-      body_fs_row<V, GenericFft<V, PPT>, DIR, MODE>(a, c.ws + w, out + o,
+      body_fs_row<V, GenericFft<V, PPT, TL>, DIR, MODE>(a, c.ws + w, out + o,
                                                     gl.kind == kGkFsRader ? x0 : nullptr, t.count, c.lds);
   }
 }
@@ -84,7 +84,7 @@ __device__ __forceinline__ void run_family(const GLen<V>& gl, const GTask& t, co
 // B threads per workgroup; at least four waves per SIMD so that several
 // workgroups share a CU and one task's memory phase overlaps another's
 // arithmetic.
-template <typename V, typename E, int FAM, int DIR, int PPT, int B>
+template <typename V, typename E, int FAM, int DIR, int PPT, int B, bool TL>
 __global__ __launch_bounds__(B, 4) void k_group_family(GroupedArgs<V> ga) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const TaskCtx<V> c{ga.off, ga.woff, ga.ws, ga.x0buf, reinterpret_cast<V*>(smem_raw), ga.tables, ga.normalize};
@@ -94,16 +94,17 @@ __global__ __launch_bounds__(B, 4) void k_group_family(GroupedArgs<V> ga) {
   for (int ti = ga.xbase[x] + blockIdx.x / kXcds; ti < ga.xbase[x + 1]; ti += step) {
     const GTask t = ga.tasks[ti];
     __syncthreads();  // the previous task's LDS reads are complete
-    run_family<V, E, FAM, DIR, PPT>(ga.lens[t.len], t, c, in, out);
+    run_family<V, E, FAM, DIR, PPT, TL>(ga.lens[t.len], t, c, in, out);
   }
 }
 
 // Launch (or, with prepare, set the dynamic-LDS limit of) one family kernel.
 // The points-per-thread bound: lo (16 FP32 / 8 FP64) or, for the LDS
-// families at 1024 threads, hi (twice that).
-template <typename V, typename E, int FAM, int DIR, int PPT, int B>
+// families at 1024 threads, hi (twice that).  TL: the four-step families
+// keep the sub-length's Stockham table in LDS after the tile.
+template <typename V, typename E, int FAM, int DIR, int PPT, int B, bool TL>
 static hipError_t group_family_op(bool prepare, const GroupedArgs<V>* a, unsigned grid, size_t lds, hipStream_t s) {
-  auto k = k_group_family<V, E, FAM, DIR, PPT, B>;
+  auto k = k_group_family<V, E, FAM, DIR, PPT, B, TL>;
   if (prepare)
     return lds > 65536 ? hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
                        : hipSuccess;
@@ -111,16 +112,25 @@ static hipError_t group_family_op(bool prepare, const GroupedArgs<V>* a, unsigne
   return hipGetLastError();
 }
 
-template <typename V, typename E, int FAM, int DIR, int PPT>
+template <typename V, typename E, int FAM, int DIR, int PPT, bool TL = false>
 static hipError_t group_family_block(bool prepare, int block, const GroupedArgs<V>* a, unsigned grid, size_t lds,
                                      hipStream_t s) {
   constexpr int LO = sizeof(V) == 8 ? 16 : 8;
   if constexpr (PPT == LO) {
-    if (block == 256) return group_family_op<V, E, FAM, DIR, PPT, 256>(prepare, a, grid, lds, s);
-    if (block == 512) return group_family_op<V, E, FAM, DIR, PPT, 512>(prepare, a, grid, lds, s);
+    if (block == 256) return group_family_op<V, E, FAM, DIR, PPT, 256, TL>(prepare, a, grid, lds, s);
+    if (block == 512) return group_family_op<V, E, FAM, DIR, PPT, 512, TL>(prepare, a, grid, lds, s);
   }
-  if (block == 1024) return group_family_op<V, E, FAM, DIR, PPT, 1024>(prepare, a, grid, lds, s);
+  if (block == 1024) return group_family_op<V, E, FAM, DIR, PPT, 1024, TL>(prepare, a, grid, lds, s);
   return hipErrorInvalidValue;
+}
+
+// Four-step families: `global_table` (the table does not fit beside the
+// tile) selects the form that reads the Stockham table from global memory.
+template <typename V, typename E, int FAM, int DIR, int PPT>
+static hipError_t group_fs_block(bool prepare, bool global_table, int block, const GroupedArgs<V>* a,
+                                 unsigned grid, size_t lds, hipStream_t s) {
+  return global_table ? group_family_block<V, E, FAM, DIR, PPT, false>(prepare, block, a, grid, lds, s)
+                      : group_family_block<V, E, FAM, DIR, PPT, true>(prepare, block, a, grid, lds, s);
 }
 
 template <typename V, typename E>
@@ -143,19 +153,19 @@ static hipError_t group_family_dispatch(bool prepare, int fam, int dir, bool hi,
                 : group_family_block<V, E, kGfBlue, -1, LO>(prepare, block, a, grid, lds, s);
     default: break;
   }
-  if (hi) return hipErrorInvalidValue;
+  // four-step families: hi selects the global-table form
   switch (fam) {
     case kGfColPlain:
-      return fwd ? group_family_block<V, E, kGfColPlain, -1, LO>(prepare, block, a, grid, lds, s)
-                 : group_family_block<V, E, kGfColPlain, +1, LO>(prepare, block, a, grid, lds, s);
+      return fwd ? group_fs_block<V, E, kGfColPlain, -1, LO>(prepare, hi, block, a, grid, lds, s)
+                 : group_fs_block<V, E, kGfColPlain, +1, LO>(prepare, hi, block, a, grid, lds, s);
     case kGfRowPlain:
-      return fwd ? group_family_block<V, E, kGfRowPlain, -1, LO>(prepare, block, a, grid, lds, s)
-                 : group_family_block<V, E, kGfRowPlain, +1, LO>(prepare, block, a, grid, lds, s);
-    case kGfColRaderA: return group_family_block<V, E, kGfColRaderA, -1, LO>(prepare, block, a, grid, lds, s);
-    case kGfColBlueA: return group_family_block<V, E, kGfColBlueA, -1, LO>(prepare, block, a, grid, lds, s);
-    case kGfRowConv: return group_family_block<V, E, kGfRowConv, -1, LO>(prepare, block, a, grid, lds, s);
-    case kGfColRaderC: return group_family_block<V, E, kGfColRaderC, +1, LO>(prepare, block, a, grid, lds, s);
-    case kGfColBlueC: return group_family_block<V, E, kGfColBlueC, +1, LO>(prepare, block, a, grid, lds, s);
+      return fwd ? group_fs_block<V, E, kGfRowPlain, -1, LO>(prepare, hi, block, a, grid, lds, s)
+                 : group_fs_block<V, E, kGfRowPlain, +1, LO>(prepare, hi, block, a, grid, lds, s);
+    case kGfColRaderA: return group_fs_block<V, E, kGfColRaderA, -1, LO>(prepare, hi, block, a, grid, lds, s);
+    case kGfColBlueA: return group_fs_block<V, E, kGfColBlueA, -1, LO>(prepare, hi, block, a, grid, lds, s);
+    case kGfRowConv: return group_fs_block<V, E, kGfRowConv, -1, LO>(prepare, hi, block, a, grid, lds, s);
+    case kGfColRaderC: return group_fs_block<V, E, kGfColRaderC, +1, LO>(prepare, hi, block, a, grid, lds, s);
+    case kGfColBlueC: return group_fs_block<V, E, kGfColBlueC, +1, LO>(prepare, hi, block, a, grid, lds, s);
   }
   return hipErrorInvalidValue;
 }

@@ -327,8 +327,10 @@ inline int family_phase(int fam) { return fam <= kGfColBlueA ? 0 : (fam <= kGfRo
 constexpr int kGroupClasses = 3;
 constexpr int kGroupClassThreads[kGroupClasses] = {256, 512, 1024};
 // One family kernel of a grouped plan: storage kF32 / kF64 (data in the
-// compute type) or kBF16 / kF16 (FP32 arithmetic, 16-bit data); hi: the
-// doubled points-per-thread bound (LDS families at 1024 threads only);
+// compute type) or kBF16 / kF16 (FP32 arithmetic, 16-bit data); hi: for
+// the LDS families the doubled points-per-thread bound (1024 threads only),
+// for the four-step families the form that reads the sub-length's Stockham
+// table from global memory (the others copy it into LDS after the tile);
 // `args` points at GroupedArgs<float2> or GroupedArgs<double2>.  With
 // prepare, only sets the kernel's dynamic-LDS limit (plan time, so the
 // launch path holds nothing but launches and can be captured in a graph).

@@ -65,14 +65,24 @@ int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads, bool al
   return 0;
 }
 
+// The runtime-pass sub-transform: its Stockham table copied into LDS after
+// the tile (tw_bytes > 0) or read from global memory.
+template <typename V, int PPT>
+static hipError_t dispatch_generic(bool col, int mode, int dir, const FsArgs<V>& a, size_t batch, size_t lds,
+                                   size_t tw_bytes, int threads, hipStream_t s) {
+  if (tw_bytes)
+    return dispatch_fs<V, GenericFft<V, PPT, true>>(col, mode, dir, a, batch, lds + tw_bytes, threads, s);
+  return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);
+}
+
 template <typename V, int PPT>
 static hipError_t dispatch_len(int fixed_n, bool col, int mode, int dir, const FsArgs<V>& a,
-                               size_t batch, size_t lds, int threads, hipStream_t s) {
+                               size_t batch, size_t lds, size_t tw_bytes, int threads, hipStream_t s) {
   constexpr int P = sizeof(V) == 8 ? 16 : 8;  // points per thread of the fixed forms
   if (fixed_n & 1) {  // odd-radix forms: plain modes only
     if (mode == (col ? kFsPlainA : kFsPlainB))
       return launch_fs_odd(sizeof(V) == 16 ? kF64 : kF32, fixed_n, col, dir, &a, batch, lds, threads, s);
-    return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);
+    return dispatch_generic<V, PPT>(col, mode, dir, a, batch, lds, tw_bytes, threads, s);
   }
   switch (fixed_n) {
     case 64: return dispatch_fs<V, FixedFft<V, 64, 64 / P, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
@@ -88,7 +98,7 @@ static hipError_t dispatch_len(int fixed_n, bool col, int mode, int dir, const F
         return dispatch_fs<V, FixedFft<V, 16384, 16384 / P, 16, 16, 8, 8>>(col, mode, dir, a, batch, lds, threads, s);
       break;
   }
-  return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);
+  return dispatch_generic<V, PPT>(col, mode, dir, a, batch, lds, tw_bytes, threads, s);
 }
 
 hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* args, size_t batch,
@@ -97,12 +107,14 @@ hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* arg
   const int threads = g.threads * g.per_block;
   if (prec == kF64) {
     const auto& a = *static_cast<const FsArgs<double2>*>(args);
-    return g.ppt <= 8 ? dispatch_len<double2, 8>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s)
-                      : dispatch_len<double2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s);
+    return g.ppt <= 8
+               ? dispatch_len<double2, 8>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s)
+               : dispatch_len<double2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s);
   }
   const auto& a = *static_cast<const FsArgs<float2>*>(args);
-  return g.ppt <= 16 ? dispatch_len<float2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s)
-                     : dispatch_len<float2, 32>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, threads, s);
+  return g.ppt <= 16
+             ? dispatch_len<float2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s)
+             : dispatch_len<float2, 32>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s);
 }
 
 }  // namespace fyfft

@@ -39,6 +39,9 @@ struct LdsGeom {
   int pad_shift = 4; // LDS index i -> i + (i >> pad_shift)
   size_t lds_bytes = 0;
   bool small_radix = false;  // FP32, no radix-16 pass: the leaner 8-points-per-thread kernel
+  // four-step tiles: bytes of the LDS copy of the sub-length's Stockham
+  // table after the tile (runtime-pass sub-transforms), 0 when it does not fit
+  size_t tw_bytes = 0;
 };
 // Fits n (elements of `elem_bytes`) in one workgroup's LDS of `lds_cap`
 // bytes; returns false if it does not fit even unpadded.  short_threads
