@@ -296,9 +296,12 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int for
 // of Rader/Bluestein plans reads and writes its rows in place, so those take
 // the LDS-aware split (small column tiles, long rows): N = 1000003 at
 // M = 2^21 = 256 x 8192 ran 0.53 ms against 0.58 ms for 1024 x 2048, while
-// plain N = 2^21 at 256 x 8192 ran 3x slower than at 1024 x 2048.
+// plain N = 2^21 at 256 x 8192 ran 3x slower than at 1024 x 2048.  Plain
+// lengths that are not powers of two take the cost-model split (their
+// runtime-pass sub-transforms vary widely in speed with the factors).
 static bool fs_split(size_t n, size_t elem, bool conv, size_t* n1, size_t* n2) {
-  return (conv && four_step_split_lds(n, elem, n1, n2)) || four_step_split(n, kFactorCap, n1, n2);
+  return (conv && four_step_split_lds(n, elem, n1, n2)) || (!conv && four_step_split_model(n, kFactorCap, elem, n1, n2)) ||
+         four_step_split(n, kFactorCap, n1, n2);
 }
 
 static bool fs_possible(size_t n, size_t elem, size_t cap) {
@@ -1475,8 +1478,13 @@ static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t c
     for (int k = 0; k < kGroupClasses; ++k) {
       if (want == 16 && k == kGroupClasses - 1) continue;
       if (fits(want, k)) {
+        // more columns while the class's threads and LDS budget allow: the
+        // persistent workgroups of a class otherwise leave threads idle
+        // (N1 = 125: 16 columns of 8 threads use half of a 256-thread class)
+        int c = want;
+        while (c < 64 && fits(2 * c, k)) c *= 2;
         *cls = k;
-        *log2c = ilog2_exact(want);
+        *log2c = ilog2_exact(c);
         return;
       }
     }

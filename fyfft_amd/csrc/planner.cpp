@@ -1,6 +1,8 @@
 // Host planner (see planner.hpp).
 #include "planner.hpp"
 
+#include <cmath>
+
 #include <algorithm>
 
 namespace fyfft {
@@ -148,6 +150,29 @@ bool four_step_split_lds(size_t n, size_t elem_bytes, size_t* n1, size_t* n2) {
   *n1 = best;
   *n2 = n / best;
   return true;
+}
+
+bool four_step_split_model(size_t n, size_t cap_elems, size_t elem_bytes, size_t* n1, size_t* n2) {
+  if (!is_smooth7(n) || (n & (n - 1)) == 0) return false;
+  auto pass_cost = [](size_t m) {
+    double c = 0;
+    for (int r : radices_for(m)) c += r == 3 ? 1.3 : r == 5 ? 1.15 : r == 7 ? 1.2 : r == 9 ? 1.1 : 1.0;
+    return c;
+  };
+  double best = 1e300;
+  for (size_t d = 2; d * d <= 64 * n && d <= cap_elems; ++d) {
+    if (n % d) continue;
+    const size_t e = n / d;
+    if (e > cap_elems || d * 8 < e || e * 8 < d || !is_smooth7(d)) continue;
+    const double c = pass_cost(d) + pass_cost(e) + ((d * elem_bytes) % 128 ? 0.5 : 0.0) +
+                     ((e * elem_bytes) % 128 ? 0.5 : 0.0) + 0.05 * std::fabs(std::log((double)d / (double)e));
+    if (c < best) {
+      best = c;
+      *n1 = d;
+      *n2 = e;
+    }
+  }
+  return best < 1e300;
 }
 
 bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2) {

@@ -62,6 +62,16 @@ size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap);  // unpadded
 // Four-step split n = n1 * n2 with both factors smooth and LDS-resident
 // (n1 <= n2, as balanced as possible).  False if none exists.
 bool four_step_split(size_t n, size_t cap_elems, size_t* n1, size_t* n2);
+// Split of a smooth length that is not a power of two by a cost model of the
+// runtime-pass sub-transforms: the passes of both factors (radices_for),
+// weighted by how much of an LDS round trip each radix uses (3, 5, 7 and 9
+// heavier), plus a penalty for each factor whose elements do not fill whole
+// 128-byte lines (the column pass's workspace rows and the row pass's
+// transposed output then write partial lines), and a small imbalance term;
+// factors within a ratio of 8 of each other.  Measured against every split
+// of eight such lengths (24000...120000) it picked the fastest or one
+// within 10 % (N = 96000: 300 x 320 0.95 ms -> 240 x 400 0.60 ms).
+bool four_step_split_model(size_t n, size_t cap_elems, size_t elem_bytes, size_t* n1, size_t* n2);
 // The split batched four-step plans prefer: the largest n1 <= sqrt(n) whose
 // 16-column tiles fit 40 KB of LDS, with n2 <= 16384 (FP32) / 8192 (FP64).
 bool four_step_split_lds(size_t n, size_t elem_bytes, size_t* n1, size_t* n2);
