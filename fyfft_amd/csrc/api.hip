@@ -230,9 +230,10 @@ static int ilog2_exact(int v) {
 }
 
 // Pass list + FP64-generated Stockham twiddle table for a smooth length.
-static fyfft_status make_pass_table(fyfft_plan* p, size_t n, dev::PassDesc* pd, void** tw) {
-  const std::vector<int> rad = radices_for(n);
-  if (rad.empty() || (int)rad.size() > dev::kMaxPasses) return FYFFT_UNSUPPORTED;
+static fyfft_status make_pass_table(fyfft_plan* p, size_t n, dev::PassDesc* pd, void** tw,
+                                    const std::vector<int>* radices = nullptr) {
+  const std::vector<int> rad = radices ? *radices : radices_for(n);
+  if (rad.empty() || (int)rad.size() > dev::kMaxPackedPasses) return FYFFT_UNSUPPORTED;
   pd->npass = (int)rad.size();
   for (size_t i = 0; i < rad.size(); ++i) pd->radix[i] = rad[i];
   fyfft_status st = p->alloc(std::max<size_t>(n, 2) * elem_bytes(p->prec), tw);
@@ -325,11 +326,21 @@ static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
   if (!fs_geometry(p->n1, elem, cap, &p->gcol, f ? f->ccol : 0) ||
       !fs_geometry(p->n2, elem, cap, &p->grow, f ? f->crow : 0))
     return FYFFT_UNSUPPORTED;
-  fyfft_status st = make_pass_table(p, p->n1, &p->pdcol, &p->twcol);
-  if (!st) st = make_pass_table(p, p->n2, &p->pdrow, &p->twrow);
+  // a compile-time sub-transform when one matches the planner's pass list;
+  // otherwise the runtime-pass sub-transforms of plain FP32 transforms take
+  // composite radices (convolutions keep the base radices)
+  auto passes = [&](size_t len, const LdsGeom& g, int* fixed, dev::PassDesc* pd, void** tw) {
+    std::vector<int> rad = radices_for(len);
+    dev::PassDesc probe{};
+    probe.npass = (int)std::min<size_t>(rad.size(), (size_t)dev::kMaxPasses);
+    for (int i = 0; i < probe.npass; ++i) probe.radix[i] = rad[i];
+    *fixed = find_fs_fixed(p->prec, probe, (int)len, g.threads, conv);
+    if (!*fixed && !conv && p->prec == kF32 && g.ppt == 16) rad = radices_composite(len);
+    return make_pass_table(p, len, pd, tw, &rad);
+  };
+  fyfft_status st = passes(p->n1, p->gcol, &p->fixed_col, &p->pdcol, &p->twcol);
+  if (!st) st = passes(p->n2, p->grow, &p->fixed_row, &p->pdrow, &p->twrow);
   if (st) return st;
-  p->fixed_col = find_fs_fixed(p->prec, p->pdcol, (int)p->n1, p->gcol.threads, conv);
-  p->fixed_row = find_fs_fixed(p->prec, p->pdrow, (int)p->n2, p->grow.threads, conv);
   int lg = 0;
   while (((size_t)1 << lg) < n) ++lg;
   p->lo_bits = (lg + 1) / 2;
@@ -436,7 +447,7 @@ static fyfft_status make_candidate(fyfft_plan* p, size_t n, const LdsChoice& c, 
                                    void** tw_alloc) {
   const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(p->device);
   *tw_alloc = nullptr;
-  if (c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses) return FYFFT_UNSUPPORTED;
+  if (c.rad.empty() || (int)c.rad.size() > dev::kMaxPackedPasses) return FYFFT_UNSUPPORTED;
   L->pd.npass = (int)c.rad.size();
   for (size_t i = 0; i < c.rad.size(); ++i) L->pd.radix[i] = c.rad[i];
   LdsGeom g;
@@ -868,7 +879,7 @@ static fyfft_status wisdom_parse(const char* text) {
                         (n >= 1 && n <= (1ULL << 20) &&
                          lds_geometry((size_t)n, prec == kF64 ? 16 : 8, (size_t)1 << 40, &geo, true) && geo.ppt == ppt);
     if (prod != n || (prec != kF32 && prec != kF64) || !ppt_ok || (ps != 30 && (ps < 2 || ps > 8)) ||
-        tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPasses)
+        tpb < 1 || tpb > 1024 || c.rad.empty() || (int)c.rad.size() > dev::kMaxPackedPasses)
       return wisdom_bad("invalid LDS entry: " + line);
     lds[std::make_tuple(dev, (size_t)n, prec, gen ? 1 : 0)] = c;
   }

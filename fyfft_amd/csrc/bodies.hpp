@@ -43,21 +43,26 @@ __device__ __forceinline__ void staged(int start, int step, int total, LoadF ld,
 // runtime passes' twiddle loads from global memory sat on the critical path
 // of every pass (N = 96000 = 300 x 320 column pass 643 us with them, 444 us
 // with none at all).
-template <typename V, int PPT, bool TL = false> struct GenericFft {
+// CMP: the pass list may hold the composite radices (FP32 four-step
+// sub-transforms at 16 points per thread, see radices_composite).
+template <typename V, int PPT, bool TL = false, bool CMP = false> struct GenericFft {
   static constexpr int kPpt = PPT;
   static constexpr bool kTwLds = TL;
   template <int D>
   static __device__ __forceinline__ void run(V* lds, int n, int t, int tid, const PassDesc& pd,
                                              const V* tw, bool active, int ps) {
-    lds_fft<V, PPT, D, TL>(lds, n, t, tid, pd, tw, active, ps);
+    lds_fft<V, PPT, D, TL, CMP>(lds, n, t, tid, pd, tw, active, ps);
   }
   // the same transform with the first pass reading src and the last writing dst
   template <int D, typename Src, typename Dst>
   static __device__ __forceinline__ void run_io(Src src, Dst dst, V* lds, int n, int t, int tid,
                                                 const PassDesc& pd, const V* tw, bool active, int ps) {
-    lds_fft_io<V, PPT, D, TL>(src, dst, lds, n, t, tid, pd, tw, active, ps);
+    lds_fft_io<V, PPT, D, TL, CMP>(src, dst, lds, n, t, tid, pd, tw, active, ps);
   }
 };
+// Four-step runtime-pass sub-transforms of plain transforms: composite
+// radices for FP32 at 16 points per thread.
+template <typename V, int PPT> constexpr bool kFsComposite = sizeof(V) == 8 && PPT == 16;
 
 // Copy of the n-entry (n - 1 used) Stockham table into LDS at `dst` when the
 // policy reads it there; the caller's next barrier publishes it (the first

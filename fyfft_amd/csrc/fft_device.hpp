@@ -286,6 +286,88 @@ template <int DIR, typename V> struct Dft<9, DIR, V> {
 template <int DIR, typename V> struct Dft<5, DIR, V> : DftOdd<5, DIR, V> {};
 template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
 
+// cos and sin of 2*pi*m/R for integers m, R (R > 0), evaluated at compile
+// time: exact reduction of 4m mod 4R to a quadrant and a remainder r < R,
+// then the Taylor series of sin and cos on theta = (pi/2) * r / R <= pi/2
+// (terms to degree 29: truncation far below double rounding).
+struct UnitRoot { double c, s; };
+__host__ __device__ constexpr UnitRoot unit_root(int m, int R) {
+  const long long m4 = ((4LL * m) % (4LL * R) + 4LL * R) % (4LL * R);
+  const long long q = m4 / R, r = m4 - q * R;
+  const double th = 1.57079632679489661923132169163975144 * ((double)r / (double)R);
+  double sn = th, cs = 1.0, ts = th, tc = 1.0;
+  for (int k = 1; k <= 14; ++k) {
+    ts *= -th * th / (double)((2 * k) * (2 * k + 1));
+    tc *= -th * th / (double)((2 * k - 1) * (2 * k));
+    sn += ts;
+    cs += tc;
+  }
+  if (r == 0) sn = 0.0, cs = 1.0;
+  switch (q) {
+    case 0: return UnitRoot{cs, sn};
+    case 1: return UnitRoot{-sn, cs};
+    case 2: return UnitRoot{-cs, -sn};
+    default: return UnitRoot{sn, -cs};
+  }
+}
+
+// a * w_R^m (w_R = exp(DIR * 2*pi*i / R)) for an m known after unrolling.
+template <int R, int DIR, typename V> __device__ __forceinline__ V wconst(V a, int m) {
+  m %= R;
+  if (m == 0) return a;
+  if (2 * m == R) return V{-a.x, -a.y};
+  if (4 * m == R) return rot90<DIR>(a);
+  if (4 * m == 3 * R) return rot90<-DIR>(a);
+  const UnitRoot u = unit_root(m, R);
+  return rotc<DIR>(a, u.c, u.s);
+}
+
+// Composite radix R = A * B in registers (two levels, like Dft<16> and
+// Dft<9>): x[B*n1 + n2]; DFT_A over n1, twiddle w_R^(n2*k1), DFT_B over n2,
+// output X[k1 + A*k2].  One pass of such a radix replaces two LDS passes.
+template <int A, int B, int DIR, typename V> struct DftAB {
+  static __device__ __forceinline__ void run(V* v) {
+    constexpr int R = A * B;
+    V y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+      V t[A];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) t[n1] = v[B * n1 + n2];
+      Dft<A, DIR, V>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) y[n2 * A + k1] = wconst<R, DIR>(t[k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      V t[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) t[n2] = y[n2 * A + k1];
+      Dft<B, DIR, V>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
+    }
+  }
+};
+// The composite radices of the runtime passes and their two factors.
+template <int R> struct Composite { static constexpr int A = 0, B = 0; };
+template <> struct Composite<6> { static constexpr int A = 2, B = 3; };
+template <> struct Composite<10> { static constexpr int A = 2, B = 5; };
+template <> struct Composite<12> { static constexpr int A = 4, B = 3; };
+template <> struct Composite<14> { static constexpr int A = 2, B = 7; };
+template <> struct Composite<15> { static constexpr int A = 3, B = 5; };
+template <> struct Composite<20> { static constexpr int A = 4, B = 5; };
+template <> struct Composite<21> { static constexpr int A = 3, B = 7; };
+template <> struct Composite<25> { static constexpr int A = 5, B = 5; };
+template <int DIR, typename V> struct Dft<6, DIR, V> : DftAB<2, 3, DIR, V> {};
+template <int DIR, typename V> struct Dft<10, DIR, V> : DftAB<2, 5, DIR, V> {};
+template <int DIR, typename V> struct Dft<12, DIR, V> : DftAB<4, 3, DIR, V> {};
+template <int DIR, typename V> struct Dft<14, DIR, V> : DftAB<2, 7, DIR, V> {};
+template <int DIR, typename V> struct Dft<15, DIR, V> : DftAB<3, 5, DIR, V> {};
+template <int DIR, typename V> struct Dft<20, DIR, V> : DftAB<4, 5, DIR, V> {};
+template <int DIR, typename V> struct Dft<21, DIR, V> : DftAB<3, 7, DIR, V> {};
+template <int DIR, typename V> struct Dft<25, DIR, V> : DftAB<5, 5, DIR, V> {};
+
 // ---------------------------------------------------------------- twiddles
 // Multiply inputs r = 1..R-1 of one butterfly by w^(k*r), where t points at
 // the pass table slice [w^k, w^2k, ..., w^((R-1)k)].  For R = 7, 8, 9 and 16
@@ -302,7 +384,24 @@ template <int DIR, typename V> struct Dft<7, DIR, V> : DftOdd<7, DIR, V> {};
 // loading the factors of passes 1 and 2 before pass 0.
 template <int R, int DIR, typename V>
 __device__ __forceinline__ void apply_twiddles(V* v, const V* __restrict__ t) {
-  if constexpr (R <= 5) {
+  if constexpr (Composite<R>::A > 0) {
+    // R = A * B: w^(a + A*b) = w^a * w^(A*b), both loaded (depth-one products)
+    constexpr int A = Composite<R>::A, B = Composite<R>::B;
+    V lo[A], hi[B];
+    lo[0] = hi[0] = V{1, 0};
+#pragma unroll
+    for (int a = 1; a < A; ++a) lo[a] = t[a - 1];
+#pragma unroll
+    for (int b = 1; b < B; ++b) hi[b] = t[A * b - 1];
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+      for (int a = 0; a < A; ++a) {
+        const int r = a + A * b;
+        if (r == 0) continue;
+        v[r] = twmul<DIR>(v[r], a == 0 ? hi[b] : (b == 0 ? lo[a] : cmul(lo[a], hi[b])));
+      }
+  } else if constexpr (R <= 5) {
 #pragma unroll
     for (int r = 1; r < R; ++r) v[r] = twmul<DIR>(v[r], t[r - 1]);
   } else if constexpr (R == 16) {
@@ -392,17 +491,18 @@ struct PassDesc {
   int radix[kMaxPasses];
 };
 
-// The pass list packed 4 bits per radix (radices are at most 16, stored as
-// R - 1) so that the runtime pass loops read the radix of pass p with a
-// shift instead of indexing the array, which would keep the descriptor in
-// scratch memory.
+// The pass list packed 5 bits per radix (radices are at most 25, stored as
+// R - 1; at most kMaxPackedPasses passes) so that the runtime pass loops
+// read the radix of pass p with a shift instead of indexing the array, which
+// would keep the descriptor in scratch memory.
+constexpr int kMaxPackedPasses = 12;
 __device__ __forceinline__ unsigned long long pack_radices(const PassDesc& pd) {
   unsigned long long code = 0;
 #pragma unroll
-  for (int i = 0; i < kMaxPasses; ++i) code |= (unsigned long long)((pd.radix[i] - 1) & 15) << (4 * i);
+  for (int i = 0; i < kMaxPackedPasses; ++i) code |= (unsigned long long)((pd.radix[i] - 1) & 31) << (5 * i);
   return code;
 }
-__device__ __forceinline__ int radix_at(unsigned long long code, int p) { return (int)((code >> (4 * p)) & 15) + 1; }
+__device__ __forceinline__ int radix_at(unsigned long long code, int p) { return (int)((code >> (5 * p)) & 31) + 1; }
 
 __host__ __device__ constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 // FP32 runtime-pass kernels bounded at 8 points per thread serve pass lists
@@ -477,6 +577,41 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
   if (!DST_G) __syncthreads();
 }
 
+template <int R> struct RadixC { static constexpr int value = R; };
+// NO16: the pass list has no radix 16 (the caller guarantees it), so that
+// case is not instantiated and the registers are sized for radix 9 at most.
+// CMP: the composite radices 6, 10, 12, 14, 15, 20, 21, 25 are cases too
+// (FP32 four-step sub-transforms at 16 points per thread; the planner emits
+// them only for those).
+template <bool NO16 = false, bool CMP = false, typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
+  if constexpr (CMP) {
+    switch (R) {
+      case 6: f(RadixC<6>{}); return;
+      case 10: f(RadixC<10>{}); return;
+      case 12: f(RadixC<12>{}); return;
+      case 14: f(RadixC<14>{}); return;
+      case 15: f(RadixC<15>{}); return;
+      case 20: f(RadixC<20>{}); return;
+      case 21: f(RadixC<21>{}); return;
+      case 25: f(RadixC<25>{}); return;
+      default: break;
+    }
+  }
+  switch (R) {
+    case 2: f(RadixC<2>{}); break;
+    case 3: f(RadixC<3>{}); break;
+    case 4: f(RadixC<4>{}); break;
+    case 5: f(RadixC<5>{}); break;
+    case 7: f(RadixC<7>{}); break;
+    case 8: f(RadixC<8>{}); break;
+    case 9: f(RadixC<9>{}); break;
+    default:
+      if constexpr (NO16) f(RadixC<8>{});
+      else f(RadixC<16>{});
+      break;
+  }
+}
+
 // One LDS -> LDS pass kept out of line: inlining every radix case into the
 // pass loop of lds_fft makes the register allocator keep all cases' arrays
 // live at once and spill; as separate functions each gets its own budget.
@@ -490,7 +625,7 @@ __device__ __noinline__ void lds_pass_ool(lds_ptr<V> lds_s, int N, int Ls, int T
 // Run every pass of `pd` on an LDS-resident transform (LDS -> LDS).
 // PPT bounds the points per thread (N <= PPT * T).  TL: tw_gen points at an
 // LDS copy of the table.
-template <typename V, int PPT, int DIR, bool TL = false>
+template <typename V, int PPT, int DIR, bool TL = false, bool CMP = false>
 __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const PassDesc& pd,
                                         const V* __restrict__ tw_gen, bool active, int ps) {
   const lds_ptr<V> lds = (lds_ptr<V>)lds_gen;
@@ -499,19 +634,10 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
   int Ls = 1;
   for (int p = 0; p < pd.npass; ++p) {
     const int R = radix_at(code, p);
-    switch (R) {
-      case 2: lds_pass_ool<V, 2, cdiv(PPT, 2), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 3: lds_pass_ool<V, 3, cdiv(PPT, 3), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 4: lds_pass_ool<V, 4, cdiv(PPT, 4), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 5: lds_pass_ool<V, 5, cdiv(PPT, 5), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 7: lds_pass_ool<V, 7, cdiv(PPT, 7), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 8: lds_pass_ool<V, 8, cdiv(PPT, 8), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      case 9: lds_pass_ool<V, 9, cdiv(PPT, 9), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps); break;
-      default:
-        if constexpr (kNo16<V, PPT>) lds_pass_ool<V, 8, cdiv(PPT, 8), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps);
-        else lds_pass_ool<V, 16, cdiv(PPT, 16), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps);
-        break;
-    }
+    radix_switch<kNo16<V, PPT>, CMP>(R, [&](auto rc) {
+      constexpr int RR = decltype(rc)::value;
+      lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps);
+    });
     Ls *= R;
   }
 }
@@ -612,29 +738,10 @@ __device__ __forceinline__ void fixed_io_passes(Src src, Dst dst, V* lds, int ps
   if constexpr (!last) fixed_io_passes<V, DIR, N, T, Ls * R, Src, Dst, Rest...>(src, dst, lds, ps, tid, tw, active);
 }
 
-template <int R> struct RadixC { static constexpr int value = R; };
-// NO16: the pass list has no radix 16 (the caller guarantees it), so that
-// case is not instantiated and the registers are sized for radix 9 at most.
-template <bool NO16 = false, typename F> __device__ __forceinline__ void radix_switch(int R, F f) {
-  switch (R) {
-    case 2: f(RadixC<2>{}); break;
-    case 3: f(RadixC<3>{}); break;
-    case 4: f(RadixC<4>{}); break;
-    case 5: f(RadixC<5>{}); break;
-    case 7: f(RadixC<7>{}); break;
-    case 8: f(RadixC<8>{}); break;
-    case 9: f(RadixC<9>{}); break;
-    default:
-      if constexpr (NO16) f(RadixC<8>{});
-      else f(RadixC<16>{});
-      break;
-  }
-}
-
 // Every pass of `pd` with the first reading `src` and the last writing
 // `dst`; `lds` (padded by ps) holds the transform between passes.  Every
 // thread of the workgroup must call it (barriers).
-template <typename V, int PPT, int DIR, bool TL = false, typename Src, typename Dst>
+template <typename V, int PPT, int DIR, bool TL = false, bool CMP = false, typename Src, typename Dst>
 __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, int T, int tid,
                                            const PassDesc& pd, const V* __restrict__ tw_gen,
                                            bool active, int ps) {
@@ -645,7 +752,7 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   const int P = pd.npass;
   if (P <= 1) {
     if (P == 1)
-      radix_switch<kNo16<V, PPT>>(radix_at(pack_radices(pd), 0), [&](auto rc) {
+      radix_switch<kNo16<V, PPT>, CMP>(radix_at(pack_radices(pd), 0), [&](auto rc) {
         constexpr int R = decltype(rc)::value;
         io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(src, lk, N, 1, T, tid, tw, active);
       });
@@ -660,7 +767,7 @@ __device__ __forceinline__ void lds_fft_io(Src src, Dst dst, V* lds_gen, int N, 
   const unsigned long long code = pack_radices(pd);
   int Ls = 1;
   for (int p = 0; p < P; ++p) {
-    radix_switch<kNo16<V, PPT>>(radix_at(code, p), [&](auto rc) {
+    radix_switch<kNo16<V, PPT>, CMP>(radix_at(code, p), [&](auto rc) {
       constexpr int R = decltype(rc)::value;
       if (p == 0) io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(src, lk, N, Ls, T, tid, tw, active);
       else if (p == P - 1) io_pass_ool<V, R, cdiv(PPT, R), DIR, TL>(ls, dst, N, Ls, T, tid, tw, active);

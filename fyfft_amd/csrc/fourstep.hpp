@@ -48,6 +48,17 @@ static hipError_t go_row(const FsArgs<V>& a, size_t batch, size_t lds, int threa
   return hipGetLastError();
 }
 
+// The plain (non-convolution) modes only.
+template <typename V, typename F>
+static hipError_t dispatch_fs_plain(bool col, int dir, const FsArgs<V>& a, size_t batch, size_t lds, int threads,
+                                    hipStream_t s) {
+  if (col)
+    return dir < 0 ? go_col<V, F, -1, kFsPlainA>(a, batch, lds, threads, s)
+                   : go_col<V, F, +1, kFsPlainA>(a, batch, lds, threads, s);
+  return dir < 0 ? go_row<V, F, -1, kFsPlainB>(a, batch, lds, threads, s)
+                 : go_row<V, F, +1, kFsPlainB>(a, batch, lds, threads, s);
+}
+
 template <typename V, typename F>
 static hipError_t dispatch_fs(bool col, int mode, int dir, const FsArgs<V>& a, size_t batch,
                               size_t lds, int threads, hipStream_t s) {

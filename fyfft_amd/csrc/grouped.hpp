@@ -74,9 +74,9 @@ __device__ __forceinline__ void run_family(const GLen<V>& gl, const GTask& t, co
     const long long o = c.off[t.first], w = c.woff[t.first];
     V* x0 = c.x0buf + t.first;
     if constexpr (col)
-      body_fs_col<V, GenericFft<V, PPT, TL>, DIR, MODE>(a, in + o, c.ws + w, out + o, x0, t.count, c.lds);
+      body_fs_col<V, GenericFft<V, PPT, TL, kFsComposite<V, PPT> && MODE == kFsPlainA>, DIR, MODE>(a, in + o, c.ws + w, out + o, x0, t.count, c.lds);
     else
-      body_fs_row<V, GenericFft<V, PPT, TL>, DIR, MODE>(a, c.ws + w, out + o,
+      body_fs_row<V, GenericFft<V, PPT, TL, kFsComposite<V, PPT> && MODE == kFsPlainB>, DIR, MODE>(a, c.ws + w, out + o,
                                                     gl.kind == kGkFsRader ? x0 : nullptr, t.count, c.lds);
   }
 }

@@ -70,6 +70,15 @@ int find_fs_fixed(int prec, const dev::PassDesc& pd, int n, int threads, bool al
 template <typename V, int PPT>
 static hipError_t dispatch_generic(bool col, int mode, int dir, const FsArgs<V>& a, size_t batch, size_t lds,
                                    size_t tw_bytes, int threads, hipStream_t s) {
+  // composite radices: the plain modes of FP32 plans (their pass lists may
+  // hold them); the convolution modes keep the base radices, whose bodies
+  // measured slower with the composite passes' register budget
+  constexpr bool CMP = kFsComposite<V, PPT>;
+  const bool plain = mode == (col ? kFsPlainA : kFsPlainB);
+  if (CMP && plain)
+    return tw_bytes ? dispatch_fs_plain<V, GenericFft<V, PPT, true, CMP>>(col, dir, a, batch, lds + tw_bytes,
+                                                                           threads, s)
+                    : dispatch_fs_plain<V, GenericFft<V, PPT, false, CMP>>(col, dir, a, batch, lds, threads, s);
   if (tw_bytes)
     return dispatch_fs<V, GenericFft<V, PPT, true>>(col, mode, dir, a, batch, lds + tw_bytes, threads, s);
   return dispatch_fs<V, GenericFft<V, PPT>>(col, mode, dir, a, batch, lds, threads, s);

@@ -91,6 +91,60 @@ std::vector<int> radices_for(size_t n) {
   return out;
 }
 
+std::vector<int> radices_composite(size_t n) {
+  std::vector<int> out;
+  if (n < 2 || !is_smooth7(n)) return out;
+  int e[4] = {0, 0, 0, 0};
+  const int pr[4] = {2, 3, 5, 7};
+  for (int i = 0; i < 4; ++i)
+    while (n % pr[i] == 0) { n /= pr[i]; ++e[i]; }
+  static const int kRad[] = {25, 21, 20, 16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+  struct Ex { int r, x[4]; };
+  std::vector<Ex> rad;
+  for (int r : kRad) {
+    Ex q{r, {0, 0, 0, 0}};
+    int m = r;
+    for (int i = 0; i < 4; ++i)
+      while (m % pr[i] == 0) { m /= pr[i]; ++q.x[i]; }
+    rad.push_back(q);
+  }
+  auto pass_cost = [](int r) {
+    const int maxb = (16 + r - 1) / r;  // butterflies per thread at 16 points per thread
+    const double used = (16.0 / r) / maxb;
+    return 1.0 + 0.3 * (1.0 / used - 1.0);
+  };
+  // best[a][b][c][d]: cost and first radix for the exponent vector
+  const int A = e[0] + 1, B = e[1] + 1, C = e[2] + 1, D = e[3] + 1;
+  auto idx = [&](int a, int b, int c, int d) { return ((a * B + b) * C + c) * D + d; };
+  std::vector<double> cost((size_t)A * B * C * D, 1e300);
+  std::vector<int> first((size_t)A * B * C * D, 0);
+  cost[0] = 0;
+  for (int a = 0; a < A; ++a)
+    for (int b = 0; b < B; ++b)
+      for (int c = 0; c < C; ++c)
+        for (int d = 0; d < D; ++d) {
+          if (a + b + c + d == 0) continue;
+          const size_t i = idx(a, b, c, d);
+          for (const Ex& q : rad) {
+            if (q.x[0] > a || q.x[1] > b || q.x[2] > c || q.x[3] > d) continue;
+            const double v = cost[idx(a - q.x[0], b - q.x[1], c - q.x[2], d - q.x[3])] + pass_cost(q.r);
+            if (v < cost[i] - 1e-12) {
+              cost[i] = v;
+              first[i] = q.r;
+            }
+          }
+        }
+  int a = e[0], b = e[1], c = e[2], d = e[3];
+  while (a + b + c + d > 0) {
+    const int r = first[idx(a, b, c, d)];
+    out.push_back(r);
+    for (const Ex& q : rad)
+      if (q.r == r) a -= q.x[0], b -= q.x[1], c -= q.x[2], d -= q.x[3];
+  }
+  std::sort(out.begin(), out.end(), [](int x, int y) { return x > y; });
+  return out;
+}
+
 size_t lds_capacity_elems(size_t elem_bytes, size_t lds_cap) { return lds_cap / elem_bytes; }
 
 bool lds_geometry(size_t n, size_t elem_bytes, size_t lds_cap, LdsGeom* g, bool short_threads, bool small_radix) {

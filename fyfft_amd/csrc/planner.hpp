@@ -25,6 +25,15 @@ int strategy_for(size_t n);
 // 7-smooth or n < 2.
 std::vector<int> radices_for(size_t n);
 
+// Pass list of a runtime-pass four-step sub-transform (FP32, 16 points per
+// thread): like radices_for, but a pass may also have one of the composite
+// radices 6, 10, 12, 14, 15, 20, 21, 25 (two levels in registers).  Chosen
+// by dynamic programming over the exponents of 2, 3, 5 and 7 to minimise a
+// cost of one unit per pass (an LDS round trip and barrier) plus 0.3 per
+// unit of idle lanes in that pass at 16 points per thread.  Empty if n is
+// not 7-smooth or n < 2.
+std::vector<int> radices_composite(size_t n);
+
 // LDS padding of the planner's geometries: index i is stored at
 // i + (i >> kLdsPadShift) (the device code's dev::kPadShift, checked equal
 // where both are visible).
