@@ -4,6 +4,7 @@ Bluestein paths against the FP64 oracle, through the C ABI.
 Tolerances (relative L2 vs the oracle): FP32 1e-7*log2(N) + 1e-7,
 FP64 4e-16*log2(N) + 1e-15 (the convolution paths chain three FFTs)."""
 import math
+import re
 
 import numpy as np
 import pytest
@@ -63,6 +64,33 @@ def test_fourstep_f32(fy, orc, cuda, n, batch):
                                      (59049, 1), (78125, 1)])  # odd: compile-time odd-radix factors
 def test_fourstep_f64(fy, orc, cuda, n, batch):
     check(fy, orc, gauss(np.random.default_rng(n), batch, n, "f64"), "fourstep")
+
+
+# Smooth lengths beyond LDS whose runtime-pass sub-transforms take the
+# composite radices (two levels in registers): between them every composite
+# radix appears, in the column and in the row pass, forward, inverse and
+# adjoint.  (n, split, composite radices the plan must show)
+COMPOSITE_CASES = [
+    (20580, "140(20x7) x 147(21x7)", (20, 21)),
+    (21000, "125(5x5x5) x 168(21x8)", (21,)),  # 125: the compile-time odd-radix column kernel
+    (21504, "112(16x7) x 192(16x12)", (12,)),
+    (21600, "144(16x9) x 150(25x6)", (25, 6)),
+    (21870, "135(15x9) x 162(9x9x2)", (15,)),
+    (21952, "112(16x7) x 196(14x14)", (14,)),
+    (25000, "125(5x5x5) x 200(20x10)", (20, 10)),
+    (96000, "240(20x12) x 400(20x20)", (20, 12)),
+]
+
+
+@pytest.mark.parametrize("n,split,radices", COMPOSITE_CASES)
+def test_fourstep_composite_radices(fy, orc, cuda, n, split, radices):
+    with fy.Plan(n, 1, fy.F32) as p:
+        desc = p.describe()
+    assert f"N={n} = {split} " in desc, desc
+    for r in radices:
+        assert re.search(rf"[(x]{r}[x)]", desc), (r, desc)
+    check(fy, orc, gauss(np.random.default_rng(n), 2, n, "f32"), "fourstep")
+    check(fy, orc, unit_phase(np.random.default_rng(n + 1), 3, n, "f32"), "fourstep", inplace=True)
 
 
 def test_fourstep_inplace(fy, orc, cuda):
