@@ -268,15 +268,25 @@ static size_t table_lds_bytes(size_t n, size_t elem) { return (std::max<size_t>(
 // octant reduction is exact in FP32), 0 (table lookups) beyond.
 static float fs_inv_n(size_t n) { return n <= ((size_t)1 << 24) ? (float)(1.0 / (double)n) : 0.0f; }
 
-static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int force_c = 0) {
+static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int force_c = 0,
+                        bool transposed_rows = false) {
   if (!lds_geometry(len, elem, cap, g)) return false;
   const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
   // 16 sub-transforms per workgroup make 128-byte row segments for FP32
   // columns; fewer where 16 would leave one workgroup per CU: measured,
   // N = 2^20 (1024 x 1024) 8 columns at two workgroups per CU 1.26 ms
   // against 16 at one 1.36 ms for 128 transforms, FP64 2^18 1.76 vs 2.06.
+  // But never below 8 where 8 fit at all: 32-byte segments cost more than
+  // the occupancy (N = 2^21 = 1024 x 2048 x 128: rows of 4 per workgroup
+  // 3.92 ms, of 8 at one workgroup per CU 2.18 ms).  The plain row pass
+  // (transposed_rows) writes its output in segments of as many rows as a
+  // workgroup holds and takes 32 where they fit half the LDS (256-byte
+  // segments: N = 65536 x 1024 0.461 -> 0.423 ms).
   int c = 16;
   while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
+  if (c < 8 && (size_t)8 * g->threads <= 1024 && (size_t)8 * stride * elem <= cap) c = 8;
+  if (transposed_rows && c == 16 && (size_t)32 * g->threads <= 1024 && (size_t)32 * stride * elem <= cap / 2)
+    c = 32;
   if (force_c > 0) {  // an autotuner candidate: any power of two that fits
     if ((force_c & (force_c - 1)) || (size_t)force_c * g->threads > 1024) return false;
     c = force_c;
@@ -324,7 +334,7 @@ static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
     return FYFFT_UNSUPPORTED;
   }
   if (!fs_geometry(p->n1, elem, cap, &p->gcol, f ? f->ccol : 0) ||
-      !fs_geometry(p->n2, elem, cap, &p->grow, f ? f->crow : 0))
+      !fs_geometry(p->n2, elem, cap, &p->grow, f ? f->crow : 0, !conv))
     return FYFFT_UNSUPPORTED;
   // a compile-time sub-transform when one matches the planner's pass list;
   // otherwise the runtime-pass sub-transforms of plain FP32 transforms take
