@@ -78,6 +78,10 @@ constexpr double kGroupedTuneSeconds = 20.0;
 // Internal plan form: TUNED restricted to what the runtime-pass kernels run
 // (sub-plans of tuned grouped plans, which never launch compile-time kernels).
 constexpr int kFormTunedGeneric = 16;
+// Internal plan form: PLAIN for the sub-plans of grouped plans, whose family
+// kernels run LDS-resident lengths at 32 points per thread better than as
+// extra four-step tasks (ragged step 6.20 ms against 6.28).
+constexpr int kFormGroupedPlain = 17;
 
 // Workspace of multi-pass plans is capped; larger batches run in chunks.
 constexpr size_t kWorkspaceCap = (size_t)1 << 30;
@@ -113,6 +117,7 @@ struct fyfft_plan {
   size_t n = 0, batch = 0;
   int prec = kF32;
   int form = FYFFT_FORM_AUTO;
+  bool grouped_sub = false;  // a sub-plan of a grouped plan (kFormGroupedPlain / kFormTunedGeneric)
   // tuning overrides for the structure being built (autotuner candidates)
   const FsChoice* force_fs = nullptr;
   size_t force_m = 0;
@@ -914,12 +919,34 @@ static fyfft_status fft_f64_inplace(double2* buf, size_t len) {
   return st;
 }
 
+// Lengths that fit one workgroup's LDS only beyond the low points-per-thread
+// bound (FP32 16384 < n <= 20480, FP64 8192 < n <= 10240) and have no
+// compile-time kernel take the four-step form in batched plans: the
+// runtime-pass kernel at 1024 threads and 32 points per thread keeps call
+// frames in scratch and one workgroup per CU (FP32 batches of 2^26 points:
+// N = 16800 1.11 -> 0.71 ms, 18432 0.80 -> 0.54 ms; Rader N = 18433
+// 2.01 -> 1.47 ms, 17011 2.76 -> 1.75 ms).  Grouped sub-plans keep them in
+// LDS (see kFormGroupedPlain).
+static bool lds_hi_to_fourstep(fyfft_plan* p, size_t n, const LdsGeom& g, bool conv = false) {
+  if (p->grouped_sub) return false;
+  const size_t elem = elem_bytes(p->prec);
+  const int lo = elem >= 16 ? 8 : 16;
+  if (g.ppt <= lo) return false;
+  std::vector<int> rad = radices_for(n);
+  dev::PassDesc pd{};
+  pd.npass = (int)std::min<size_t>(rad.size(), (size_t)dev::kMaxPasses);
+  for (int i = 0; i < pd.npass; ++i) pd.radix[i] = rad[i];
+  LdsGeom q = g;
+  if (conv ? find_conv_fixed(p->prec, pd, &q) >= 0 : find_fixed_kernel(p->prec, pd, &q) >= 0) return false;
+  return fs_possible(n, elem, lds_cap_bytes(p->device));
+}
+
 // Convolution structure of length m (LDS-fused or four-step) for a Rader or
 // Bluestein plan.
 static fyfft_status plan_conv_structure(fyfft_plan* p, size_t m, bool rader) {
   p->m = m;
   LdsGeom g;
-  if (lds_geometry(m, elem_bytes(p->prec), lds_cap_bytes(p->device), &g)) {
+  if (lds_geometry(m, elem_bytes(p->prec), lds_cap_bytes(p->device), &g) && !lds_hi_to_fourstep(p, m, g, true)) {
     p->kind = rader ? kRaderLds : kBlueLds;
     fyfft_status st = plan_lds(p, m, &p->lds, false, false);
     if (!st) p->lds.fixed = find_conv_fixed(p->prec, p->lds.pd, &p->lds.geom);
@@ -1361,8 +1388,10 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     return FYFFT_INVALID_ARGUMENT;
   if (n < 1 || n > FYFFT_MAX_LENGTH) return FYFFT_INVALID_LENGTH;
   if (form != FYFFT_FORM_AUTO && form != FYFFT_FORM_PLAIN && form != FYFFT_FORM_TUNED && form != FYFFT_FORM_MFMA &&
-      form != kFormTunedGeneric)
+      form != kFormTunedGeneric && form != kFormGroupedPlain)
     return FYFFT_INVALID_ARGUMENT;
+  const bool grouped_sub = form == kFormGroupedPlain || form == kFormTunedGeneric;
+  if (form == kFormGroupedPlain) form = FYFFT_FORM_PLAIN;
   const bool low = precision == FYFFT_BF16 || precision == FYFFT_F16;
   const bool tuned = form == FYFFT_FORM_TUNED || form == kFormTunedGeneric;
   if (form == FYFFT_FORM_MFMA && !low) return FYFFT_INVALID_ARGUMENT;
@@ -1381,6 +1410,7 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
   p->prec = precision == FYFFT_F64 ? kF64 : (precision == FYFFT_BF16 ? kBF16 : (precision == FYFFT_F16 ? kF16 : kF32));
   p->device = dev;
   p->form = form;
+  p->grouped_sub = grouped_sub;
   p->force_fs = force_fs;
   p->force_m = force_m;
   FsChoice tuned_fs;
@@ -1427,7 +1457,7 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
     st = plan_rader(p);
   } else if (is_smooth7(n)) {
     LdsGeom g;
-    if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g)) {
+    if (lds_geometry(n, elem_bytes(p->prec), lds_cap_bytes(dev), &g) && !lds_hi_to_fourstep(p, n, g)) {
       p->kind = kLds;
       st = tuned ? plan_lds_tuned(p, n, form == kFormTunedGeneric)
                  : plan_lds(p, n, &p->lds, form == FYFFT_FORM_AUTO, true, true);
@@ -2133,9 +2163,13 @@ static fyfft_status create_real_impl(fyfft_plan** out, size_t n, size_t batch, f
   p->strategy = strategy_for(n);
   const size_t elem = elem_bytes(p->prec), cap = lds_cap_bytes(dev), h = n / 2;
   LdsGeom g;
-  if (n % 2 == 0 && h >= 2 && is_smooth7(h) && lds_geometry(h, elem, cap, &g)) p->real_form = kRealFusedHalf;
+  // (the complex sub-plan takes the four-step form where a batched plan of
+  // its length would, see lds_hi_to_fourstep)
+  if (n % 2 == 0 && h >= 2 && is_smooth7(h) && lds_geometry(h, elem, cap, &g) && !lds_hi_to_fourstep(p, h, g))
+    p->real_form = kRealFusedHalf;
   else if (n % 2 == 0 && h >= 2 && is_smooth7(h) && fs_possible(h, elem, cap)) p->real_form = kRealHalfPasses;
-  else if (n >= 2 && is_smooth7(n) && lds_geometry(n, elem, cap, &g)) p->real_form = kRealFusedFull;
+  else if (n >= 2 && is_smooth7(n) && lds_geometry(n, elem, cap, &g) && !lds_hi_to_fourstep(p, n, g))
+    p->real_form = kRealFusedFull;
   else p->real_form = kRealFullPasses;
   const bool half = p->real_form == kRealFusedHalf || p->real_form == kRealHalfPasses;
   const bool fused = p->real_form == kRealFusedHalf || p->real_form == kRealFusedFull;
@@ -2733,13 +2767,13 @@ fyfft_status fyfft_plan_create_grouped(fyfft_plan** out, const size_t* lengths, 
       if (create_plan_impl(&probe, len, std::max<size_t>(cnt, 64), sub_prec, kFormTunedGeneric) == FYFFT_SUCCESS)
         fyfft_plan_destroy(probe);
       else
-        tune_form[len] = FYFFT_FORM_PLAIN;
+        tune_form[len] = kFormGroupedPlain;
     }
   }
   for (const auto& kv : by_len) {
     fyfft_plan* sp = nullptr;
     auto tf = tune_form.find(kv.first);
-    st = create_plan_impl(&sp, kv.first, 1, sub_prec, tf == tune_form.end() ? FYFFT_FORM_PLAIN : tf->second);
+    st = create_plan_impl(&sp, kv.first, 1, sub_prec, tf == tune_form.end() ? kFormGroupedPlain : tf->second);
     if (st) break;
     for (size_t i : kv.second) p->sub_of[i] = (int)p->subs.size();
     p->subs.push_back(sp);
