@@ -141,6 +141,7 @@ struct fyfft_plan {
   void* chirp[2] = {nullptr, nullptr};
   int* perm_in = nullptr;
   int* perm_out = nullptr;
+  int* pos_out = nullptr;  // LDS-resident Rader: inverse of perm_out (output position -> cyclic index)
   // four-step Rader: bucketed output permutation (chunk size rb_q; tables
   // of L entries) and the chunk x L staging buffer it runs through
   int rb_q = 0;
@@ -1046,6 +1047,13 @@ static fyfft_status plan_rader(fyfft_plan* p) {
   p->perm_out = (int*)d;
   HIP_TRY(hipMemcpy(p->perm_in, pin.data(), L * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->perm_out, pout.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  if (p->kind == kRaderLds) {
+    std::vector<int> pos(L);
+    for (size_t i = 0; i < L; ++i) pos[pout[i] - 1] = (int)i;
+    if ((st = p->alloc(L * sizeof(int), &d))) return st;
+    p->pos_out = (int*)d;
+    HIP_TRY(hipMemcpy(p->pos_out, pos.data(), L * sizeof(int), hipMemcpyHostToDevice));
+  }
   double2* bb = nullptr;
   HIP_TRY(hipMalloc(&bb, L * sizeof(double2)));
   for (int slot = 0; slot < 2 && !st; ++slot) {
@@ -1271,6 +1279,7 @@ static fyfft_status exec_conv_lds(const fyfft_plan* p, const V* in, V* out, bool
   a.tw = (const V*)p->lds.tw;
   a.perm_in = p->perm_in;
   a.perm_out = p->perm_out;
+  a.pos_out = p->pos_out;
   const int slot = inv_tables ? 1 : 0;
   a.chirp = (const V*)p->chirp[slot];
   a.kern = (const V*)p->kern[slot];
@@ -1597,6 +1606,7 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
   };
   g.lds.perm_in = s->perm_in;
   g.lds.perm_out = s->perm_out;
+  g.lds.pos_out = s->pos_out;
   g.lds.chirp = (const V*)s->chirp[0];
   g.lds.kern = (const V*)s->kern[0];
   g.chirp_inv = (const V*)s->chirp[1];

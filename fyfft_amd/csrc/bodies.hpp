@@ -174,22 +174,6 @@ template <typename V, typename E = V> struct GlobalSink {  // y[i] = v * scale
   S<V> scale;
   __device__ void operator()(int i, V v) const { Elem<V, E>::stnt((E*)y + i, cscale(v, scale)); }
 };
-template <typename V, typename E = V> struct RaderSrc {  // a[j] = x[g^-j mod N]
-  static constexpr bool kFromLds = false;
-  gbl_cptr<E> x;
-  gbl_cptr<int> perm;
-  __device__ V operator()(int j) const { return Elem<V, E>::ld((const E*)x + ((const int*)perm)[j]); }
-};
-template <typename V, typename E = V> struct RaderSink {  // y[g^i mod N] = (v + x0) * scale
-  static constexpr bool kToLds = false;
-  gbl_ptr<E> y;
-  gbl_cptr<int> perm;
-  V x0;
-  S<V> scale;
-  __device__ void operator()(int i, V v) const {
-    Elem<V, E>::st((E*)y + ((const int*)perm)[i], cscale(cadd(v, x0), scale));
-  }
-};
 // Spectrum times the kernel spectrum, back into LDS.  With y0 set, bin 0
 // (the sum of the Rader sequence) also gives y[0] = (x0 + bin 0) * scale.
 template <typename V, typename E = V> struct KernSink {
@@ -269,61 +253,68 @@ template <typename F> constexpr bool kStageConv = F::kThreads >= 640;
 // ------------------------------------------------------------ Rader
 // X[0] = x0 + sum_{n>=1} x[n];  X[g^i] = x0 + (a (*) b)[i],  a[j] = x[g^-j]
 // F: the I/O policy of the LDS transforms (GenericIo or FixedIo).
+// Both permutations run inside LDS, so every HBM access is coalesced:
+// x[1..L] is copied into LDS slots 0..L-1 as it lies, the first FFT pass
+// reads a[j] from slot g^-j - 1 (an in-place pass: all its reads precede its
+// writes), and after the inverse FFT y[1..L] is written in order, each y[s+1]
+// read from the cyclic position pos_out[s].  Gathering x[g^-j] and
+// scattering to y[g^i] straight from and to HBM cost an L2 request per
+// element (and partial-line writes) instead.
+template <typename V> struct LdsPermSrc {  // a[j] = LDS slot perm[j] - 1
+  static constexpr bool kFromLds = true;
+  lds_ptr<V> p;
+  int ps;
+  gbl_cptr<int> perm;
+  __device__ V operator()(int j) const {
+    const int q = ((const int*)perm)[j] - 1;
+    return ((const V*)p)[q + (q >> ps)];
+  }
+};
+
 template <typename V, typename F, typename E>
 __device__ __forceinline__ void body_rader(const LdsBody<V>& a, const E* x, E* y, V* lds_in, int tid,
                                            bool active) {
   using El = Elem<V, E>;
   V* lds = as_lds(lds_in);
+  const int L = a.m, ps = a.ps;
   const V x0 = active ? El::ld(x) : V{0, 0};
-  if constexpr (!kStageConv<F>) {
-    __syncthreads();  // every read of x[0] precedes the write of y[0] (in place)
-    F::template run<-1>(RaderSrc<V, E>{(gbl_cptr<E>)x, (gbl_cptr<int>)a.perm_in},
-                        KernSink<V, E>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<E>)y, x0, a.scale},
-                        lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-    F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                        RaderSink<V, E>{(gbl_ptr<E>)y, (gbl_cptr<int>)a.perm_out, x0, a.scale},
-                        lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-    return;
-  }
-  // the permuted input gathered into LDS eight points per thread at a time;
-  // every read of x precedes the barrier, so the writes of y below may
-  // overwrite it (in place)
+  // x[1..L] into LDS, eight loads per thread in flight
   if (active) {
-    const int* perm = a.perm_in;
-    for (int j0 = tid; j0 < a.m; j0 += 8 * a.threads) {
+    for (int s0 = tid; s0 < L; s0 += 8 * a.threads) {
       V v[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int j = j0 + q * a.threads;
-        if (j < a.m) v[q] = El::ld(x + perm[j]);
+        const int s = s0 + q * a.threads;
+        if (s < L) v[q] = El::ldnt(x + 1 + s);
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int j = j0 + q * a.threads;
-        if (j < a.m) lds[j + (j >> a.ps)] = v[q];
+        const int s = s0 + q * a.threads;
+        if (s < L) lds[s + (s >> ps)] = v[q];
       }
     }
   }
+  // every read of x (x[0] included) precedes this barrier, so y may alias x
   __syncthreads();
-  F::template run<-1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps},
-                      KernSink<V, E>{(lds_ptr<V>)lds, a.ps, (gbl_cptr<V>)a.kern, (gbl_ptr<E>)y, x0, a.scale},
-                      lds, a.m, a.threads, tid, a.pd, a.tw, active, a.ps);
-  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, a.ps}, LdsSink<V>{(lds_ptr<V>)lds, a.ps}, lds, a.m, a.threads,
-                      tid, a.pd, a.tw, active, a.ps);
-  // y[g^i] = (c[i] + x0) * scale, staged like the input
+  F::template run<-1>(LdsPermSrc<V>{(lds_ptr<V>)lds, ps, (gbl_cptr<int>)a.perm_in},
+                      KernSink<V, E>{(lds_ptr<V>)lds, ps, (gbl_cptr<V>)a.kern, (gbl_ptr<E>)y, x0, a.scale},
+                      lds, L, a.threads, tid, a.pd, a.tw, active, ps);
+  F::template run<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, L, a.threads,
+                      tid, a.pd, a.tw, active, ps);
+  // y[s + 1] = (c[pos_out[s]] + x0) * scale, in order
   if (active) {
-    const int* perm = a.perm_out;
-    for (int i0 = tid; i0 < a.m; i0 += 8 * a.threads) {
-      V v[8];
+    const int* pos = a.pos_out;
+    for (int s0 = tid; s0 < L; s0 += 8 * a.threads) {
+      int d[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int i = i0 + q * a.threads;
-        if (i < a.m) v[q] = cscale(cadd(lds[i + (i >> a.ps)], x0), a.scale);
+        const int s = s0 + q * a.threads;
+        d[q] = s < L ? pos[s] : 0;
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const int i = i0 + q * a.threads;
-        if (i < a.m) El::st(y + perm[i], v[q]);
+        const int s = s0 + q * a.threads;
+        if (s < L) El::stnt(y + 1 + s, cscale(cadd(lds[d[q] + (d[q] >> ps)], x0), a.scale));
       }
     }
   }

@@ -21,6 +21,7 @@ template <typename V> struct LdsBody {
   const V* tw = nullptr;
   const int* perm_in = nullptr;
   const int* perm_out = nullptr;
+  const int* pos_out = nullptr;  // Rader: pos_out[s] = i with g^i = s + 1 (mod n)
   const V* chirp = nullptr;
   const V* kern = nullptr;
   S<V> scale = 1;
@@ -142,6 +143,7 @@ template <typename V> struct ConvArgs {
   const V* tw = nullptr;       // Stockham table of m
   const int* perm_in = nullptr;   // Rader: g^-j mod N
   const int* perm_out = nullptr;  // Rader: g^i mod N
+  const int* pos_out = nullptr;   // Rader: i with g^i = s + 1 mod N, s < N - 1
   const V* chirp = nullptr;    // Bluestein chirp for this direction
   const V* kern = nullptr;     // kernel spectrum / m, natural order
   dev::S<V> scale = 1;

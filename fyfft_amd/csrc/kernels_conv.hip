@@ -30,6 +30,7 @@ template <typename V> __device__ __forceinline__ LdsBody<V> conv_body(const Conv
   b.tw = a.tw;
   b.perm_in = a.perm_in;
   b.perm_out = a.perm_out;
+  b.pos_out = a.pos_out;
   b.chirp = a.chirp;
   b.kern = a.kern;
   b.scale = a.scale;
