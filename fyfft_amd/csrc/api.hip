@@ -297,13 +297,16 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int for
     if ((force_c & (force_c - 1)) || (size_t)force_c * g->threads > 1024) return false;
     c = force_c;
   }
-  if ((size_t)c * stride * elem > cap) return false;
+  // the twiddled column passes and the convolution row passes add their
+  // tables of roots (fs_col_twiddled, fs_row_tw_len)
+  const size_t roots = (size_t)c * std::max(fs_col_tw_row(g->ppt), fs_row_tw_len((int)len)) * elem;
+  if ((size_t)c * stride * elem + roots > cap) return false;
   g->per_block = c;
   g->lds_bytes = (size_t)c * stride * elem;
   // the runtime-pass sub-transform reads its Stockham table from a copy in
   // LDS after the tile where that fits (compile-time sub-transforms ignore it)
   const size_t tw = table_lds_bytes(len, elem);
-  g->tw_bytes = g->lds_bytes + tw <= cap ? tw : 0;
+  g->tw_bytes = g->lds_bytes + tw + roots <= cap ? tw : 0;
   return true;
 }
 
@@ -317,6 +320,10 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int for
 // lengths that are not powers of two take the cost-model split (their
 // runtime-pass sub-transforms vary widely in speed with the factors).
 static bool fs_split(size_t n, size_t elem, bool conv, size_t* n1, size_t* n2) {
+  if (const char* e = std::getenv("FYFFT_EXP_CONV_N1"); conv && e) {
+    const size_t d = (size_t)std::atol(e);
+    if (d > 1 && n % d == 0) { *n1 = d; *n2 = n / d; return true; }
+  }
   return (conv && four_step_split_lds(n, elem, n1, n2)) || (!conv && four_step_split_model(n, kFactorCap, elem, n1, n2)) ||
          four_step_split(n, kFactorCap, n1, n2);
 }
@@ -1525,9 +1532,11 @@ static size_t class_lds_cap(size_t cap, int k) { return cap * (size_t)kGroupClas
 // The tile is followed by the LDS copy of the sub-length's Stockham table
 // (*tl) when some class holds a one-column tile and the table; otherwise
 // the table is read from global memory.
-static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t cap, int* cls, int* log2c,
-                             bool* tl) {
-  const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem;
+static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t cap, size_t roots, int* cls,
+                             int* log2c, bool* tl) {
+  // roots: bytes of root tables per sub-transform (twiddled column passes,
+  // convolution row passes)
+  const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem + roots;
   const size_t twb = table_lds_bytes(len, elem);
   *tl = per + twb <= class_lds_cap(cap, kGroupClasses - 1);
   const size_t extra = *tl ? twb : 0;
@@ -1589,7 +1598,11 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
                      int* cls, bool* tl) {
     a.n1 = (int)s->n1;
     a.n2 = (int)s->n2;
-    grouped_fs_class(q, len, sizeof(V), cap, cls, &a.log2c, tl);
+    // root tables per sub-transform: a row of roots per column (the family
+    // kernels run 16 FP32 / 8 FP64 points per thread), per convolution row
+    const size_t roots = (&a == &g.col ? fs_col_tw_row(sizeof(V) == 8 ? 16 : 8)
+                          : g.kind != kGkFsPlain ? fs_row_tw_len((int)len) : 0) * sizeof(V);
+    grouped_fs_class(q, len, sizeof(V), cap, roots, cls, &a.log2c, tl);
     a.threads = q.threads;
     a.ps = q.pad_shift;
     a.stride = dev::lds_len((int)len, q.pad_shift);
@@ -1813,12 +1826,14 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
       // hi = 1: the global-table form of the family (no room for the table)
       const int chi = lp.col_tl ? 0 : 1, rhi = lp.row_tl ? 0 : 1;
       const size_t col_lds = (size_t)C * g.col.stride * elem + (lp.col_tl ? table_lds_bytes(g.col.n1, elem) : 0);
-      const size_t row_lds = (size_t)R * g.row.stride * elem + (lp.row_tl ? table_lds_bytes(g.col.n2, elem) : 0);
+      const size_t colA_lds = col_lds + (size_t)C * fs_col_tw_row(kLo) * elem;
+      const size_t row_lds = (size_t)R * g.row.stride * elem + (lp.row_tl ? table_lds_bytes(g.col.n2, elem) : 0) +
+                             (g.kind != kGkFsPlain ? (size_t)R * fs_row_tw_len(g.col.n2) * elem : 0);
       for (size_t i = 0; i < idx.size(); ++i) {
         for (int c0 = 0; c0 < g.col.n2; c0 += C) {
           const double frac = (double)g.n / ((double)g.col.n1 * g.col.n2);  // data share of the conv length
           add(famA, kc, chi, T{GTask{li, 0, first + (int)i, c0}, (double)C * g.col.n1 * lg,
-                               (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);
+                               (double)C * g.col.n1 * (frac * eio + elem)}, colA_lds);
           if (g.kind != kGkFsPlain)
             add(famC, kc, chi, T{GTask{li, 2, first + (int)i, c0}, (double)C * g.col.n1 * lg,
                                  (double)C * g.col.n1 * (frac * eio + elem)}, col_lds);

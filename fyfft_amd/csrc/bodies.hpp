@@ -36,6 +36,15 @@ __device__ __forceinline__ void staged(int start, int step, int total, LoadF ld,
   }
 }
 
+// The same value, opaque to the optimiser: index arithmetic recomputed from
+// it is not merged with identical arithmetic before a long stretch of code
+// (the compiler otherwise keeps every element's LDS offset of a load loop
+// alive across the FFT for the store loop, and spills).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // FFT policies for the bodies: how the LDS-resident sub-transform is run.
 // GenericFft: runtime pass list (any smooth length), passes out of line.
 // TL: the four-step bodies stage the sub-length's Stockham table in LDS
@@ -211,24 +220,17 @@ template <typename V, typename E = V> struct BlueSink {  // y[k] = v * chirp[k] 
 
 // Conjugate four-step twiddle at the store: y[k2] = v * conj(w_N^(k2*k1)),
 // w_N^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)].
+// y[k2] = v * w_N^(k1*k2) (the inverse direction), the root as the product
+// of two entries of the row's table in LDS: hi[k2 >> 4] = w^(16*k1*(k2 >> 4)),
+// lo[k2 & 15] = w^(k1*(k2 & 15)) (lo at tab + h).
 template <typename V> struct TwSink {
   static constexpr bool kToLds = false;
   gbl_ptr<V> y;
-  gbl_cptr<V> hi, lo;
-  int lo_bits;
-  long long k1;
-  int n;        // n1 * n2
-  float inv_n;  // nonzero: FP32 root computed (big_twiddle)
+  lds_cptr<V> tab;
+  int h;
   __device__ void operator()(int k2, V v) const {
-    const long long m = (long long)k2 * k1;
-    if constexpr (sizeof(V) == 8) {
-      if (inv_n != 0.0f) {
-        ((V*)y)[k2] = cmul(v, root_f32<+1>((int)m, n, inv_n));
-        return;
-      }
-    }
-    const V w = cmul(((const V*)hi)[m >> lo_bits], ((const V*)lo)[m & ((1LL << lo_bits) - 1)]);
-    ((V*)y)[k2] = cmulc(v, w);
+    const V* t = (const V*)tab;
+    ((V*)y)[k2] = cmul(v, cmul(t[k2 >> 4], t[h + (k2 & 15)]));
   }
 };
 
@@ -394,84 +396,106 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
 
 // Column pass over columns [c0, c0 + C) of one transform (C = 1 << log2c).
 // x: the transform's input, tmp: its workspace, out: its output, x0: its
-// Rader x[0] slot.  Element i of the workgroup's N1 x C tile is row
-// n1 = i >> log2c, column c = i & (C - 1), so a wavefront reads C-element
-// row segments.
+// Rader x[0] slot.  Thread t moves column c = t & (C - 1) of the N1 x C
+// tile, rows n1 = (t >> log2c) + j*s with s = blockDim >> log2c (blockDim is
+// a multiple of C), so a wavefront reads and writes C-element row segments
+// and a thread's offsets advance by constant steps.  The twiddled modes
+// take w_N^(n2*k1) as w^(n2*k0) * w^(n2*s*j): the first factor once per
+// thread, the second from the workgroup's table of C x P roots in LDS
+// (filled before the FFT, a few roots per thread) -- one complex product
+// per element instead of one computed root (the pass was bound by VALU
+// issue, half of it the per-element roots and index arithmetic).
 template <typename V, typename F, int DIR, int MODE, typename E>
 __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* tmp, E* out, V* x0,
                                             int c0, V* lds_in, V* zw = nullptr) {
   using El = Elem<V, E>;
   constexpr int P = F::kPpt;
+  constexpr bool kTw = fs_col_twiddled(MODE);
+  constexpr int TR = fs_col_tw_row(P);  // root table row length
   V* lds_all = as_lds(lds_in);
   const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c, lc = a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
-  const int total = N1 << lc;
   const int ncols = min(C, N2 - c0);
-  staged<P>(threadIdx.x, blockDim.x, total,
-            [&](int i) {
-              const int n1 = i >> lc, c = i & (C - 1);
-              const int n = n1 * N2 + c0 + (c < ncols ? c : 0);
-              V v;
-              if constexpr (MODE == kFsPlainA) v = a.col_cached ? El::ld(x + n) : El::ldnt(x + n);
-              else if constexpr (MODE == kFsRaderA) v = El::ld(x + a.perm[n]);
-              else if constexpr (MODE == kFsBlueA) v = n < a.n_orig ? cmul(El::ld(x + n), a.chirp[n]) : V{0, 0};
-              else v = tmp[n];
-              return v;
-            },
-            [&](int i, V v) {
-              const int n1 = i >> lc, c = i & (C - 1);
-              if (c < ncols) lds_all[c * stride + n1 + (n1 >> ps)] = v;
-            });
+  const int c = threadIdx.x & (C - 1), kr0 = threadIdx.x >> lc, s = blockDim.x >> lc;
+  const bool cok = c < ncols;
+  const int n2 = c0 + (cok ? c : 0);  // the thread's column of the transform (clamped)
+  const int gs = s * N2;              // offset step from row to row
+  V* lcol = lds_all + c * stride;
+  {
+    const int k0 = kr0, g0 = k0 * N2 + n2;
+    V buf[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int n = g0 + j * gs;
+      if (k0 + j * s < N1) {
+        if constexpr (MODE == kFsPlainA) buf[j] = a.col_cached ? El::ld(x + n) : El::ldnt(x + n);
+        else if constexpr (MODE == kFsRaderA) buf[j] = El::ld(x + a.perm[n]);
+        else if constexpr (MODE == kFsBlueA) buf[j] = n < a.n_orig ? cmul(El::ld(x + n), a.chirp[n]) : V{0, 0};
+        else buf[j] = tmp[n];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int n1 = k0 + j * s;
+      if (n1 < N1 && cok) lcol[n1 + (n1 >> ps)] = buf[j];
+    }
+  }
+  const int g0 = opaque(kr0 * N2 + n2);
   if constexpr (MODE == kFsRaderA) {
     if (c0 == 0 && threadIdx.x == 0) *x0 = El::ld(x);
   }
   const V* tw = stage_table<F>(a.tw, N1, lds_all + C * stride);
+  V* ptab = lds_all + C * stride + (F::kTwLds ? max(N1, 2) - 1 : 0);
+  if constexpr (kTw) {
+    for (int e = threadIdx.x; e < C * P; e += blockDim.x) {
+      const int ec = e / P, j = e - ec * P, k = j * s;
+      ptab[ec * TR + j] = k < N1 ? big_twiddle<DIR>(a, (long long)(c0 + (ec < ncols ? ec : 0)) * k)
+                                             : V{1, 0};
+    }
+  }
   __syncthreads();
   const bool active = slot < C && slot < ncols;
   F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N1, a.threads, tid, a.pd, tw, active, ps);
   V xz = V{0, 0};
   if constexpr (MODE == kFsRaderC || MODE == kFsRaderC2) xz = *x0;
+  const int k0 = opaque(kr0);
   if constexpr (MODE == kFsRaderC2) {
     // runs ordered by destination chunk, written contiguously
     const long long bstart = (long long)c0 * N1;  // this column group's first element
     staged<P>(threadIdx.x, blockDim.x, N1 * ncols, [&](int i) { return a.lidx[bstart + i]; },
               [&](int i, int l) { zw[bstart + i] = cscale(cadd(lds_all[l], xz), a.scale); });
-  } else if constexpr (MODE == kFsPlainA || MODE == kFsRaderA || MODE == kFsBlueA) {
-    staged<P>(threadIdx.x, blockDim.x, total,
-                 [&](int i) {
-                   const int k1 = i >> lc, c = i & (C - 1), n2 = c0 + c;
-                   return big_twiddle<DIR>(a, (long long)(c < ncols ? n2 : c0) * k1);
-                 },
-                 [&](int i, V w) {
-                   const int k1 = i >> lc, c = i & (C - 1);
-                   if (c < ncols) tmp[(long long)k1 * N2 + c0 + c] = cmul(lds_all[c * stride + k1 + (k1 >> ps)], w);
-                 });
-  } else if constexpr (MODE == kFsRaderC) {
-    // k1 runs over n1 of the convolution output here
-    staged<P>(threadIdx.x, blockDim.x, total,
-                 [&](int i) {
-                   const int k1 = i >> lc, c = i & (C - 1);
-                   return a.perm[k1 * N2 + c0 + (c < ncols ? c : 0)];
-                 },
-                 [&](int i, int d) {
-                   const int k1 = i >> lc, c = i & (C - 1);
-                   if (c < ncols) El::st(out + d, cscale(cadd(lds_all[c * stride + k1 + (k1 >> ps)], xz), a.scale));
-                 });
-  } else {  // kFsBlueC
-    staged<P>(threadIdx.x, blockDim.x, total,
-                 [&](int i) {
-                   const int k1 = i >> lc, c = i & (C - 1);
-                   const int n = k1 * N2 + c0 + (c < ncols ? c : 0);
-                   return n < a.n_orig ? a.chirp[n] : V{0, 0};
-                 },
-                 [&](int i, V ch) {
-                   const int k1 = i >> lc, c = i & (C - 1);
-                   const int n = k1 * N2 + c0 + c;
-                   if (c < ncols && n < a.n_orig)
-                     El::st(out + n, cscale(cmul(lds_all[c * stride + k1 + (k1 >> ps)], ch), a.scale));
-                 });
+  } else {
+    // every element's LDS value (and table entry) read before the first
+    // store, so the reads overlap; rows past N1 read row N1 - 1 and store
+    // nothing
+    V val[P];
+    [[maybe_unused]] V aux[P];
+    [[maybe_unused]] int dst[P];
+    [[maybe_unused]] V wb = V{1, 0};
+    if constexpr (kTw) wb = big_twiddle<DIR>(a, (long long)n2 * min(k0, N1 - 1));
+    const V* prow = ptab + c * TR;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int k1 = min(k0 + j * s, N1 - 1), n = g0 + j * gs;
+      val[j] = lcol[k1 + (k1 >> ps)];
+      if constexpr (kTw) aux[j] = prow[j];
+      else if (k0 + j * s < N1 && cok) {
+        // closing modes (k1 runs over n1 of the convolution output)
+        if constexpr (MODE == kFsRaderC) dst[j] = a.perm[n];
+        else aux[j] = n < a.n_orig ? a.chirp[n] : V{0, 0};
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int n = g0 + j * gs;
+      if (k0 + j * s < N1 && cok) {
+        if constexpr (kTw) tmp[n] = cmul(val[j], cmul(wb, aux[j]));
+        else if constexpr (MODE == kFsRaderC) El::st(out + dst[j], cscale(cadd(val[j], xz), a.scale));
+        else if (n < a.n_orig) El::st(out + n, cscale(cmul(val[j], aux[j]), a.scale));
+      }
+    }
   }
 }
 
@@ -522,6 +546,18 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
   // the table copy is published by the barrier of stage_rows or of the
   // first pass, which reads no twiddles
   const V* tw = stage_table<F>(a.tw, N2, lds_all + C * stride);
+  // convolution rows: each row's table of conjugate four-step roots (the
+  // inverse FFT's last store reads it, many barriers later)
+  V* rtab = lds_all + C * stride + (F::kTwLds ? max(N2, 2) - 1 : 0);
+  const int th = fs_row_tw_hi(N2), tl = fs_row_tw_len(N2);
+  if constexpr (MODE == kFsConvB) {
+    for (int e = threadIdx.x; e < C * tl; e += blockDim.x) {
+      const int r = e / tl, i = e - r * tl;
+      const long long k1 = min(r0 + r, N1 - 1);
+      const int b = i - th;
+      rtab[e] = i < th ? big_twiddle<+1>(a, k1 * 16 * i) : (b < N2 ? big_twiddle<+1>(a, k1 * b) : V{1, 0});
+    }
+  }
   if (a.stage_rows) stage_rows<V, F::kPpt>(tmp + (long long)r0 * N2, rows, N2, lds_all, stride, ps);
   if constexpr (MODE == kFsPlainB) {
     // each slot's first pass reads its (contiguous) row straight from HBM
@@ -533,15 +569,28 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     else
       F::template run_io<DIR>(GlobalSrc<V>{(gbl_cptr<V>)src}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, N2,
                               a.threads, tid, a.pd, tw, active, ps);
-    for (int i = threadIdx.x; i < (N2 << a.log2c); i += blockDim.x) {
-      const int k2 = i >> a.log2c, c = i & (C - 1);
-      if (c >= rows) continue;
-      E* dst = out + (long long)(r0 + c) + (long long)N1 * k2;
-      const V v = cscale(lds_all[c * stride + k2 + (k2 >> ps)], a.scale);
-      if (a.out_cached)
-        Elem<V, E>::st(dst, v);
-      else
-        Elem<V, E>::stnt(dst, v);
+    // thread t stores row c = t & (C - 1) at k2 = (t >> log2c) + j*s, so a
+    // wavefront writes C-element segments of the output
+    // (all of a thread's LDS reads first, so they overlap; positions past
+    // N2 read position N2 - 1 and store nothing)
+    constexpr int P = F::kPpt;
+    const int c = threadIdx.x & (C - 1), k0 = threadIdx.x >> a.log2c, s = blockDim.x >> a.log2c;
+    const V* lrow = lds_all + c * stride;
+    const int o0 = r0 + c + N1 * k0, os = N1 * s;
+    V val[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int k2 = min(k0 + j * s, N2 - 1);
+      val[j] = cscale(lrow[k2 + (k2 >> ps)], a.scale);
+    }
+    if (a.out_cached) {
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+        if (k0 + j * s < N2 && c < rows) Elem<V, E>::st(out + (o0 + j * os), val[j]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < P; ++j)
+        if (k0 + j * s < N2 && c < rows) Elem<V, E>::stnt(out + (o0 + j * os), val[j]);
     }
   } else {  // kFsConvB: one row per slot, read and written in place, contiguous
     // forward FFT straight from HBM, kernel multiply as the last pass's sink,
@@ -558,7 +607,7 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     else
       F::template run_io<-1>(GlobalSrc<V>{(gbl_cptr<V>)row}, ks, lds, N2, a.threads, tid, a.pd, tw, active, ps);
     F::template run_io<+1>(LdsSrc<V>{(lds_ptr<V>)lds, ps},
-                           TwSink<V>{(gbl_ptr<V>)row, (gbl_cptr<V>)a.tw_hi, (gbl_cptr<V>)a.tw_lo, a.lo_bits, k1, N1 * N2, a.inv_n},
+                           TwSink<V>{(gbl_ptr<V>)row, (lds_cptr<V>)(rtab + (slot < C ? slot : 0) * tl), th},
                            lds, N2, a.threads, tid, a.pd, tw, active, ps);
   }
 }

@@ -77,6 +77,18 @@ enum FsMode {
   kFsConvB = 6,   // row pass of a convolution (fwd, * kernel, inv, conj twiddle)
   kFsRaderC2 = 8  // inverse column pass writing bucket runs (zw) + x0
 };
+// Column passes that apply the four-step twiddle on their store.  Their
+// workgroups keep C rows of roots in LDS after the tile and the Stockham
+// table: entry j of column c is w_N^(n2 * s * j), j below the points per
+// thread, and one entry of padding against bank conflicts.
+constexpr bool fs_col_twiddled(int mode) { return mode == kFsPlainA || mode == kFsRaderA || mode == kFsBlueA; }
+constexpr int fs_col_tw_row(int ppt) { return ppt + 1; }
+// The convolution row pass applies the conjugate four-step twiddle
+// w_N^(-k1*k2) at its last store as hi[k2 >> 4] * lo[k2 & 15]: per row a
+// table of fs_row_tw_hi(n2) + 16 roots in LDS after the tile and the
+// Stockham table.
+constexpr int fs_row_tw_hi(int n2) { return (n2 - 1) / 16 + 1; }
+constexpr int fs_row_tw_len(int n2) { return fs_row_tw_hi(n2) + 16; }
 
 template <typename V> struct FsArgs {
   const V* in = nullptr;

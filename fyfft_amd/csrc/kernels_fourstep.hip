@@ -114,16 +114,22 @@ hipError_t launch_fs_pass(int prec, bool col, int mode, int dir, const void* arg
                           const LdsGeom& g, int fixed_n, hipStream_t s) {
   if (batch == 0) return hipSuccess;
   const int threads = g.threads * g.per_block;
+  // the twiddled column passes and the convolution row passes keep their
+  // tables of roots after the tile (and after the Stockham table, which
+  // dispatch_generic adds)
+  const size_t elem = prec == kF64 ? 16 : 8;
+  const size_t roots = col && fs_col_twiddled(mode) ? (size_t)fs_col_tw_row(g.ppt)
+                       : !col && mode == kFsConvB ? (size_t)fs_row_tw_len(g.n)
+                                                  : 0;
+  const size_t lds = g.lds_bytes + (size_t)g.per_block * roots * elem;
   if (prec == kF64) {
     const auto& a = *static_cast<const FsArgs<double2>*>(args);
-    return g.ppt <= 8
-               ? dispatch_len<double2, 8>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s)
-               : dispatch_len<double2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s);
+    return g.ppt <= 8 ? dispatch_len<double2, 8>(fixed_n, col, mode, dir, a, batch, lds, g.tw_bytes, threads, s)
+                      : dispatch_len<double2, 16>(fixed_n, col, mode, dir, a, batch, lds, g.tw_bytes, threads, s);
   }
   const auto& a = *static_cast<const FsArgs<float2>*>(args);
-  return g.ppt <= 16
-             ? dispatch_len<float2, 16>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s)
-             : dispatch_len<float2, 32>(fixed_n, col, mode, dir, a, batch, g.lds_bytes, g.tw_bytes, threads, s);
+  return g.ppt <= 16 ? dispatch_len<float2, 16>(fixed_n, col, mode, dir, a, batch, lds, g.tw_bytes, threads, s)
+                     : dispatch_len<float2, 32>(fixed_n, col, mode, dir, a, batch, lds, g.tw_bytes, threads, s);
 }
 
 }  // namespace fyfft
