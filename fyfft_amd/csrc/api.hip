@@ -1563,6 +1563,32 @@ static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t c
   *log2c = ilog2_exact(c);
 }
 
+// Rows per workgroup of a convolution row pass.  Its rows are read and
+// written in place, contiguously, so any count serves (the column and plain
+// row passes keep powers of two for their segment widths): the count that
+// fills the most threads of a class within its LDS budget, the smallest
+// class reaching 90 %.  A row of 300 points (19 threads) takes 13 rows in
+// the 256-thread class (96 % of its threads) instead of 16 in the 512-thread
+// class (59 %).
+static void grouped_conv_rows(const LdsGeom& q, size_t len, size_t elem, size_t cap, size_t roots, bool tl,
+                              int* cls, int* rows_per) {
+  const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem + roots;
+  const size_t extra = tl ? table_lds_bytes(len, elem) : 0;
+  double best = 0;
+  for (int k = 0; k < kGroupClasses; ++k) {
+    if ((size_t)q.threads > (size_t)kGroupClassThreads[k] || per + extra > class_lds_cap(cap, k)) continue;
+    const int r = (int)std::min<size_t>({64, (size_t)kGroupClassThreads[k] / q.threads,
+                                         (class_lds_cap(cap, k) - extra) / per});
+    const double use = (double)r * q.threads / kGroupClassThreads[k];
+    if (use > best + 1e-9) {
+      best = use;
+      *cls = k;
+      *rows_per = r;
+    }
+    if (best >= 0.9) break;
+  }
+}
+
 template <typename V>
 struct GLenPlan {
   GLen<V> g;
@@ -1600,9 +1626,11 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
     a.n2 = (int)s->n2;
     // root tables per sub-transform: a row of roots per column (the family
     // kernels run 16 FP32 / 8 FP64 points per thread), per convolution row
+    const bool conv_row = &a == &g.row && g.kind != kGkFsPlain;
     const size_t roots = (&a == &g.col ? fs_col_tw_row(sizeof(V) == 8 ? 16 : 8)
-                          : g.kind != kGkFsPlain ? fs_row_tw_len((int)len) : 0) * sizeof(V);
+                          : conv_row ? fs_row_tw_len((int)len) : 0) * sizeof(V);
     grouped_fs_class(q, len, sizeof(V), cap, roots, cls, &a.log2c, tl);
+    if (conv_row) grouped_conv_rows(q, len, sizeof(V), cap, roots, *tl, cls, &a.rows_per);
     a.threads = q.threads;
     a.ps = q.pad_shift;
     a.stride = dev::lds_len((int)len, q.pad_shift);
@@ -1817,7 +1845,7 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
       }
     } else {
       if (g.col_ppt > kLo || g.row_ppt > kLo) return FYFFT_UNSUPPORTED;
-      const int C = 1 << g.col.log2c, R = 1 << g.row.log2c;
+      const int C = 1 << g.col.log2c, R = g.row.rows_per ? g.row.rows_per : 1 << g.row.log2c;
       const double conv = g.kind == kGkFsPlain ? 1.0 : 2.0;
       const int kc = lp.col_cls, kr = lp.row_cls;
       const int famA = g.kind == kGkFsPlain ? kGfColPlain : (g.kind == kGkFsRader ? kGfColRaderA : kGfColBlueA);

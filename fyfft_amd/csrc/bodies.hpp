@@ -526,7 +526,8 @@ __device__ __forceinline__ void stage_rows(const V* span, int rows, int N2, V* l
   __syncthreads();
 }
 
-// Row pass over rows [r0, r0 + C) of one transform's workspace.  Rows whose
+// Row pass over rows [r0, r0 + C) of one transform's workspace (C = 1 <<
+// log2c, or rows_per for convolution rows).  Rows whose
 // length is a whole number of 128-byte lines are read by each slot's first
 // pass straight from HBM (5-11 % faster than staging them); other rows start
 // mid-line, so a slot's loads touch two lines per segment and the pass read
@@ -536,7 +537,7 @@ template <typename V, typename F, int DIR, int MODE, typename E>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, const V* x0,
                                             int r0, V* lds_in) {
   V* lds_all = as_lds(lds_in);
-  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c;
+  const int N1 = a.n1, N2 = a.n2, C = a.rows_per ? a.rows_per : 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
