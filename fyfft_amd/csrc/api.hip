@@ -1563,27 +1563,28 @@ static void grouped_fs_class(const LdsGeom& q, size_t len, size_t elem, size_t c
   *log2c = ilog2_exact(c);
 }
 
-// Rows per workgroup of a convolution row pass.  Its rows are read and
-// written in place, contiguously, so any count serves (the column and plain
-// row passes keep powers of two for their segment widths): the count that
-// fills the most threads of a class within its LDS budget, the smallest
-// class reaching 90 %.  A row of 300 points (19 threads) takes 13 rows in
-// the 256-thread class (96 % of its threads) instead of 16 in the 512-thread
-// class (59 %).
-static void grouped_conv_rows(const LdsGeom& q, size_t len, size_t elem, size_t cap, size_t roots, bool tl,
-                              int* cls, int* rows_per) {
+// Sub-transforms per workgroup (a multiple of gran, up to 64) of a grouped
+// four-step pass: the count that fills the most threads of a class within
+// its LDS budget, the smallest class reaching 90 %; unchanged (the power of
+// two of grouped_fs_class) where no class fits gran of them.  A convolution
+// row of 300 points (19 threads) takes 13 rows in the 256-thread class (96 %
+// of its threads) instead of 16 in the 512-thread class (59 %).
+static void grouped_sub_count(const LdsGeom& q, size_t len, size_t elem, size_t cap, size_t roots, bool tl,
+                              int gran, int* cls, int* nsub) {
   const size_t per = (size_t)dev::lds_len((int)len, q.pad_shift) * elem + roots;
   const size_t extra = tl ? table_lds_bytes(len, elem) : 0;
   double best = 0;
   for (int k = 0; k < kGroupClasses; ++k) {
-    if ((size_t)q.threads > (size_t)kGroupClassThreads[k] || per + extra > class_lds_cap(cap, k)) continue;
-    const int r = (int)std::min<size_t>({64, (size_t)kGroupClassThreads[k] / q.threads,
-                                         (class_lds_cap(cap, k) - extra) / per});
+    if ((size_t)gran * q.threads > (size_t)kGroupClassThreads[k] || gran * per + extra > class_lds_cap(cap, k))
+      continue;
+    int r = (int)std::min<size_t>({64, (size_t)kGroupClassThreads[k] / q.threads,
+                                   (class_lds_cap(cap, k) - extra) / per});
+    r -= r % gran;
     const double use = (double)r * q.threads / kGroupClassThreads[k];
     if (use > best + 1e-9) {
       best = use;
       *cls = k;
-      *rows_per = r;
+      *nsub = r;
     }
     if (best >= 0.9) break;
   }
@@ -1630,7 +1631,12 @@ static GLenPlan<V> make_glen(const fyfft_plan* s, size_t count_len, size_t cap, 
     const size_t roots = (&a == &g.col ? fs_col_tw_row(sizeof(V) == 8 ? 16 : 8)
                           : conv_row ? fs_row_tw_len((int)len) : 0) * sizeof(V);
     grouped_fs_class(q, len, sizeof(V), cap, roots, cls, &a.log2c, tl);
-    if (conv_row) grouped_conv_rows(q, len, sizeof(V), cap, roots, *tl, cls, &a.rows_per);
+    // then the count filling the class best: any count for convolution rows
+    // (read and written in place), multiples of 16 for columns and plain
+    // rows, whose segments stay whole 128-byte lines (ragged step, 20000
+    // lengths: powers of two 5.48 ms, multiples of 16 5.26, of 8 5.28, of 4
+    // 5.37, any 5.57)
+    grouped_sub_count(q, len, sizeof(V), cap, roots, *tl, conv_row ? 1 : 16, cls, &a.nsub);
     a.threads = q.threads;
     a.ps = q.pad_shift;
     a.stride = dev::lds_len((int)len, q.pad_shift);
@@ -1845,7 +1851,7 @@ static fyfft_status build_grouped(fyfft_plan* p, const std::map<size_t, std::vec
       }
     } else {
       if (g.col_ppt > kLo || g.row_ppt > kLo) return FYFFT_UNSUPPORTED;
-      const int C = 1 << g.col.log2c, R = g.row.rows_per ? g.row.rows_per : 1 << g.row.log2c;
+      const int C = g.col.nsub ? g.col.nsub : 1 << g.col.log2c, R = g.row.nsub ? g.row.nsub : 1 << g.row.log2c;
       const double conv = g.kind == kGkFsPlain ? 1.0 : 2.0;
       const int kc = lp.col_cls, kr = lp.row_cls;
       const int famA = g.kind == kGkFsPlain ? kGfColPlain : (g.kind == kGkFsRader ? kGfColRaderA : kGfColBlueA);

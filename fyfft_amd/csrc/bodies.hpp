@@ -394,12 +394,12 @@ __device__ __forceinline__ V big_twiddle(const FsArgs<V>& a, long long m) {
   return DIR < 0 ? w : V{w.x, -w.y};
 }
 
-// Column pass over columns [c0, c0 + C) of one transform (C = 1 << log2c).
-// x: the transform's input, tmp: its workspace, out: its output, x0: its
-// Rader x[0] slot.  Thread t moves column c = t & (C - 1) of the N1 x C
-// tile, rows n1 = (t >> log2c) + j*s with s = blockDim >> log2c (blockDim is
-// a multiple of C), so a wavefront reads and writes C-element row segments
-// and a thread's offsets advance by constant steps.  The twiddled modes
+// Column pass over columns [c0, c0 + C) of one transform (C = 1 << log2c,
+// or nsub).  x: the transform's input, tmp: its workspace, out: its output,
+// x0: its Rader x[0] slot.  Thread t < s*C moves column c = t mod C of the
+// N1 x C tile, rows n1 = t / C + j*s with s = blockDim / C, so a wavefront
+// reads and writes C-element row segments and a thread's offsets advance by
+// constant steps.  The twiddled modes
 // take w_N^(n2*k1) as w^(n2*k0) * w^(n2*s*j): the first factor once per
 // thread, the second from the workgroup's table of C x P roots in LDS
 // (filled before the FFT, a few roots per thread) -- one complex product
@@ -413,13 +413,14 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
   constexpr bool kTw = fs_col_twiddled(MODE);
   constexpr int TR = fs_col_tw_row(P);  // root table row length
   V* lds_all = as_lds(lds_in);
-  const int N1 = a.n1, N2 = a.n2, C = 1 << a.log2c, lc = a.log2c;
+  const int N1 = a.n1, N2 = a.n2, C = a.nsub ? a.nsub : 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
   const int ncols = min(C, N2 - c0);
-  const int c = threadIdx.x & (C - 1), kr0 = threadIdx.x >> lc, s = blockDim.x >> lc;
-  const bool cok = c < ncols;
+  const int s = blockDim.x / C, kw = threadIdx.x / C, c = threadIdx.x - kw * C;
+  const int kr0 = min(kw, s - 1);  // threads past s*C walk nothing
+  const bool cok = kw < s && c < ncols;
   const int n2 = c0 + (cok ? c : 0);  // the thread's column of the transform (clamped)
   const int gs = s * N2;              // offset step from row to row
   V* lcol = lds_all + c * stride;
@@ -429,7 +430,7 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const int n = g0 + j * gs;
-      if (k0 + j * s < N1) {
+      if (k0 + j * s < N1 && kw < s) {
         if constexpr (MODE == kFsPlainA) buf[j] = a.col_cached ? El::ld(x + n) : El::ldnt(x + n);
         else if constexpr (MODE == kFsRaderA) buf[j] = El::ld(x + a.perm[n]);
         else if constexpr (MODE == kFsBlueA) buf[j] = n < a.n_orig ? cmul(El::ld(x + n), a.chirp[n]) : V{0, 0};
@@ -527,7 +528,7 @@ __device__ __forceinline__ void stage_rows(const V* span, int rows, int N2, V* l
 }
 
 // Row pass over rows [r0, r0 + C) of one transform's workspace (C = 1 <<
-// log2c, or rows_per for convolution rows).  Rows whose
+// log2c, or nsub for convolution rows).  Rows whose
 // length is a whole number of 128-byte lines are read by each slot's first
 // pass straight from HBM (5-11 % faster than staging them); other rows start
 // mid-line, so a slot's loads touch two lines per segment and the pass read
@@ -537,7 +538,7 @@ template <typename V, typename F, int DIR, int MODE, typename E>
 __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, const V* x0,
                                             int r0, V* lds_in) {
   V* lds_all = as_lds(lds_in);
-  const int N1 = a.n1, N2 = a.n2, C = a.rows_per ? a.rows_per : 1 << a.log2c;
+  const int N1 = a.n1, N2 = a.n2, C = a.nsub ? a.nsub : 1 << a.log2c;
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
@@ -570,12 +571,13 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     else
       F::template run_io<DIR>(GlobalSrc<V>{(gbl_cptr<V>)src}, LdsSink<V>{(lds_ptr<V>)lds, ps}, lds, N2,
                               a.threads, tid, a.pd, tw, active, ps);
-    // thread t stores row c = t & (C - 1) at k2 = (t >> log2c) + j*s, so a
-    // wavefront writes C-element segments of the output
+    // thread t < s*C stores row c = t mod C at k2 = t / C + j*s (s =
+    // blockDim / C), so a wavefront writes C-element segments of the output
     // (all of a thread's LDS reads first, so they overlap; positions past
     // N2 read position N2 - 1 and store nothing)
     constexpr int P = F::kPpt;
-    const int c = threadIdx.x & (C - 1), k0 = threadIdx.x >> a.log2c, s = blockDim.x >> a.log2c;
+    const int s = blockDim.x / C, kw = threadIdx.x / C, c = threadIdx.x - kw * C, k0 = min(kw, s - 1);
+    const bool wok = kw < s && c < rows;
     const V* lrow = lds_all + c * stride;
     const int o0 = r0 + c + N1 * k0, os = N1 * s;
     V val[P];
@@ -587,11 +589,11 @@ __device__ __forceinline__ void body_fs_row(const FsArgs<V>& a, V* tmp, E* out, 
     if (a.out_cached) {
 #pragma unroll
       for (int j = 0; j < P; ++j)
-        if (k0 + j * s < N2 && c < rows) Elem<V, E>::st(out + (o0 + j * os), val[j]);
+        if (k0 + j * s < N2 && wok) Elem<V, E>::st(out + (o0 + j * os), val[j]);
     } else {
 #pragma unroll
       for (int j = 0; j < P; ++j)
-        if (k0 + j * s < N2 && c < rows) Elem<V, E>::stnt(out + (o0 + j * os), val[j]);
+        if (k0 + j * s < N2 && wok) Elem<V, E>::stnt(out + (o0 + j * os), val[j]);
     }
   } else {  // kFsConvB: one row per slot, read and written in place, contiguous
     // forward FFT straight from HBM, kernel multiply as the last pass's sink,

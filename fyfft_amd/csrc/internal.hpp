@@ -97,7 +97,7 @@ template <typename V> struct FsArgs {
   long long in_dist = 0, out_dist = 0, tmp_dist = 0;  // per transform, elements
   int n1 = 0, n2 = 0;          // column length, row length
   int log2c = 0;               // columns (rows) per workgroup = 1 << log2c
-  int rows_per = 0;            // convolution row pass: rows per workgroup when nonzero (any count)
+  int nsub = 0;                // grouped plans: columns (rows) per workgroup when nonzero, any count
   int threads = 0;             // threads per sub-transform
   int ps = 4, stride = 0;      // LDS pad shift and per-sub-transform stride
   dev::PassDesc pd{};          // passes of the sub-length this kernel runs

@@ -43,8 +43,10 @@ def test_tuned_records_and_reuses_wisdom(fy, cuda):
     fy.wisdom_forget()
 
 
+# N = 10007: the candidate lengths M = 20160 (four-step) and 20480 (LDS)
+# time within a few percent of each other, so either may win
 @pytest.mark.parametrize("n,batch,kind", [(100000, 6, "fourstep(tuned)"), (65537, 3, "rader-fourstep(tuned)"),
-                                          (10007, 8, "bluestein-lds(tuned)"), (100003, 3, "bluestein-fourstep(tuned)")])
+                                          (10007, 8, "bluestein-"), (100003, 3, "bluestein-fourstep(tuned)")])
 def test_tuned_structures_vs_oracle(fy, orc, cuda, n, batch, kind):
     """Tuned four-step splits, Rader convolution splits and Bluestein
     convolution lengths compute the same transform, and their choice is
@@ -56,7 +58,7 @@ def test_tuned_structures_vs_oracle(fy, orc, cuda, n, batch, kind):
     xd = torch.from_numpy(x).cuda()
     with fy.Plan(n, batch, fy.F32, fy.FORM_TUNED) as p:
         d = p.describe()
-        assert d.startswith(kind), d
+        assert d.startswith(kind) and "(tuned)" in d, d
         y = p.execute(xd, None, fy.FORWARD)
         z = p.execute(y, None, fy.INVERSE)
     torch.cuda.synchronize()
