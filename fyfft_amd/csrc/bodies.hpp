@@ -488,6 +488,23 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
         else aux[j] = n < a.n_orig ? a.chirp[n] : V{0, 0};
       }
     }
+    if constexpr (MODE == kFsRaderC) {
+      if (a.win_bits > 0) {
+        // destinations window by window: the workgroups of one transform
+        // (adjacent on one XCD) sweep the windows together, so a window's
+        // lines are completed in L2 before they are evicted instead of being
+        // written back piecemeal (8-byte scatters into a transform of up to
+        // a megabyte)
+        const int nw = ((N1 * N2) >> a.win_bits) + 1;
+        for (int w = 0; w < nw; ++w) {
+#pragma unroll
+          for (int j = 0; j < P; ++j)
+            if (k0 + j * s < N1 && cok && (dst[j] >> a.win_bits) == w)
+              El::st(out + dst[j], cscale(cadd(val[j], xz), a.scale));
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const int n = g0 + j * gs;

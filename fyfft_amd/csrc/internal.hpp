@@ -98,6 +98,7 @@ template <typename V> struct FsArgs {
   int n1 = 0, n2 = 0;          // column length, row length
   int log2c = 0;               // columns (rows) per workgroup = 1 << log2c
   int nsub = 0;                // grouped plans: columns (rows) per workgroup when nonzero, any count
+  int win_bits = 0;            // Rader scatter (kFsRaderC): destinations in windows of 2^win_bits, 0: one sweep
   int threads = 0;             // threads per sub-transform
   int ps = 4, stride = 0;      // LDS pad shift and per-sub-transform stride
   dev::PassDesc pd{};          // passes of the sub-length this kernel runs
