@@ -1235,6 +1235,7 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     // (measured faster than gathering it through chunked runs as well), the
     // output permutation through buckets
     const bool buckets = p->kind == kRaderFs && p->rb_q > 0;
+    a.win_bits = std::getenv("FYFFT_EXP_FUSED") ? 99 : 0;
     if (p->kind == kFourStep) { colA = kFsPlainA; rowB = kFsPlainB; }
     else if (p->kind == kRaderFs) {
       colA = kFsRaderA;
