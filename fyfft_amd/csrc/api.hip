@@ -320,10 +320,6 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int for
 // lengths that are not powers of two take the cost-model split (their
 // runtime-pass sub-transforms vary widely in speed with the factors).
 static bool fs_split(size_t n, size_t elem, bool conv, size_t* n1, size_t* n2) {
-  if (const char* e = std::getenv("FYFFT_EXP_CONV_N1"); conv && e) {
-    const size_t d = (size_t)std::atol(e);
-    if (d > 1 && n % d == 0) { *n1 = d; *n2 = n / d; return true; }
-  }
   return (conv && four_step_split_lds(n, elem, n1, n2)) || (!conv && four_step_split_model(n, kFactorCap, elem, n1, n2)) ||
          four_step_split(n, kFactorCap, n1, n2);
 }
@@ -1235,7 +1231,6 @@ static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int 
     // (measured faster than gathering it through chunked runs as well), the
     // output permutation through buckets
     const bool buckets = p->kind == kRaderFs && p->rb_q > 0;
-    a.win_bits = std::getenv("FYFFT_EXP_FUSED") ? 99 : 0;
     if (p->kind == kFourStep) { colA = kFsPlainA; rowB = kFsPlainB; }
     else if (p->kind == kRaderFs) {
       colA = kFsRaderA;
