@@ -132,6 +132,13 @@ struct fyfft_plan {
   void *twcol = nullptr, *twrow = nullptr, *tw_hi = nullptr, *tw_lo = nullptr;
   int lo_bits = 0;
   int fixed_col = 0, fixed_row = 0;  // compile-time sub-transform lengths (0 = generic)
+  // three-pass form (n3 > 0): n = n1 x n2 x n3, a second column pass over
+  // the n2 x n3 rows (geometry gmid, twiddles of the row length n2 * n3)
+  size_t n3 = 0;
+  LdsGeom gmid;
+  dev::PassDesc pdmid{};
+  void *twmid = nullptr, *tw2_hi = nullptr, *tw2_lo = nullptr;
+  int lo_bits2 = 0, fixed_mid = 0;
   size_t chunk = 0;     // transforms per workspace chunk
   void* tmp = nullptr;  // chunk * conv length elements
   void* x0buf = nullptr;
@@ -324,6 +331,33 @@ static bool fs_split(size_t n, size_t elem, bool conv, size_t* n1, size_t* n2) {
          four_step_split(n, kFactorCap, n1, n2);
 }
 
+// Three-pass split n = a x b x c of a smooth length, each factor at most
+// 512 (FP32) / 256 (FP64) points so that every pass holds 16 or more
+// sub-transforms per workgroup within half a CU's LDS (128-byte segments):
+// fewest passes, then the most balanced factors.
+static bool three_pass_split(size_t n, size_t elem, size_t* a, size_t* b, size_t* c) {
+  const size_t fmax = elem >= 16 ? 256 : 512;
+  auto passes = [](size_t m) { return (double)radices_for(m).size(); };
+  double best = 1e300;
+  for (size_t x = 2; x <= fmax; ++x) {
+    if (n % x || !is_smooth7(x)) continue;
+    for (size_t y = 2; y <= fmax; ++y) {
+      if ((n / x) % y || !is_smooth7(y)) continue;
+      const size_t z = n / x / y;
+      if (z < 2 || z > fmax) continue;
+      const double cost = passes(x) + passes(y) + passes(z) +
+                          0.1 * (std::fabs(std::log((double)x / (double)z)) + std::fabs(std::log((double)y / (double)z)));
+      if (cost < best) {
+        best = cost;
+        *a = x;
+        *b = y;
+        *c = z;
+      }
+    }
+  }
+  return best < 1e300;
+}
+
 static bool fs_possible(size_t n, size_t elem, size_t cap) {
   size_t n1, n2;
   LdsGeom a, b;
@@ -339,12 +373,13 @@ static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
       return FYFFT_UNSUPPORTED;
     p->n1 = f->n1;
     p->n2 = n / f->n1;
-  } else if (!fs_split(n, elem, conv, &p->n1, &p->n2)) {
-    return FYFFT_UNSUPPORTED;
   }
-  if (!fs_geometry(p->n1, elem, cap, &p->gcol, f ? f->ccol : 0) ||
-      !fs_geometry(p->n2, elem, cap, &p->grow, f ? f->crow : 0, !conv))
-    return FYFFT_UNSUPPORTED;
+  // two passes where the split and its tiles exist; the three-pass form
+  // below otherwise (and for two-pass tiles narrower than 8 sub-transforms)
+  const bool two = (f || fs_split(n, elem, conv, &p->n1, &p->n2)) &&
+                   fs_geometry(p->n1, elem, cap, &p->gcol, f ? f->ccol : 0) &&
+                   fs_geometry(p->n2, elem, cap, &p->grow, f ? f->crow : 0, !conv);
+  if (!two && (f || conv || p->grouped_sub)) return FYFFT_UNSUPPORTED;
   // a compile-time sub-transform when one matches the planner's pass list;
   // otherwise the runtime-pass sub-transforms of plain FP32 transforms take
   // composite radices (convolutions keep the base radices)
@@ -357,16 +392,41 @@ static fyfft_status plan_fourstep(fyfft_plan* p, size_t n, bool conv) {
     if (!*fixed && !conv && p->prec == kF32 && g.ppt == 16) rad = radices_composite(len);
     return make_pass_table(p, len, pd, tw, &rad);
   };
+  // very long lengths whose two-pass tiles hold fewer than 8 sub-transforms
+  // (segments under 64 bytes: N = 2^24 = 4096 x 4096 ran 2 / 2 per
+  // workgroup at 4.5 % of 8 TB/s) take three passes of short sub-transforms
+  size_t a3 = 0, b3 = 0, c3 = 0;
+  LdsGeom ga, gb, gc;
+  if (!conv && !f && !p->grouped_sub && (!two || p->gcol.per_block < 8 || p->grow.per_block < 8) &&
+      three_pass_split(n, elem, &a3, &b3, &c3) &&
+      fs_geometry(a3, elem, cap, &ga) && fs_geometry(b3, elem, cap, &gb) && fs_geometry(c3, elem, cap, &gc, 0, true)) {
+    p->n1 = a3;
+    p->n2 = b3;
+    p->n3 = c3;
+    p->gcol = ga;
+    p->gmid = gb;
+    p->grow = gc;
+  } else if (!two) {
+    return FYFFT_UNSUPPORTED;
+  }
   fyfft_status st = passes(p->n1, p->gcol, &p->fixed_col, &p->pdcol, &p->twcol);
-  if (!st) st = passes(p->n2, p->grow, &p->fixed_row, &p->pdrow, &p->twrow);
+  if (!st && p->n3) st = passes(p->n2, p->gmid, &p->fixed_mid, &p->pdmid, &p->twmid);
+  if (!st) st = passes(p->n3 ? p->n3 : p->n2, p->grow, &p->fixed_row, &p->pdrow, &p->twrow);
   if (st) return st;
-  int lg = 0;
-  while (((size_t)1 << lg) < n) ++lg;
-  p->lo_bits = (lg + 1) / 2;
-  const size_t lo = (size_t)1 << p->lo_bits, hi = (n >> p->lo_bits) + 1;
-  if ((st = p->alloc(lo * elem, &p->tw_lo)) || (st = p->alloc(hi * elem, &p->tw_hi))) return st;
-  HIP_TRY(launch_gen_roots(p->prec, p->tw_lo, (long long)lo, 1, (long long)n, nullptr));
-  HIP_TRY(launch_gen_roots(p->prec, p->tw_hi, (long long)hi, (long long)lo, (long long)n, nullptr));
+  // hi/lo root tables of length `len` (FP64, and beyond 2^24 points)
+  auto roots = [&](size_t len, int* lo_bits, void** t_lo, void** t_hi) {
+    int lg = 0;
+    while (((size_t)1 << lg) < len) ++lg;
+    *lo_bits = (lg + 1) / 2;
+    const size_t lo = (size_t)1 << *lo_bits, hi = (len >> *lo_bits) + 1;
+    fyfft_status e;
+    if ((e = p->alloc(lo * elem, t_lo)) || (e = p->alloc(hi * elem, t_hi))) return e;
+    HIP_TRY(launch_gen_roots(p->prec, *t_lo, (long long)lo, 1, (long long)len, nullptr));
+    HIP_TRY(launch_gen_roots(p->prec, *t_hi, (long long)hi, (long long)lo, (long long)len, nullptr));
+    return FYFFT_SUCCESS;
+  };
+  if ((st = roots(n, &p->lo_bits, &p->tw_lo, &p->tw_hi))) return st;
+  if (p->n3 && (st = roots(p->n2 * p->n3, &p->lo_bits2, &p->tw2_lo, &p->tw2_hi))) return st;
   return FYFFT_SUCCESS;
 }
 
@@ -1201,6 +1261,65 @@ static Workspace plan_workspace(const fyfft_plan* p) {
   return w;
 }
 
+// Three-pass form of a plain length n = A x B x C (n3 = C): pass 1 is the
+// four-step column pass over A (x -> out, twiddles of n), pass 2 treats
+// each of the A rows of length B*C as a B x C four-step and runs its column
+// pass over B (out -> workspace, twiddles of B*C), writing row kb of row k1
+// at (kb*A + k1)*C, and pass 3 runs the length-C rows of that workspace with
+// the transposed store, X[q + A*B*kc] for row q = kb*A + k1.
+template <typename V>
+static fyfft_status exec_fourstep3(const fyfft_plan* p, const V* in, V* out, int dir, double scale, size_t rows,
+                                   size_t dist, const Workspace& ws, hipStream_t s) {
+  const size_t n = p->n, A = p->n1, B = p->n2, C = p->n3, M = B * C, elem = elem_bytes(p->prec);
+  for (size_t b0 = 0; b0 < rows; b0 += ws.chunk) {
+    const size_t cb = std::min(ws.chunk, rows - b0);
+    FsArgs<V> a;
+    a.in = in + b0 * dist;
+    a.tmp = out + b0 * dist;
+    a.in_dist = a.tmp_dist = (long long)dist;
+    a.n1 = (int)A;
+    a.n2 = (int)M;
+    a.tw_hi = (const V*)p->tw_hi;
+    a.tw_lo = (const V*)p->tw_lo;
+    a.lo_bits = p->lo_bits;
+    a.inv_n = fs_inv_n(n);
+    fill_pass(a, p->gcol, A, p->pdcol, p->twcol);
+    a.col_cached = rows_misaligned(M, elem);
+    HIP_TRY(launch_fs_pass(p->prec, true, kFsPlainA, dir, &a, cb, p->gcol, p->fixed_col, s));
+    FsArgs<V> m;
+    m.in = out + b0 * dist;
+    m.tmp = (V*)ws.tmp;
+    m.group = (int)A;
+    m.in_dist = (long long)M;
+    m.in_gdist = (long long)dist;
+    m.tmp_dist = (long long)C;
+    m.tmp_gdist = (long long)n;
+    m.tmp_row = (int)(A * C);
+    m.n1 = (int)B;
+    m.n2 = (int)C;
+    m.tw_hi = (const V*)p->tw2_hi;
+    m.tw_lo = (const V*)p->tw2_lo;
+    m.lo_bits = p->lo_bits2;
+    m.inv_n = fs_inv_n(M);
+    fill_pass(m, p->gmid, B, p->pdmid, p->twmid);
+    m.col_cached = rows_misaligned(C, elem);
+    HIP_TRY(launch_fs_pass(p->prec, true, kFsPlainA, dir, &m, cb * A, p->gmid, p->fixed_mid, s));
+    FsArgs<V> r;
+    r.tmp = (V*)ws.tmp;
+    r.tmp_dist = (long long)n;
+    r.out = out + b0 * dist;
+    r.out_dist = (long long)dist;
+    r.n1 = (int)(A * B);
+    r.n2 = (int)C;
+    r.scale = (dev::S<V>)scale;
+    fill_pass(r, p->grow, C, p->pdrow, p->twrow);
+    r.stage_rows = rows_misaligned(C, elem);
+    r.out_cached = rows_misaligned(A * B, elem);
+    HIP_TRY(launch_fs_pass(p->prec, false, kFsPlainB, dir, &r, cb, p->grow, p->fixed_row, s));
+  }
+  return FYFFT_SUCCESS;
+}
+
 template <typename V>
 static fyfft_status exec_fourstep(const fyfft_plan* p, const V* in, V* out, int dir, bool inv_tables,
                                   double scale, size_t rows, size_t dist, const Workspace& ws, hipStream_t s) {
@@ -1486,14 +1605,20 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
       if (tuned && !p->force_fs) {
         st = tune_fourstep(p, n, kWfSmooth, &tuned_fs);
         if (!st) p->force_fs = &tuned_fs;
+        else if (st == FYFFT_UNSUPPORTED) st = FYFFT_SUCCESS;  // no two-pass candidate: the three-pass form
       }
       if (!st) st = plan_fourstep(p, n, false);
       if (!st) st = alloc_workspace(p, n, false);
       if (!st)
-        p->desc = std::string(tuned ? "fourstep(tuned)" : "fourstep") + " N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
-                  radix_string(p->pdcol) + ") x " + std::to_string(p->n2) + "(" +
-                  radix_string(p->pdrow) + ") C=" + std::to_string(p->gcol.per_block) + "/" +
-                  std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk);
+        p->desc = p->n3 ? "fourstep N=" + std::to_string(n) + " = " + std::to_string(p->n1) + "(" +
+                              radix_string(p->pdcol) + ") x " + std::to_string(p->n2) + "(" + radix_string(p->pdmid) +
+                              ") x " + std::to_string(p->n3) + "(" + radix_string(p->pdrow) + ") three-pass C=" +
+                              std::to_string(p->gcol.per_block) + "/" + std::to_string(p->gmid.per_block) + "/" +
+                              std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk)
+                        : std::string(tuned ? "fourstep(tuned)" : "fourstep") + " N=" + std::to_string(n) + " = " +
+                              std::to_string(p->n1) + "(" + radix_string(p->pdcol) + ") x " + std::to_string(p->n2) +
+                              "(" + radix_string(p->pdrow) + ") C=" + std::to_string(p->gcol.per_block) + "/" +
+                              std::to_string(p->grow.per_block) + " chunk=" + std::to_string(p->chunk);
     }
   } else if (p->strategy == kRader) {
     st = plan_rader(p);
@@ -2103,6 +2228,11 @@ static fyfft_status execute_rows(const fyfft_plan* p, const void* in, void* out,
     case kBlueFs: {
       const int d = dir == kForward ? -1 : +1;
       const Workspace ws = wsp ? *wsp : plan_workspace(p);
+      if (p->kind == kFourStep && p->n3) {
+        if (p->prec == kF64)
+          return exec_fourstep3<double2>(p, (const double2*)in, (double2*)out, d, scale, rows, dist, ws, s);
+        return exec_fourstep3<float2>(p, (const float2*)in, (float2*)out, d, scale, rows, dist, ws, s);
+      }
       if (p->prec == kF64)
         return exec_fourstep<double2>(p, (const double2*)in, (double2*)out, d, inv_tables, scale, rows, dist, ws, s);
       return exec_fourstep<float2>(p, (const float2*)in, (float2*)out, d, inv_tables, scale, rows, dist, ws, s);

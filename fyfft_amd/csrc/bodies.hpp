@@ -481,7 +481,8 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
   } else {
     // every element's LDS value (and table entry) read before the first
     // store, so the reads overlap; rows past N1 read row N1 - 1 and store
-    // nothing
+    // nothing.  The twiddled modes store row k1 of the workspace at k1 *
+    // tmp_row (a three-pass plan's middle pass writes its rows transposed).
     V val[P];
     [[maybe_unused]] V aux[P];
     [[maybe_unused]] int dst[P];
@@ -516,11 +517,13 @@ __device__ __forceinline__ void body_fs_col(const FsArgs<V>& a, const E* x, V* t
         return;
       }
     }
+    const int trow = a.tmp_row ? a.tmp_row : N2;
+    const int t0 = opaque(kr0 * trow + n2), ts = s * trow;
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       const int n = g0 + j * gs;
       if (k0 + j * s < N1 && cok) {
-        if constexpr (kTw) tmp[n] = cmul(val[j], cmul(wb, aux[j]));
+        if constexpr (kTw) tmp[t0 + j * ts] = cmul(val[j], cmul(wb, aux[j]));
         else if constexpr (MODE == kFsRaderC) El::st(out + dst[j], cscale(cadd(val[j], xz), a.scale));
         else if (n < a.n_orig) El::st(out + n, cscale(cmul(val[j], aux[j]), a.scale));
       }

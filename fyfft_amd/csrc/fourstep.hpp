@@ -13,7 +13,8 @@ __global__ __launch_bounds__(1024) void k_fs_col(FsArgs<V> a) {
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const long long t = lb / groups;
   const int c0 = (int)(lb - t * groups) << a.log2c;
-  body_fs_col<V, F, DIR, MODE>(a, a.in + t * a.in_dist, a.tmp + t * a.tmp_dist,
+  const long long tg = a.group ? t / a.group : 0, ti = a.group ? t - tg * a.group : t;
+  body_fs_col<V, F, DIR, MODE>(a, a.in + tg * a.in_gdist + ti * a.in_dist, a.tmp + tg * a.tmp_gdist + ti * a.tmp_dist,
                                a.out + t * a.out_dist, a.x0buf ? a.x0buf + t : nullptr, c0,
                                reinterpret_cast<V*>(smem_raw), a.zw ? a.zw + t * a.zw_dist : nullptr);
 }

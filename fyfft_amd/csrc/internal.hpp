@@ -98,6 +98,11 @@ template <typename V> struct FsArgs {
   int n1 = 0, n2 = 0;          // column length, row length
   int log2c = 0;               // columns (rows) per workgroup = 1 << log2c
   int nsub = 0;                // grouped plans: columns (rows) per workgroup when nonzero, any count
+  int tmp_row = 0;             // twiddled column pass: workspace row stride when nonzero (else n2)
+  // column pass: transform t at in + (t / group) * in_gdist + (t % group) *
+  // in_dist when group > 0 (likewise tmp), else in + t * in_dist
+  int group = 0;
+  long long in_gdist = 0, tmp_gdist = 0;
   int win_bits = 0;            // Rader scatter (kFsRaderC): destinations in windows of 2^win_bits, 0: one sweep
   int threads = 0;             // threads per sub-transform
   int ps = 4, stride = 0;      // LDS pad shift and per-sub-transform stride

@@ -93,6 +93,18 @@ def test_fourstep_composite_radices(fy, orc, cuda, n, split, radices):
     check(fy, orc, unit_phase(np.random.default_rng(n + 1), 3, n, "f32"), "fourstep", inplace=True)
 
 
+# Lengths whose two-pass tiles would hold fewer than 8 sub-transforms run
+# three passes of short sub-transforms (n = a x b x c)
+@pytest.mark.parametrize("n,batch,prec", [(1 << 23, 2, "f32"), (1 << 24, 1, "f32"), (3 ** 15, 1, "f32"),
+                                          (1 << 21, 2, "f64"), (1 << 22, 1, "f64")])
+def test_fourstep_three_pass(fy, orc, cuda, n, batch, prec):
+    with fy.Plan(n, batch, fy.F32 if prec == "f32" else fy.F64) as p:
+        assert "three-pass" in p.describe(), p.describe()
+    x = gauss(np.random.default_rng(n), batch, n, prec)
+    check(fy, orc, x, "fourstep")
+    check(fy, orc, x[:1], "fourstep", inplace=True)
+
+
 def test_fourstep_inplace(fy, orc, cuda):
     check(fy, orc, gauss(np.random.default_rng(1), 2, 65536, "f32"), "fourstep", inplace=True)
 
