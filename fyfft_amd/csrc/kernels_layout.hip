@@ -8,6 +8,11 @@ using namespace dev;
 
 // Columns [c0, c0 + C) of one plane: C*n elements read as n rows of C
 // adjacent values, C column transforms in LDS, written back the same way.
+// Thread t moves column c = t & (C - 1), rows t >> log2c + j*s (s =
+// blockDim >> log2c), with offsets advancing by constant steps; its LDS
+// values are all read before its first store.  The twiddle w_N^(b*k) of
+// row k in inner plane b (uniform across the workgroup) is w^(b*k0) *
+// T[j], T[j] = w^(b*s*j) a table of P roots in LDS after the tile.
 template <typename V, typename F, int DIR>
 __device__ __forceinline__ void body_columns(const ColArgs<V>& a, const V* x, V* y, int c0, V* lds_in, int b) {
   constexpr int P = F::kPpt;
@@ -16,31 +21,49 @@ __device__ __forceinline__ void body_columns(const ColArgs<V>& a, const V* x, V*
   const int slot = threadIdx.x / a.threads;
   const int tid = threadIdx.x - slot * a.threads;
   const int ps = a.ps, stride = a.stride;
-  const int total = N << lc;
   const int ncols = min(C, a.ncols - c0);
-  staged<P>(threadIdx.x, blockDim.x, total,
-            [&](int i) {
-              const int j = i >> lc, c = i & (C - 1);
-              return gload(x + (long long)j * a.istride + c0 + (c < ncols ? c : 0));
-            },
-            [&](int i, V v) {
-              const int j = i >> lc, c = i & (C - 1);
-              if (c < ncols) lds_all[c * stride + j + (j >> ps)] = v;
-            });
+  const int c = threadIdx.x & (C - 1), kr0 = threadIdx.x >> lc, s = blockDim.x >> lc;
+  const bool cok = c < ncols;
+  V* lcol = lds_all + c * stride;
+  {
+    const int k0 = kr0;
+    const V* src = x + (long long)k0 * a.istride + c0 + (cok ? c : 0);
+    const long long step = (long long)s * a.istride;
+    V buf[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+      if (k0 + j * s < N) buf[j] = gload(src + j * step);
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const int k = k0 + j * s;
+      if (k < N && cok) lcol[k + (k >> ps)] = buf[j];
+    }
+  }
+  const bool twiddle = a.tw_hi != nullptr;
+  const int lo_mask = (1 << a.lo_bits) - 1;
+  auto root = [&](long long m) { return cmul(a.tw_hi[m >> a.lo_bits], a.tw_lo[m & lo_mask]); };
+  V* ttab = lds_all + C * stride;
+  if (twiddle)
+    for (int j = threadIdx.x; j < P; j += blockDim.x) ttab[j] = j * s < N ? root((long long)b * s * j) : V{1, 0};
   __syncthreads();
   const bool active = slot < ncols;
   F::template run<DIR>(lds_all + (slot < C ? slot : 0) * stride, N, a.threads, tid, a.pd, a.tw, active, ps);
-  const bool twiddle = a.tw_hi != nullptr;
-  const int lo_mask = (1 << a.lo_bits) - 1;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    const int k = i >> lc, c = i & (C - 1);
-    if (c < ncols) {
-      V v = lds_all[c * stride + k + (k >> ps)];
-      if (twiddle) {
-        const int m = b * k;
-        v = twmul<DIR>(v, cmul(a.tw_hi[m >> a.lo_bits], a.tw_lo[m & lo_mask]));
-      }
-      gstore(y + (long long)k * a.ostride + c0 + c, cscale(v, a.scale));
+  const int k0 = opaque(kr0);
+  V val[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const int k = min(k0 + j * s, N - 1);
+    val[j] = lcol[k + (k >> ps)];
+  }
+  const V wb = twiddle ? root((long long)b * min(k0, N - 1)) : V{1, 0};
+  V* dst = y + (long long)k0 * a.ostride + c0 + c;
+  const long long step = (long long)s * a.ostride;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    if (k0 + j * s < N && cok) {
+      V v = val[j];
+      if (twiddle) v = twmul<DIR>(v, cmul(wb, ttab[j]));
+      gstore(dst + j * step, cscale(v, a.scale));
     }
   }
 }
@@ -91,14 +114,16 @@ hipError_t launch_columns(int prec, int dir, const void* args, size_t planes, co
                           hipStream_t s) {
   if (planes == 0) return hipSuccess;
   const int threads = g.threads * g.per_block;
+  // the tile, then the table of g.ppt twiddle roots (body_columns)
+  const size_t lds = g.lds_bytes + (size_t)g.ppt * (prec == kF64 ? 16 : 8);
   if (prec == kF64) {
     const auto& a = *static_cast<const ColArgs<double2>*>(args);
-    return g.ppt <= 8 ? columns_len<double2, 8>(fixed_n, dir, a, planes, g.lds_bytes, threads, s)
-                      : columns_len<double2, 16>(fixed_n, dir, a, planes, g.lds_bytes, threads, s);
+    return g.ppt <= 8 ? columns_len<double2, 8>(fixed_n, dir, a, planes, lds, threads, s)
+                      : columns_len<double2, 16>(fixed_n, dir, a, planes, lds, threads, s);
   }
   const auto& a = *static_cast<const ColArgs<float2>*>(args);
-  return g.ppt <= 16 ? columns_len<float2, 16>(fixed_n, dir, a, planes, g.lds_bytes, threads, s)
-                     : columns_len<float2, 32>(fixed_n, dir, a, planes, g.lds_bytes, threads, s);
+  return g.ppt <= 16 ? columns_len<float2, 16>(fixed_n, dir, a, planes, lds, threads, s)
+                     : columns_len<float2, 32>(fixed_n, dir, a, planes, lds, threads, s);
 }
 
 // ------------------------------------------------------------ gather / scatter
