@@ -1482,6 +1482,13 @@ static fyfft_status exec_spectral_half(const fyfft_plan* p, bool inverse, const 
   a.pd = p->half.pd;
   a.tw = (const V*)p->half.tw;
   a.rtw = (const V*)p->half_rtw;
+  if (inverse && k > 0) {
+    // the modes array's extent: under 2 GiB the inverse reads it through a
+    // buffer resource whose out-of-range reads return zero
+    const unsigned long long span =
+        ((unsigned long long)(a.rows - 1) * a.rdist + (unsigned long long)(k - 1) * a.kdist + 1) * sizeof(V);
+    a.mode_bytes = span < (1ull << 31) && scale != 0 ? (unsigned)span : 0u;
+  }
   a.a0 = (dev::S<V>)scale0;
   a.a = (dev::S<V>)scale;
   HIP_TRY(launch_spectral_half(p->prec, inverse, &a, g, p->half.fixed >= 0 ? (int)h : 0, s));

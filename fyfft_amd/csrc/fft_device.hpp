@@ -664,6 +664,16 @@ template <typename V> struct LdsSink {
   __device__ void operator()(int i, V v) const { ((V*)p)[i + (i >> ps)] = v; }
 };
 
+// A source with kSplit = true takes the index of a first-pass point as
+// (j, u): j the butterfly (per thread) and u = r * (N / R), uniform across
+// the workgroup and a compile-time constant in the fixed pass lists, and
+// works in two phases: load(j, u) for every point of the butterfly (its
+// loads all in flight), then finish(raw, j, u).
+template <typename Src, typename = void> struct SplitSrc { static constexpr bool value = false; };
+template <typename Src> struct SplitSrc<Src, decltype((void)Src::kSplit)> {
+  static constexpr bool value = Src::kSplit;
+};
+
 template <typename V, int R, int MAXB, int DIR, typename Src, typename Dst>
 __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, int tid,
                                         const V* __restrict__ tw, bool active) {
@@ -682,8 +692,16 @@ __device__ __forceinline__ void io_pass(Src src, Dst dst, int N, int Ls, int T, 
     for (int b = 0; b < MAXB; ++b) {
       const int j = tid + b * T;
       if (active && j < nb) {
+        if constexpr (SplitSrc<Src>::value) {
+          typename Src::Raw raw[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = src(j + r * nb);
+          for (int r = 0; r < R; ++r) raw[r] = src.load(j, r * nb);
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[b][r] = src.finish(raw[r], j, r * nb);
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[b][r] = src(j + r * nb);
+        }
         if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
       }

@@ -212,6 +212,7 @@ template <typename V> struct SpecArgs {
   const V* tw = nullptr;
   dev::S<V> a0 = 1, a = 1;             // scale of mode 0 and of modes 1..k-1
   const V* rtw = nullptr;              // half-length forms: exp(-2 pi i k / n), k <= n/2
+  unsigned mode_bytes = 0;             // inverse: bytes the modes array spans (under 2 GiB, a != 0), else 0
 };
 hipError_t launch_spectral(int prec, bool inverse, const void* args, const LdsGeom& g, hipStream_t s);
 // Half-length forms (even n, k <= n/2 + 1): pd/tw/g describe the length n/2;

@@ -101,6 +101,88 @@ template <typename V, int TW> struct ModesHermSrc {
   }
 };
 
+// Loads through a buffer resource (raw, stride 0): an offset at or past
+// the resource's size returns zero and makes no memory access.  The LLVM
+// intrinsics are declared directly: this compiler's
+// __builtin_amdgcn_raw_buffer_load_b64 is emitted as a 32-bit load.
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+__device__ f32x2_t raw_buffer_load_f32x2(i32x4_t rsrc, int voff, int soff, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.v2f32");
+__device__ f64x2_t raw_buffer_load_f64x2(i32x4_t rsrc, int voff, int soff, int aux)
+    __asm("llvm.amdgcn.raw.buffer.load.v2f64");
+constexpr unsigned kOob = 0x80000000u;  // past every resource built here (< 2 GiB)
+// Resource over `bytes` bytes from p (uniform); word 3 is the gfx9 default
+// for untyped dword accesses.
+__device__ __forceinline__ i32x4_t buffer_rsrc(const void* p, unsigned bytes) {
+  const unsigned long long a = (unsigned long long)p;
+  return i32x4_t{(int)(unsigned)a, (int)((unsigned)(a >> 32) & 0xffffu), (int)bytes, 0x00020000};
+}
+template <typename V> __device__ __forceinline__ V bload(i32x4_t r, unsigned off) {
+  if constexpr (sizeof(V) == 8) {
+    const f32x2_t u = raw_buffer_load_f32x2(r, (int)off, 0, 0);
+    return V{u.x, u.y};
+  } else {
+    const f64x2_t u = raw_buffer_load_f64x2(r, (int)off, 0, 0);
+    return V{u.x, u.y};
+  }
+}
+
+// A per-thread value the compiler may not re-associate with the uniform
+// offsets added to it (it would otherwise fold base + (h - j)*s - u*s into
+// one multiply per point); identical uses still merge.
+__device__ __forceinline__ unsigned keep(unsigned v) { asm("" : "+v"(v)); return v; }
+
+// The same source for the first pass when the modes array spans less than
+// 2 GiB, in the two phases of a split source (j per thread, u uniform):
+// load() reads the kept modes k = j + u and h - k through a buffer resource
+// over the array (a lane whose mode is at or past K gets the out-of-range
+// offset and reads zero), each load only where some lane of the wave needs
+// it and with no use behind its branch, so a thread's loads are all in
+// flight before finish() combines them; w^(j+u) = w^j w^u with w^u a scalar
+// load.  The values are those of ModesHermSrc divided by q = a/4 (the sink
+// scales by 2q): c0 = 2 a0 / a for mode 0 and 2 for mode h, whose
+// imaginary parts are dropped.
+template <typename V> struct ModesHermBuf {
+  static constexpr bool kFromLds = false, kSplit = true;
+  typedef S<V> e2 __attribute__((ext_vector_type(2)));
+  struct Raw { V mk, mm, t; };
+  i32x4_t res;                                    // the modes array
+  gbl_cptr<V> w;                                  // exp(-2 pi i k / L), k <= h
+  const __attribute__((address_space(4))) e2* wu; // the same table, scalar loads
+  unsigned base, kstep;                           // bytes: mode 0 of the row, between modes
+  int K, h;
+  S<V> c0;
+  __device__ Raw load(int j, int u) const {
+    const unsigned kb = keep(base + (unsigned)j * kstep), mb = keep(base + (unsigned)(h - j) * kstep);
+    const unsigned us = (unsigned)u * kstep;
+    const bool ink = j < K - u, inm = j > h - K - u;
+    Raw r{V{0, 0}, V{0, 0}, ((const V*)w)[j]};  // w^j: the same load for every u
+    if (__builtin_amdgcn_ballot_w64(ink)) r.mk = bload<V>(res, ink ? kb + us : kOob);
+    if (__builtin_amdgcn_ballot_w64(inm)) r.mm = bload<V>(res, inm ? mb - us : kOob);
+    return r;
+  }
+  __device__ V finish(Raw r, int j, int u) const {
+    if (!__builtin_amdgcn_ballot_w64(j < K - u || j > h - K - u)) return V{0, 0};
+    V mk = r.mk, mm = r.mm;
+    if (u == 0) {
+      const bool z = j == 0;
+      mk = z ? V{mk.x * c0, 0} : mk;
+      mm = z ? V{mm.x * 2, 0} : mm;
+    }
+    V t = r.t;
+    if (u != 0) {
+      const e2 s = wu[u];
+      t = cmul(t, V{s.x, s.y});
+    }
+    const V e{mk.x + mm.x, mk.y - mm.y}, d{mk.x - mm.x, mk.y + mm.y};
+    const V o = cmulc(d, t);
+    return V{e.x - o.y, e.y + o.x};
+  }
+  __device__ V operator()(int i) const { return finish(load(i, 0), i, 0); }
+};
+
 // One workgroup slot per row.  TPB > 0: F is a compile-time policy (FixedIo)
 // with TPB rows per workgroup: geometry and LDS (static) are compile-time and
 // for TPB == 1 the row guard is too; w^k is then formed as a product of a
@@ -150,7 +232,10 @@ __global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_rfft_mod
   }
 }
 
-template <typename V, typename F, int TPB>
+// BUF: the modes come through ModesHermBuf (a.mode_bytes > 0: the array
+// spans less than 2 GiB and the scale a is not zero), else through
+// ModesHermSrc.
+template <typename V, typename F, int TPB, bool BUF>
 __global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_irfft_modes_half(SpecArgs<V> a) {
   constexpr bool kFixed = TPB > 0;
   constexpr int kStride = kFixed ? lds_len(F::kN, F::kPs) : 1;
@@ -166,14 +251,23 @@ __global__ __launch_bounds__(TPB > 0 ? F::kThreads * TPB : 1024) void k_irfft_mo
   V* y = reinterpret_cast<V*>(a.real_out) + r * h;
   constexpr int kTw = (kFixed && (F::kThreads & (F::kThreads - 1)) == 0 && F::kThreads + 16 <= F::kN)
                           ? F::kThreads : 0;
-  F::template run<+1>(ModesHermSrc<V, kTw>{(gbl_cptr<V>)(a.cplx_in + r * a.rdist), (gbl_cptr<V>)a.rtw, a.kdist,
-                                           a.k, h, a.a0, a.a},
-                      GlobalSink<V>{(gbl_ptr<V>)y, (S<V>)2}, lds, h, T, tid, a.pd, a.tw, active, ps);
+  if constexpr (BUF) {
+    using Src = ModesHermBuf<V>;
+    const Src src{buffer_rsrc(a.cplx_in, a.mode_bytes), (gbl_cptr<V>)a.rtw, (decltype(Src::wu))a.rtw,
+                  (unsigned)(r * a.rdist * (long long)sizeof(V)), (unsigned)(a.kdist * (long long)sizeof(V)),
+                  a.k, h, 2 * a.a0 / a.a};
+    F::template run<+1>(src, GlobalSink<V>{(gbl_ptr<V>)y, a.a * (S<V>)0.5}, lds, h, T, tid, a.pd, a.tw, active, ps);
+  } else {
+    F::template run<+1>(ModesHermSrc<V, kTw>{(gbl_cptr<V>)(a.cplx_in + r * a.rdist), (gbl_cptr<V>)a.rtw, a.kdist,
+                                             a.k, h, a.a0, a.a},
+                        GlobalSink<V>{(gbl_ptr<V>)y, (S<V>)2}, lds, h, T, tid, a.pd, a.tw, active, ps);
+  }
 }
 
 template <typename V, typename F, int TPB>
 static hipError_t go_half(bool inverse, const SpecArgs<V>& a, const LdsGeom& g, hipStream_t s) {
-  auto k = inverse ? k_irfft_modes_half<V, F, TPB> : k_rfft_modes_half<V, F, TPB>;
+  auto k = !inverse ? k_rfft_modes_half<V, F, TPB>
+           : a.mode_bytes ? k_irfft_modes_half<V, F, TPB, true> : k_irfft_modes_half<V, F, TPB, false>;
   size_t lds = (size_t)a.stride * g.per_block * sizeof(V);
   if constexpr (TPB > 0) {
     // compile-time geometry: the plan's must be the same, LDS is static
