@@ -512,12 +512,14 @@ template <typename V, int PPT> constexpr bool kNo16 = (sizeof(V) == 8 && PPT <= 
 // ---------------------------------------------------------------- pass
 // One Stockham pass of radix R over the transform of length N held by T
 // threads (this thread is `tid`).  SRC_G / DST_G select global (HBM) or LDS
-// addressing (LDS index i is stored at i + (i >> ps)).  Each thread owns at most MAXB
-// butterflies.  When source and destination are both LDS the pass is in
+// addressing (LDS index i is stored at i + (i >> ps); LIN: the transform is
+// stored unpadded, ps >= 30, and the index is used as is, so the addresses
+// of a butterfly's points are one base plus uniform steps).  Each thread
+// owns at most MAXB butterflies.  When source and destination are both LDS the pass is in
 // place: every read completes (barrier) before any write.  A barrier
 // follows LDS writes so the next pass sees them.  `scale` is applied only
 // on global stores.
-template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G>
+template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G, bool LIN = false>
 __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int Ls, int T, int tid,
                                               const V* __restrict__ tw, bool active, S<V> scale,
                                               int ps) {
@@ -541,7 +543,7 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int idx = j + r * nb;
-          v[b][r] = SRC_G ? gload(src + idx) : src[idx + (idx >> ps)];
+          v[b][r] = SRC_G ? gload(src + idx) : src[LIN ? idx : idx + (idx >> ps)];
         }
         if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
@@ -564,7 +566,7 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
         for (int r = 0; r < R; ++r) {
           const int idx = base + r * Ls;
           if (DST_G) gstore(dst + idx, cscale(v[b][r], scale));
-          else dst[idx + (idx >> ps)] = v[b][r];
+          else dst[LIN ? idx : idx + (idx >> ps)] = v[b][r];
         }
       }
       if (Ls > 1) {
@@ -615,11 +617,11 @@ template <bool NO16 = false, bool CMP = false, typename F> __device__ __forceinl
 // One LDS -> LDS pass kept out of line: inlining every radix case into the
 // pass loop of lds_fft makes the register allocator keep all cases' arrays
 // live at once and spill; as separate functions each gets its own budget.
-template <typename V, int R, int MAXB, int DIR, bool TL>
+template <typename V, int R, int MAXB, int DIR, bool TL, bool LIN>
 __device__ __noinline__ void lds_pass_ool(lds_ptr<V> lds_s, int N, int Ls, int T, int tid,
                                           tw_ptr<TL, V> tw_g, bool active, int ps) {
   V* lds = (V*)lds_s;
-  stockham_pass<V, R, MAXB, DIR, false, false>(lds, lds, N, Ls, T, tid, (const V*)tw_g, active, 1, ps);
+  stockham_pass<V, R, MAXB, DIR, false, false, LIN>(lds, lds, N, Ls, T, tid, (const V*)tw_g, active, 1, ps);
 }
 
 // Run every pass of `pd` on an LDS-resident transform (LDS -> LDS).
@@ -636,7 +638,8 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
     const int R = radix_at(code, p);
     radix_switch<kNo16<V, PPT>, CMP>(R, [&](auto rc) {
       constexpr int RR = decltype(rc)::value;
-      lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL>(lds, N, Ls, T, tid, tw, active, ps);
+      if (ps >= 30) lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, true>(lds, N, Ls, T, tid, tw, active, ps);
+      else lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, false>(lds, N, Ls, T, tid, tw, active, ps);
     });
     Ls *= R;
   }
