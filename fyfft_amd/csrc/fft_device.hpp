@@ -512,14 +512,17 @@ template <typename V, int PPT> constexpr bool kNo16 = (sizeof(V) == 8 && PPT <= 
 // ---------------------------------------------------------------- pass
 // One Stockham pass of radix R over the transform of length N held by T
 // threads (this thread is `tid`).  SRC_G / DST_G select global (HBM) or LDS
-// addressing (LDS index i is stored at i + (i >> ps); LIN: the transform is
-// stored unpadded, ps >= 30, and the index is used as is, so the addresses
-// of a butterfly's points are one base plus uniform steps).  Each thread
-// owns at most MAXB butterflies.  When source and destination are both LDS the pass is in
+// addressing (LDS index i is stored at i + (i >> ps)).  AM, the LDS address
+// form: 0 the padded index per point; 1 the transform is stored unpadded
+// (ps >= 30) and the index is used as is; 2 padded, where the pass's strides
+// keep a butterfly's points at one padded base plus uniform steps (N / R a
+// multiple of 2^ps for the reads; Ls a multiple of 2^ps, or R * Ls dividing
+// 2^ps, for the writes: fast_pad_pass).  Forms 1 and 2 address each point
+// with one add.  Each thread owns at most MAXB butterflies.  When source and destination are both LDS the pass is in
 // place: every read completes (barrier) before any write.  A barrier
 // follows LDS writes so the next pass sees them.  `scale` is applied only
 // on global stores.
-template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G, bool LIN = false>
+template <typename V, int R, int MAXB, int DIR, bool SRC_G, bool DST_G, int AM = 0>
 __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int Ls, int T, int tid,
                                               const V* __restrict__ tw, bool active, S<V> scale,
                                               int ps) {
@@ -543,7 +546,8 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           const int idx = j + r * nb;
-          v[b][r] = SRC_G ? gload(src + idx) : src[LIN ? idx : idx + (idx >> ps)];
+          v[b][r] = SRC_G ? gload(src + idx)
+                    : src[AM == 1 ? idx : AM == 2 ? j + (j >> ps) + r * (nb + (nb >> ps)) : idx + (idx >> ps)];
         }
         if (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + k * (R - 1));
         Dft<R, DIR, V>::run(v[b]);
@@ -566,7 +570,8 @@ __device__ __forceinline__ void stockham_pass(const V* src, V* dst, int N, int L
         for (int r = 0; r < R; ++r) {
           const int idx = base + r * Ls;
           if (DST_G) gstore(dst + idx, cscale(v[b][r], scale));
-          else dst[LIN ? idx : idx + (idx >> ps)] = v[b][r];
+          else if (AM == 2) dst[base + (base >> ps) + r * (Ls + (((Ls >> ps) << ps) == Ls ? Ls >> ps : 0))] = v[b][r];
+          else dst[AM == 1 ? idx : idx + (idx >> ps)] = v[b][r];
         }
       }
       if (Ls > 1) {
@@ -617,11 +622,18 @@ template <bool NO16 = false, bool CMP = false, typename F> __device__ __forceinl
 // One LDS -> LDS pass kept out of line: inlining every radix case into the
 // pass loop of lds_fft makes the register allocator keep all cases' arrays
 // live at once and spill; as separate functions each gets its own budget.
-template <typename V, int R, int MAXB, int DIR, bool TL, bool LIN>
+template <typename V, int R, int MAXB, int DIR, bool TL, int AM>
 __device__ __noinline__ void lds_pass_ool(lds_ptr<V> lds_s, int N, int Ls, int T, int tid,
                                           tw_ptr<TL, V> tw_g, bool active, int ps) {
   V* lds = (V*)lds_s;
-  stockham_pass<V, R, MAXB, DIR, false, false, LIN>(lds, lds, N, Ls, T, tid, (const V*)tw_g, active, 1, ps);
+  stockham_pass<V, R, MAXB, DIR, false, false, AM>(lds, lds, N, Ls, T, tid, (const V*)tw_g, active, 1, ps);
+}
+
+// Whether a padded radix-R pass at stride Ls over N points keeps each
+// butterfly's padded addresses linear (stockham_pass address form 2).
+__device__ __forceinline__ bool fast_pad_pass(int N, int Ls, int R, int ps) {
+  const int m = (1 << ps) - 1;
+  return ((N / R) & m) == 0 && ((Ls & m) == 0 || ((m + 1) % (Ls * R)) == 0);
 }
 
 // Run every pass of `pd` on an LDS-resident transform (LDS -> LDS).
@@ -638,8 +650,9 @@ __device__ __forceinline__ void lds_fft(V* lds_gen, int N, int T, int tid, const
     const int R = radix_at(code, p);
     radix_switch<kNo16<V, PPT>, CMP>(R, [&](auto rc) {
       constexpr int RR = decltype(rc)::value;
-      if (ps >= 30) lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, true>(lds, N, Ls, T, tid, tw, active, ps);
-      else lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, false>(lds, N, Ls, T, tid, tw, active, ps);
+      if (ps >= 30) lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, 1>(lds, N, Ls, T, tid, tw, active, ps);
+      else if (fast_pad_pass(N, Ls, RR, ps)) lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, 2>(lds, N, Ls, T, tid, tw, active, ps);
+      else lds_pass_ool<V, RR, cdiv(PPT, RR), DIR, TL, 0>(lds, N, Ls, T, tid, tw, active, ps);
     });
     Ls *= R;
   }
