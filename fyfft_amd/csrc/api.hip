@@ -284,14 +284,6 @@ static float fs_inv_n(size_t n) { return n <= ((size_t)1 << 24) ? (float)(1.0 / 
 static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int force_c = 0,
                         bool transposed_rows = false) {
   if (!lds_geometry(len, elem, cap, g)) return false;
-  // sub-lengths that are not multiples of 16 have no pass whose strides
-  // fall on one bank: their tiles are stored unpadded, which lets the
-  // runtime passes address a butterfly's points as one base plus uniform
-  // steps (22-30 % fewer VALU instructions per pass); measured, N = 75000
-  // (200 x 375) 0.623 -> 0.587 ms, 140000 (350 x 400) 0.621 -> 0.600 ms, the
-  // multiples of 16 5-60 % slower unpadded
-  if (len % 16) g->pad_shift = 30;
-  const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
   // 16 sub-transforms per workgroup make 128-byte row segments for FP32
   // columns; fewer where 16 would leave one workgroup per CU: measured,
   // N = 2^20 (1024 x 1024) 8 columns at two workgroups per CU 1.26 ms
@@ -302,11 +294,25 @@ static bool fs_geometry(size_t len, size_t elem, size_t cap, LdsGeom* g, int for
   // (transposed_rows) writes its output in segments of as many rows as a
   // workgroup holds and takes 32 where they fit half the LDS (256-byte
   // segments: N = 65536 x 1024 0.461 -> 0.423 ms).
-  int c = 16;
-  while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
-  if (c < 8 && (size_t)8 * g->threads <= 1024 && (size_t)8 * stride * elem <= cap) c = 8;
-  if (transposed_rows && c == 16 && (size_t)32 * g->threads <= 1024 && (size_t)32 * stride * elem <= cap / 2)
-    c = 32;
+  auto count = [&](size_t stride) {
+    int c = 16;
+    while (c > 1 && ((size_t)c * g->threads > 1024 || (size_t)c * stride * elem > cap / 2)) c >>= 1;
+    if (c < 8 && (size_t)8 * g->threads <= 1024 && (size_t)8 * stride * elem <= cap) c = 8;
+    if (transposed_rows && c == 16 && (size_t)32 * g->threads <= 1024 && (size_t)32 * stride * elem <= cap / 2)
+      c = 32;
+    return c;
+  };
+  int c = count((size_t)dev::lds_len((int)len, g->pad_shift));
+  // Sub-lengths that are not multiples of 16 have no pass whose strides
+  // fall on one bank: their tiles are stored unpadded where that keeps the
+  // count per workgroup, which lets the runtime passes address a
+  // butterfly's points as one base plus uniform steps (22-30 % fewer VALU
+  // instructions per pass).  Measured: N = 75000 (200 x 375) 0.614 -> 0.599
+  // ms; multiples of 16 run 5-60 % slower unpadded, and N = 78125 (125 x
+  // 625), whose unpadded rows would fit 16 per workgroup at two workgroups
+  // per CU instead of 8 at three, 2-4 % slower.
+  if (len % 16 && count(len) == c) g->pad_shift = 30;
+  const size_t stride = (size_t)dev::lds_len((int)len, g->pad_shift);
   if (force_c > 0) {  // an autotuner candidate: any power of two that fits
     if ((force_c & (force_c - 1)) || (size_t)force_c * g->threads > 1024) return false;
     c = force_c;
