@@ -95,8 +95,108 @@ __global__ __launch_bounds__(T * TPB) void k_lds_fixed(const V* in, V* out, long
   run_fixed<V, DIR, N, T, PS, 1, Rs...>(gin, lds_all + slot * stride, gout, tid, tw, active, scale);
 }
 
+// ------------------------------------------------------------ split exchange
+// Lengths whose transform fills most of a CU's LDS (one workgroup per CU,
+// so nothing overlaps a transform's loads with another's passes) keep the
+// transform in registers between passes and use LDS only as an exchange
+// buffer of N scalars: each exchange writes the real parts of a pass's
+// outputs, the next pass reads the real parts of its inputs, then the same
+// for the imaginary parts.  Four barriers per exchange instead of two, half
+// the LDS: two workgroups share a CU.
+
+// Butterfly j = tid + b*T of a radix-R pass reads points j + r*(N/R).
+template <int N, int T, int R, typename F>
+__device__ __forceinline__ void split_inputs(int tid, F f) {
+  constexpr int nb = N / R, MAXB = cdiv(nb, T);
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int j = tid + b * T;
+    if (nb % T == 0 || j < nb)
+#pragma unroll
+      for (int r = 0; r < R; ++r) f(b, r, j + r * nb);
+  }
+}
+// ... and writes point r to q*Ls*R + k + r*Ls (k = j mod Ls, q = j / Ls).
+template <int N, int T, int Ls, int R, typename F>
+__device__ __forceinline__ void split_outputs(int tid, F f) {
+  constexpr int nb = N / R, MAXB = cdiv(nb, T);
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int j = tid + b * T;
+    if (nb % T == 0 || j < nb) {
+      const int q = j / Ls, k = j - q * Ls;
+#pragma unroll
+      for (int r = 0; r < R; ++r) f(b, r, q * Ls * R + k + r * Ls);
+    }
+  }
+}
+// Twiddles and butterflies of a radix-R pass at stride Ls on registers.
+template <typename V, int DIR, int N, int T, int Ls, int R>
+__device__ __forceinline__ void split_compute(V (&v)[cdiv(N / R, T)][R], int tid, const V* __restrict__ tw) {
+  constexpr int nb = N / R, MAXB = cdiv(nb, T);
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int j = tid + b * T;
+    if (nb % T == 0 || j < nb) {
+      if constexpr (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + (j % Ls) * (R - 1));
+      Dft<R, DIR, V>::run(v[b]);
+    }
+  }
+}
+
+// Exchange the outputs `prev` of the pass (RP, Ls / RP) into the inputs of
+// the pass (R, Ls), run it, and go on; the last pass stores to HBM.
+template <typename V, int DIR, int N, int T, int PS, int Ls, int RP, int R, int... Rest>
+__device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* lds, V* gout, int tid,
+                                          const V* __restrict__ tw, S<V> scale) {
+  constexpr int LP = Ls / RP;
+  auto at = [](int i) { return i + (i >> PS); };
+  V cur[cdiv(N / R, T)][R];
+  __syncthreads();  // the previous exchange's reads are complete
+  split_outputs<N, T, LP, RP>(tid, [&](int b, int r, int i) { lds[at(i)] = prev[b][r].x; });
+  __syncthreads();
+  split_inputs<N, T, R>(tid, [&](int b, int r, int i) { cur[b][r].x = lds[at(i)]; });
+  __syncthreads();
+  split_outputs<N, T, LP, RP>(tid, [&](int b, int r, int i) { lds[at(i)] = prev[b][r].y; });
+  __syncthreads();
+  split_inputs<N, T, R>(tid, [&](int b, int r, int i) { cur[b][r].y = lds[at(i)]; });
+  split_compute<V, DIR, N, T, Ls, R>(cur, tid, tw);
+  if constexpr (sizeof...(Rest) == 0)
+    split_outputs<N, T, Ls, R>(tid, [&](int b, int r, int i) { gstore(gout + i, cscale(cur[b][r], scale)); });
+  else
+    run_split<V, DIR, N, T, PS, Ls * R, R, Rest...>(cur, lds, gout, tid, tw, scale);
+}
+
+// One transform per workgroup (the grid is the batch), two workgroups per CU
+// (the second launch bound is the waves per SIMD that two workgroups make).
+template <typename V, int DIR, int N, int T, int PS, int R0, int... Rs>
+__global__ __launch_bounds__(T, cdiv(2 * cdiv(T, 64), 4)) void k_lds_fixed_split(const V* in, V* out, long long dist,
+                                                          const V* __restrict__ tw, S<V> scale) {
+  __shared__ S<V> lds[lds_len(N, PS)];
+  const int tid = threadIdx.x;
+  const V* gin = in + (long long)blockIdx.x * dist;
+  V* gout = out + (long long)blockIdx.x * dist;
+  V v[cdiv(N / R0, T)][R0];
+  split_inputs<N, T, R0>(tid, [&](int b, int r, int i) { v[b][r] = gload(gin + i); });
+  split_compute<V, DIR, N, T, 1, R0>(v, tid, tw);
+  run_split<V, DIR, N, T, PS, R0, R0, Rs...>(v, lds, gout, tid, tw, scale);
+}
+
 using FixedLauncher = hipError_t (*)(int dir, const void* in, void* out, size_t batch,
                                      size_t dist, const void* tw, double scale, hipStream_t s);
+
+template <typename V, int N, int T, int PS, int... Rs>
+static hipError_t launch_fixed_split(int dir, const void* in, void* out, size_t batch, size_t dist,
+                                     const void* tw, double scale, hipStream_t s) {
+  const dim3 grid((unsigned)batch), block(T);
+  if (dir == kForward)
+    hipLaunchKernelGGL((k_lds_fixed_split<V, -1, N, T, PS, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
+                       (long long)dist, (const V*)tw, (S<V>)scale);
+  else
+    hipLaunchKernelGGL((k_lds_fixed_split<V, +1, N, T, PS, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
+                       (long long)dist, (const V*)tw, (S<V>)scale);
+  return hipGetLastError();
+}
 
 template <typename V, int N, int T, int TPB, int PS, int... Rs>
 static hipError_t launch_fixed(int dir, const void* in, void* out, size_t batch, size_t dist,
@@ -180,6 +280,8 @@ struct FixedEntry {
 // butterfly index.
 #define FIXED(P, V, N, T, TPB, PS, NP, ...) \
   {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, PS, __VA_ARGS__>, false}
+#define XFIXED(P, V, N, T, PS, NP, ...) \
+  {P, N, T, 1, PS, NP, {__VA_ARGS__}, &launch_fixed_split<V, N, T, PS, __VA_ARGS__>, false}
 #define SFIXED(P, V, N, T, TPB, NP, ...) \
   {P, N, T, TPB, 30, NP, {__VA_ARGS__}, &launch_fixed_staged<V, N, T, TPB, __VA_ARGS__>, true}
 
@@ -254,7 +356,13 @@ static const FixedEntry kFixed[] = {
     SFIXED(kF64, double2, 81, 9, 28, 2, 9, 9),
     FIXED(kF64, double2, 1024, 64, 4, 30, 3, 16, 8, 8),
     FIXED(kF64, double2, 4096, 256, 1, 30, 3, 16, 16, 16),
-    FIXED(kF64, double2, 6720, 960, 1, 4, 5, 8, 8, 7, 5, 3),
+    // FP64 lengths whose transform fills most of a CU's LDS: split exchange
+    // (k_lds_fixed_split), two workgroups per CU: 6720 x 4096 0.216 ->
+    // 0.186 ms (51 -> 59 % of 8 TB/s), 6561 x 8192 0.416 -> 0.349, 8192 x
+    // 4096 0.211 -> 0.181 (the same results to the bit).  The FP32 lengths
+    // of that size (15625, 16384, 16807) spill at the register budget of two
+    // workgroups per CU and ran 17-64 % slower split.
+    XFIXED(kF64, double2, 6720, 480, 4, 5, 8, 8, 7, 5, 3),
     // FP64 forms of the same lengths and 256 / 512 / 2048 / 8192 (the
     // runtime-pass FP64 kernel keeps call frames in scratch): 1.4-2.9x
     // faster, 52-74 % of 8 TB/s
@@ -269,10 +377,11 @@ static const FixedEntry kFixed[] = {
     FIXED(kF64, double2, 2187, 243, 1, 30, 4, 9, 9, 9, 3),
     FIXED(kF64, double2, 2401, 343, 1, 30, 4, 7, 7, 7, 7),
     FIXED(kF64, double2, 3125, 625, 1, 30, 5, 5, 5, 5, 5, 5),
-    FIXED(kF64, double2, 6561, 729, 1, 30, 4, 9, 9, 9, 9),
-    FIXED(kF64, double2, 8192, 1024, 1, 4, 4, 16, 8, 8, 8),  // padded: 2.5 % faster
+    XFIXED(kF64, double2, 6561, 729, 30, 4, 9, 9, 9, 9),
+    XFIXED(kF64, double2, 8192, 512, 4, 4, 16, 8, 8, 8),
 };
 #undef FIXED
+#undef XFIXED
 #undef SFIXED
 
 int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
