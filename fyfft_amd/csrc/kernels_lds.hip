@@ -167,10 +167,10 @@ __device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* 
     run_split<V, DIR, N, T, PS, Ls * R, R, Rest...>(cur, lds, gout, tid, tw, scale);
 }
 
-// One transform per workgroup (the grid is the batch), two workgroups per CU
-// (the second launch bound is the waves per SIMD that two workgroups make).
-template <typename V, int DIR, int N, int T, int PS, int R0, int... Rs>
-__global__ __launch_bounds__(T, cdiv(2 * cdiv(T, 64), 4)) void k_lds_fixed_split(const V* in, V* out, long long dist,
+// One transform per workgroup (the grid is the batch), W workgroups per CU
+// (the second launch bound is the waves per SIMD that W workgroups make).
+template <typename V, int DIR, int N, int T, int PS, int W, int R0, int... Rs>
+__global__ __launch_bounds__(T, cdiv(W * cdiv(T, 64), 4)) void k_lds_fixed_split(const V* in, V* out, long long dist,
                                                           const V* __restrict__ tw, S<V> scale) {
   __shared__ S<V> lds[lds_len(N, PS)];
   const int tid = threadIdx.x;
@@ -185,15 +185,15 @@ __global__ __launch_bounds__(T, cdiv(2 * cdiv(T, 64), 4)) void k_lds_fixed_split
 using FixedLauncher = hipError_t (*)(int dir, const void* in, void* out, size_t batch,
                                      size_t dist, const void* tw, double scale, hipStream_t s);
 
-template <typename V, int N, int T, int PS, int... Rs>
+template <typename V, int N, int T, int PS, int W, int... Rs>
 static hipError_t launch_fixed_split(int dir, const void* in, void* out, size_t batch, size_t dist,
                                      const void* tw, double scale, hipStream_t s) {
   const dim3 grid((unsigned)batch), block(T);
   if (dir == kForward)
-    hipLaunchKernelGGL((k_lds_fixed_split<V, -1, N, T, PS, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
+    hipLaunchKernelGGL((k_lds_fixed_split<V, -1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
                        (long long)dist, (const V*)tw, (S<V>)scale);
   else
-    hipLaunchKernelGGL((k_lds_fixed_split<V, +1, N, T, PS, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
+    hipLaunchKernelGGL((k_lds_fixed_split<V, +1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
                        (long long)dist, (const V*)tw, (S<V>)scale);
   return hipGetLastError();
 }
@@ -280,8 +280,8 @@ struct FixedEntry {
 // butterfly index.
 #define FIXED(P, V, N, T, TPB, PS, NP, ...) \
   {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, PS, __VA_ARGS__>, false}
-#define XFIXED(P, V, N, T, PS, NP, ...) \
-  {P, N, T, 1, PS, NP, {__VA_ARGS__}, &launch_fixed_split<V, N, T, PS, __VA_ARGS__>, false}
+#define XFIXED(P, V, N, T, PS, W, NP, ...) \
+  {P, N, T, 1, PS, NP, {__VA_ARGS__}, &launch_fixed_split<V, N, T, PS, W, __VA_ARGS__>, false}
 #define SFIXED(P, V, N, T, TPB, NP, ...) \
   {P, N, T, TPB, 30, NP, {__VA_ARGS__}, &launch_fixed_staged<V, N, T, TPB, __VA_ARGS__>, true}
 
@@ -355,14 +355,14 @@ static const FixedEntry kFixed[] = {
     SFIXED(kF64, double2, 64, 8, 32, 2, 8, 8),
     SFIXED(kF64, double2, 81, 9, 28, 2, 9, 9),
     FIXED(kF64, double2, 1024, 64, 4, 30, 3, 16, 8, 8),
-    FIXED(kF64, double2, 4096, 256, 1, 30, 3, 16, 16, 16),
+    FIXED(kF64, double2, 4096, 256, 1, 30, 3, 16, 16, 16),  // split exchange at 4 per CU: no faster (74 %)
     // FP64 lengths whose transform fills most of a CU's LDS: split exchange
     // (k_lds_fixed_split), two workgroups per CU: 6720 x 4096 0.216 ->
     // 0.186 ms (51 -> 59 % of 8 TB/s), 6561 x 8192 0.416 -> 0.349, 8192 x
     // 4096 0.211 -> 0.181 (the same results to the bit).  The FP32 lengths
     // of that size (15625, 16384, 16807) spill at the register budget of two
     // workgroups per CU and ran 17-64 % slower split.
-    XFIXED(kF64, double2, 6720, 480, 4, 5, 8, 8, 7, 5, 3),
+    XFIXED(kF64, double2, 6720, 480, 4, 2, 5, 8, 8, 7, 5, 3),
     // FP64 forms of the same lengths and 256 / 512 / 2048 / 8192 (the
     // runtime-pass FP64 kernel keeps call frames in scratch): 1.4-2.9x
     // faster, 52-74 % of 8 TB/s
@@ -377,8 +377,8 @@ static const FixedEntry kFixed[] = {
     FIXED(kF64, double2, 2187, 243, 1, 30, 4, 9, 9, 9, 3),
     FIXED(kF64, double2, 2401, 343, 1, 30, 4, 7, 7, 7, 7),
     FIXED(kF64, double2, 3125, 625, 1, 30, 5, 5, 5, 5, 5, 5),
-    XFIXED(kF64, double2, 6561, 729, 30, 4, 9, 9, 9, 9),
-    XFIXED(kF64, double2, 8192, 512, 4, 4, 16, 8, 8, 8),
+    XFIXED(kF64, double2, 6561, 729, 30, 2, 4, 9, 9, 9, 9),
+    XFIXED(kF64, double2, 8192, 512, 4, 2, 4, 16, 8, 8, 8),
 };
 #undef FIXED
 #undef XFIXED
