@@ -104,29 +104,48 @@ __global__ __launch_bounds__(T * TPB) void k_lds_fixed(const V* in, V* out, long
 // for the imaginary parts.  Four barriers per exchange instead of two, half
 // the LDS: two workgroups share a CU.
 
-// Butterfly j = tid + b*T of a radix-R pass reads points j + r*(N/R).
-template <int N, int T, int R, typename F>
+// Butterfly j = tid + b*T of a radix-R pass reads points i = j + r*(N/R);
+// f(b, r, i, l) also gets the point's padded LDS index l (pad shift PS),
+// formed as one base plus a compile-time step where the strides allow it
+// (then the LDS instructions take the steps as immediate offsets and a
+// thread keeps one address register per butterfly, not one per point).
+template <int N, int T, int R, int PS, typename F>
 __device__ __forceinline__ void split_inputs(int tid, F f) {
   constexpr int nb = N / R, MAXB = cdiv(nb, T);
-#pragma unroll
-  for (int b = 0; b < MAXB; ++b) {
-    const int j = tid + b * T;
-    if (nb % T == 0 || j < nb)
-#pragma unroll
-      for (int r = 0; r < R; ++r) f(b, r, j + r * nb);
-  }
-}
-// ... and writes point r to q*Ls*R + k + r*Ls (k = j mod Ls, q = j / Ls).
-template <int N, int T, int Ls, int R, typename F>
-__device__ __forceinline__ void split_outputs(int tid, F f) {
-  constexpr int nb = N / R, MAXB = cdiv(nb, T);
+  constexpr bool lin = PS >= 30 || nb % (1 << PS) == 0;
+  constexpr int step = PS >= 30 ? nb : nb + (nb >> PS);
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     const int j = tid + b * T;
     if (nb % T == 0 || j < nb) {
-      const int q = j / Ls, k = j - q * Ls;
+      const int pj = PS >= 30 ? j : j + (j >> PS);
 #pragma unroll
-      for (int r = 0; r < R; ++r) f(b, r, q * Ls * R + k + r * Ls);
+      for (int r = 0; r < R; ++r) {
+        const int i = j + r * nb;
+        f(b, r, i, lin ? pj + r * step : i + (i >> PS));
+      }
+    }
+  }
+}
+// ... and writes point r to i = q*Ls*R + k + r*Ls (k = j mod Ls, q = j / Ls).
+template <int N, int T, int Ls, int R, int PS, typename F>
+__device__ __forceinline__ void split_outputs(int tid, F f) {
+  constexpr int nb = N / R, MAXB = cdiv(nb, T);
+  constexpr bool wide = PS < 30 && Ls % (1 << PS) == 0;        // steps of Ls + Ls / 2^PS
+  constexpr bool inside = PS < 30 && (1 << PS) % (Ls * R) == 0;  // the butterfly within one pad block
+  constexpr bool lin = PS >= 30 || wide || inside;
+  constexpr int step = wide ? Ls + (Ls >> PS) : Ls;
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    const int j = tid + b * T;
+    if (nb % T == 0 || j < nb) {
+      const int q = j / Ls, k = j - q * Ls, base = q * Ls * R + k;
+      const int pb = PS >= 30 ? base : base + (base >> PS);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int i = base + r * Ls;
+        f(b, r, i, lin ? pb + r * step : i + (i >> PS));
+      }
     }
   }
 }
@@ -150,19 +169,18 @@ template <typename V, int DIR, int N, int T, int PS, int Ls, int RP, int R, int.
 __device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* lds, V* gout, int tid,
                                           const V* __restrict__ tw, S<V> scale) {
   constexpr int LP = Ls / RP;
-  auto at = [](int i) { return i + (i >> PS); };
   V cur[cdiv(N / R, T)][R];
   __syncthreads();  // the previous exchange's reads are complete
-  split_outputs<N, T, LP, RP>(tid, [&](int b, int r, int i) { lds[at(i)] = prev[b][r].x; });
+  split_outputs<N, T, LP, RP, PS>(tid, [&](int b, int r, int, int l) { lds[l] = prev[b][r].x; });
   __syncthreads();
-  split_inputs<N, T, R>(tid, [&](int b, int r, int i) { cur[b][r].x = lds[at(i)]; });
+  split_inputs<N, T, R, PS>(tid, [&](int b, int r, int, int l) { cur[b][r].x = lds[l]; });
   __syncthreads();
-  split_outputs<N, T, LP, RP>(tid, [&](int b, int r, int i) { lds[at(i)] = prev[b][r].y; });
+  split_outputs<N, T, LP, RP, PS>(tid, [&](int b, int r, int, int l) { lds[l] = prev[b][r].y; });
   __syncthreads();
-  split_inputs<N, T, R>(tid, [&](int b, int r, int i) { cur[b][r].y = lds[at(i)]; });
+  split_inputs<N, T, R, PS>(tid, [&](int b, int r, int, int l) { cur[b][r].y = lds[l]; });
   split_compute<V, DIR, N, T, Ls, R>(cur, tid, tw);
   if constexpr (sizeof...(Rest) == 0)
-    split_outputs<N, T, Ls, R>(tid, [&](int b, int r, int i) { gstore(gout + i, cscale(cur[b][r], scale)); });
+    split_outputs<N, T, Ls, R, PS>(tid, [&](int b, int r, int i, int) { gstore(gout + i, cscale(cur[b][r], scale)); });
   else
     run_split<V, DIR, N, T, PS, Ls * R, R, Rest...>(cur, lds, gout, tid, tw, scale);
 }
@@ -177,7 +195,7 @@ __global__ __launch_bounds__(T, cdiv(W * cdiv(T, 64), 4)) void k_lds_fixed_split
   const V* gin = in + (long long)blockIdx.x * dist;
   V* gout = out + (long long)blockIdx.x * dist;
   V v[cdiv(N / R0, T)][R0];
-  split_inputs<N, T, R0>(tid, [&](int b, int r, int i) { v[b][r] = gload(gin + i); });
+  split_inputs<N, T, R0, PS>(tid, [&](int b, int r, int i, int) { v[b][r] = gload(gin + i); });
   split_compute<V, DIR, N, T, 1, R0>(v, tid, tw);
   run_split<V, DIR, N, T, PS, R0, R0, Rs...>(v, lds, gout, tid, tw, scale);
 }
@@ -298,7 +316,9 @@ static const FixedEntry kFixed[] = {
     FIXED(kF32, float2, 2048, 128, 2, 30, 3, 16, 16, 8),
     FIXED(kF32, float2, 4096, 256, 1, 30, 3, 16, 16, 16),
     FIXED(kF32, float2, 8192, 512, 1, 30, 4, 16, 8, 8, 8),
-    FIXED(kF32, float2, 16384, 1024, 1, 4, 4, 16, 16, 8, 8),  // padded: 2 % faster than unpadded (one workgroup per CU either way)
+    // split exchange, two workgroups per CU: 16384 x 16384 0.941 -> 0.740 ms
+    // (57 -> 72 % of 8 TB/s)
+    XFIXED(kF32, float2, 16384, 1024, 4, 2, 4, 16, 16, 8, 8),
     FIXED(kF32, float2, 6720, 512, 1, 30, 5, 8, 8, 7, 5, 3),
     // odd-radix lengths of the reference's table, one butterfly of the
     // largest radix per thread and pass, unpadded (odd strides spread over
@@ -359,9 +379,10 @@ static const FixedEntry kFixed[] = {
     // FP64 lengths whose transform fills most of a CU's LDS: split exchange
     // (k_lds_fixed_split), two workgroups per CU: 6720 x 4096 0.216 ->
     // 0.186 ms (51 -> 59 % of 8 TB/s), 6561 x 8192 0.416 -> 0.349, 8192 x
-    // 4096 0.211 -> 0.181 (the same results to the bit).  The FP32 lengths
-    // of that size (15625, 16384, 16807) spill at the register budget of two
-    // workgroups per CU and ran 17-64 % slower split.
+    // 4096 0.211 -> 0.181 (the same results to the bit).  FP32 16384 above
+    // likewise; the FP32 odd-radix lengths of that size (15625, 16807,
+    // 19683: 20-27 points per thread at 1024 threads) spill at the register
+    // budget of two workgroups per CU and ran 28-64 % slower split.
     XFIXED(kF64, double2, 6720, 480, 4, 2, 5, 8, 8, 7, 5, 3),
     // FP64 forms of the same lengths and 256 / 512 / 2048 / 8192 (the
     // runtime-pass FP64 kernel keeps call frames in scratch): 1.4-2.9x
