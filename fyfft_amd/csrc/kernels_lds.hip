@@ -11,6 +11,7 @@
 #include <cstdio>
 
 #include "bodies.hpp"
+#include "split.hpp"
 
 namespace fyfft {
 using namespace dev;
@@ -95,111 +96,6 @@ __global__ __launch_bounds__(T * TPB) void k_lds_fixed(const V* in, V* out, long
   run_fixed<V, DIR, N, T, PS, 1, Rs...>(gin, lds_all + slot * stride, gout, tid, tw, active, scale);
 }
 
-// ------------------------------------------------------------ split exchange
-// Lengths whose transform fills most of a CU's LDS (one workgroup per CU,
-// so nothing overlaps a transform's loads with another's passes) keep the
-// transform in registers between passes and use LDS only as an exchange
-// buffer of N scalars: each exchange writes the real parts of a pass's
-// outputs, the next pass reads the real parts of its inputs, then the same
-// for the imaginary parts.  Four barriers per exchange instead of two, half
-// the LDS: two workgroups share a CU.
-
-// Butterfly j = tid + b*T of a radix-R pass reads points i = j + r*(N/R);
-// f(b, r, i, l) also gets the point's padded LDS index l (pad shift PS),
-// formed as one base plus a compile-time step where the strides allow it
-// (then the LDS instructions take the steps as immediate offsets and a
-// thread keeps one address register per butterfly, not one per point).
-template <int N, int T, int R, int PS, typename F>
-__device__ __forceinline__ void split_inputs(int tid, F f) {
-  constexpr int nb = N / R, MAXB = cdiv(nb, T);
-  constexpr bool lin = PS >= 30 || nb % (1 << PS) == 0;
-  constexpr int step = PS >= 30 ? nb : nb + (nb >> PS);
-#pragma unroll
-  for (int b = 0; b < MAXB; ++b) {
-    const int j = tid + b * T;
-    if (nb % T == 0 || j < nb) {
-      const int pj = PS >= 30 ? j : j + (j >> PS);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int i = j + r * nb;
-        f(b, r, i, lin ? pj + r * step : i + (i >> PS));
-      }
-    }
-  }
-}
-// ... and writes point r to i = q*Ls*R + k + r*Ls (k = j mod Ls, q = j / Ls).
-template <int N, int T, int Ls, int R, int PS, typename F>
-__device__ __forceinline__ void split_outputs(int tid, F f) {
-  constexpr int nb = N / R, MAXB = cdiv(nb, T);
-  constexpr bool wide = PS < 30 && Ls % (1 << PS) == 0;        // steps of Ls + Ls / 2^PS
-  constexpr bool inside = PS < 30 && (1 << PS) % (Ls * R) == 0;  // the butterfly within one pad block
-  constexpr bool lin = PS >= 30 || wide || inside;
-  constexpr int step = wide ? Ls + (Ls >> PS) : Ls;
-#pragma unroll
-  for (int b = 0; b < MAXB; ++b) {
-    const int j = tid + b * T;
-    if (nb % T == 0 || j < nb) {
-      const int q = j / Ls, k = j - q * Ls, base = q * Ls * R + k;
-      const int pb = PS >= 30 ? base : base + (base >> PS);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int i = base + r * Ls;
-        f(b, r, i, lin ? pb + r * step : i + (i >> PS));
-      }
-    }
-  }
-}
-// Twiddles and butterflies of a radix-R pass at stride Ls on registers.
-template <typename V, int DIR, int N, int T, int Ls, int R>
-__device__ __forceinline__ void split_compute(V (&v)[cdiv(N / R, T)][R], int tid, const V* __restrict__ tw) {
-  constexpr int nb = N / R, MAXB = cdiv(nb, T);
-#pragma unroll
-  for (int b = 0; b < MAXB; ++b) {
-    const int j = tid + b * T;
-    if (nb % T == 0 || j < nb) {
-      if constexpr (Ls > 1) apply_twiddles<R, DIR>(v[b], tw + (Ls - 1) + (j % Ls) * (R - 1));
-      Dft<R, DIR, V>::run(v[b]);
-    }
-  }
-}
-
-// Exchange the outputs `prev` of the pass (RP, Ls / RP) into the inputs of
-// the pass (R, Ls), run it, and go on; the last pass stores to HBM.
-template <typename V, int DIR, int N, int T, int PS, int Ls, int RP, int R, int... Rest>
-__device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* lds, V* gout, int tid,
-                                          const V* __restrict__ tw, S<V> scale) {
-  constexpr int LP = Ls / RP;
-  V cur[cdiv(N / R, T)][R];
-  __syncthreads();  // the previous exchange's reads are complete
-  split_outputs<N, T, LP, RP, PS>(tid, [&](int b, int r, int, int l) { lds[l] = prev[b][r].x; });
-  __syncthreads();
-  split_inputs<N, T, R, PS>(tid, [&](int b, int r, int, int l) { cur[b][r].x = lds[l]; });
-  __syncthreads();
-  split_outputs<N, T, LP, RP, PS>(tid, [&](int b, int r, int, int l) { lds[l] = prev[b][r].y; });
-  __syncthreads();
-  split_inputs<N, T, R, PS>(tid, [&](int b, int r, int, int l) { cur[b][r].y = lds[l]; });
-  split_compute<V, DIR, N, T, Ls, R>(cur, tid, tw);
-  if constexpr (sizeof...(Rest) == 0)
-    split_outputs<N, T, Ls, R, PS>(tid, [&](int b, int r, int i, int) { gstore(gout + i, cscale(cur[b][r], scale)); });
-  else
-    run_split<V, DIR, N, T, PS, Ls * R, R, Rest...>(cur, lds, gout, tid, tw, scale);
-}
-
-// One transform per workgroup (the grid is the batch), W workgroups per CU
-// (the second launch bound is the waves per SIMD that W workgroups make).
-template <typename V, int DIR, int N, int T, int PS, int W, int R0, int... Rs>
-__global__ __launch_bounds__(T, cdiv(W * cdiv(T, 64), 4)) void k_lds_fixed_split(const V* in, V* out, long long dist,
-                                                          const V* __restrict__ tw, S<V> scale) {
-  __shared__ S<V> lds[lds_len(N, PS)];
-  const int tid = threadIdx.x;
-  const V* gin = in + (long long)blockIdx.x * dist;
-  V* gout = out + (long long)blockIdx.x * dist;
-  V v[cdiv(N / R0, T)][R0];
-  split_inputs<N, T, R0, PS>(tid, [&](int b, int r, int i, int) { v[b][r] = gload(gin + i); });
-  split_compute<V, DIR, N, T, 1, R0>(v, tid, tw);
-  run_split<V, DIR, N, T, PS, R0, R0, Rs...>(v, lds, gout, tid, tw, scale);
-}
-
 using FixedLauncher = hipError_t (*)(int dir, const void* in, void* out, size_t batch,
                                      size_t dist, const void* tw, double scale, hipStream_t s);
 
@@ -208,10 +104,10 @@ static hipError_t launch_fixed_split(int dir, const void* in, void* out, size_t 
                                      const void* tw, double scale, hipStream_t s) {
   const dim3 grid((unsigned)batch), block(T);
   if (dir == kForward)
-    hipLaunchKernelGGL((k_lds_fixed_split<V, -1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
+    hipLaunchKernelGGL((k_lds_fixed_split<V, V, -1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
                        (long long)dist, (const V*)tw, (S<V>)scale);
   else
-    hipLaunchKernelGGL((k_lds_fixed_split<V, +1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
+    hipLaunchKernelGGL((k_lds_fixed_split<V, V, +1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,
                        (long long)dist, (const V*)tw, (S<V>)scale);
   return hipGetLastError();
 }

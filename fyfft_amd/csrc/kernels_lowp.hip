@@ -5,6 +5,7 @@
 // (4 in, 4 out); arithmetic, twiddles and LDS stay FP32, so the only
 // rounding beyond FP32 level is the final store.
 #include "bodies.hpp"
+#include "split.hpp"
 
 namespace fyfft {
 using namespace dev;
@@ -62,6 +63,28 @@ static hipError_t launch_lowp_n(bool bf16, int dir, const void* in, void* out, s
               : launch_lowp_h<_Float16, N, T, TPB, PS, Rs...>(dir, in, out, batch, tw, scale, s);
 }
 
+// The split-exchange form (split.hpp): registers between passes, LDS an
+// exchange buffer of N floats, W workgroups per CU.
+template <typename H, int N, int T, int PS, int W, int... Rs>
+static hipError_t launch_lowp_split_h(int dir, const void* in, void* out, size_t batch, const void* tw,
+                                      double scale, hipStream_t s) {
+  const dim3 grid((unsigned)batch), block(T);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_lds_fixed_split<float2, cplx16<H>, -1, N, T, PS, W, Rs...>), grid, block, 0, s,
+                       (const cplx16<H>*)in, (cplx16<H>*)out, (long long)N, (const float2*)tw, (float)scale);
+  else
+    hipLaunchKernelGGL((k_lds_fixed_split<float2, cplx16<H>, +1, N, T, PS, W, Rs...>), grid, block, 0, s,
+                       (const cplx16<H>*)in, (cplx16<H>*)out, (long long)N, (const float2*)tw, (float)scale);
+  return hipGetLastError();
+}
+
+template <int N, int T, int PS, int W, int... Rs>
+static hipError_t launch_lowp_split_n(bool bf16, int dir, const void* in, void* out, size_t batch, const void* tw,
+                                      double scale, hipStream_t s) {
+  return bf16 ? launch_lowp_split_h<__bf16, N, T, PS, W, Rs...>(dir, in, out, batch, tw, scale, s)
+              : launch_lowp_split_h<_Float16, N, T, PS, W, Rs...>(dir, in, out, batch, tw, scale, s);
+}
+
 struct LowpEntry {
   int n, npass;
   int radix[6];
@@ -69,15 +92,22 @@ struct LowpEntry {
 };
 
 // Geometry and pass lists of the FP32 compile-time kernels; the radix lists
-// equal radices_for(N) so the plan's FP32 Stockham table matches.
+// equal radices_for(N) so the plan's FP32 Stockham table matches.  4096,
+// 8192 and 16384 take the split-exchange form (registers between passes, LDS
+// an exchange buffer of N floats) at 8 / 4 / 2 workgroups per CU: BF16
+// 4096 x 65536 0.484 -> 0.451 ms, 8192 x 32768 0.620 -> 0.505, 16384 x 16384
+// 0.741 -> 0.581 (the same results to the bit; 6 workgroups per CU for 4096
+// measured the same as 8).
 #define LOWP(N, T, TPB, PS, NP, ...) {N, NP, {__VA_ARGS__}, &launch_lowp_n<N, T, TPB, PS, __VA_ARGS__>}
+#define LOWPX(N, T, PS, W, NP, ...) {N, NP, {__VA_ARGS__}, &launch_lowp_split_n<N, T, PS, W, __VA_ARGS__>}
 static const LowpEntry kLowp[] = {
     LOWP(256, 16, 16, 30, 2, 16, 16),      LOWP(512, 32, 8, 30, 3, 8, 8, 8),
     LOWP(1024, 64, 4, 30, 3, 16, 8, 8),    LOWP(2048, 128, 2, 30, 3, 16, 16, 8),
-    LOWP(4096, 256, 1, 30, 3, 16, 16, 16), LOWP(8192, 512, 1, 30, 4, 16, 8, 8, 8),
-    LOWP(16384, 1024, 1, 30, 4, 16, 16, 8, 8),
+    LOWPX(4096, 256, 4, 8, 3, 16, 16, 16), LOWPX(8192, 512, 4, 4, 4, 16, 8, 8, 8),
+    LOWPX(16384, 1024, 4, 2, 4, 16, 16, 8, 8),
 };
 #undef LOWP
+#undef LOWPX
 
 bool lowp_fixed_supported(size_t n, dev::PassDesc* pd) {
   for (const LowpEntry& e : kLowp)
