@@ -1616,9 +1616,10 @@ static fyfft_status create_plan_impl(fyfft_plan** out, size_t n, size_t batch, f
         const LdsGeom& q = p->lds.geom;
         p->desc = std::string(p->lds.fixed >= 0 ? "lds-fixed" : "lds-generic") +
                   (tuned ? "(tuned)" : "") + " N=" +
-                  std::to_string(n) + " radices=" + radix_string(p->lds.pd) +
-                  " T=" + std::to_string(q.threads) + " TPB=" + std::to_string(q.per_block) +
-                  " pad=" + std::to_string(q.pad_shift);
+                  std::to_string(n) + " radices=" + radix_string(p->lds.pd) + " T=" + std::to_string(q.threads) +
+                  (fixed_kernel_split_w(p->lds.fixed) > 0
+                       ? " split-exchange W=" + std::to_string(fixed_kernel_split_w(p->lds.fixed))
+                       : " TPB=" + std::to_string(q.per_block) + " pad=" + std::to_string(q.pad_shift));
       }
     } else {
       p->kind = kFourStep;

@@ -65,6 +65,9 @@ hipError_t launch_lds(int prec, int dir, const LdsLaunch& L, const void* in, voi
 // Index of a compile-time specialised kernel for (prec, radices), or -1; on
 // a hit the geometry is replaced by the kernel's own.
 int find_fixed_kernel(int prec, const dev::PassDesc& pd, LdsGeom* g);
+// Workgroups per CU of a compile-time kernel of the split-exchange form, 0
+// for the other forms.
+int fixed_kernel_split_w(int idx);
 
 // --- kernels_fourstep.hip
 enum FsMode {

@@ -182,6 +182,7 @@ struct FixedEntry {
   int radix[6];
   FixedLauncher launch;
   bool staged;  // k_lds_fixed_staged: transforms stored back to back (dist == n) only
+  int split_w;  // k_lds_fixed_split with split_w workgroups per CU, 0 otherwise
 };
 
 // PS = 30 stores the transform unpadded: for these power-of-two radix-8/16
@@ -193,11 +194,14 @@ struct FixedEntry {
 // transforms.  Thread counts need not divide N: every pass guards its
 // butterfly index.
 #define FIXED(P, V, N, T, TPB, PS, NP, ...) \
-  {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, PS, __VA_ARGS__>, false}
-#define XFIXED(P, V, N, T, PS, W, NP, ...) \
-  {P, N, T, 1, PS, NP, {__VA_ARGS__}, &launch_fixed_split<V, N, T, PS, W, __VA_ARGS__>, false}
+  {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed<V, N, T, TPB, PS, __VA_ARGS__>, false, 0}
+// Split-exchange entries: the geometry (T, TPB, PS) the plan records (which
+// the real and spectral kernels of the same length reuse), then the split
+// kernel's own pad shift KPS and workgroups per CU W.
+#define XFIXED(P, V, N, T, TPB, PS, KPS, W, NP, ...) \
+  {P, N, T, TPB, PS, NP, {__VA_ARGS__}, &launch_fixed_split<V, N, T, KPS, W, __VA_ARGS__>, false, W}
 #define SFIXED(P, V, N, T, TPB, NP, ...) \
-  {P, N, T, TPB, 30, NP, {__VA_ARGS__}, &launch_fixed_staged<V, N, T, TPB, __VA_ARGS__>, true}
+  {P, N, T, TPB, 30, NP, {__VA_ARGS__}, &launch_fixed_staged<V, N, T, TPB, __VA_ARGS__>, true, 0}
 
 // Radix lists here must equal radices_for(N) (planner.cpp) so that the plan's
 // twiddle table matches; find_fixed_kernel checks this at plan time.  The
@@ -209,12 +213,15 @@ static const FixedEntry kFixed[] = {
     FIXED(kF32, float2, 256, 16, 16, 30, 2, 16, 16),
     FIXED(kF32, float2, 512, 32, 8, 30, 3, 8, 8, 8),
     FIXED(kF32, float2, 1024, 64, 4, 30, 3, 16, 8, 8),
-    FIXED(kF32, float2, 2048, 128, 2, 30, 3, 16, 16, 8),
-    FIXED(kF32, float2, 4096, 256, 1, 30, 3, 16, 16, 16),
-    FIXED(kF32, float2, 8192, 512, 1, 30, 4, 16, 8, 8, 8),
-    // split exchange, two workgroups per CU: 16384 x 16384 0.941 -> 0.740 ms
-    // (57 -> 72 % of 8 TB/s)
-    XFIXED(kF32, float2, 16384, 1024, 4, 2, 4, 16, 16, 8, 8),
+    // 2048 ... 16384 through the split-exchange kernel (registers between
+    // passes, an exchange buffer of N floats, W workgroups per CU): 2048 x
+    // 131072 0.712 -> 0.698 ms (75 -> 77 % of 8 TB/s), 4096 x 65536 0.723 ->
+    // 0.707 (74 -> 76 %), 8192 x 32768 0.768 -> 0.719 (70 -> 75 %), 16384 x
+    // 16384 0.941 -> 0.740 (57 -> 72 %); the same results to the bit
+    XFIXED(kF32, float2, 2048, 128, 2, 30, 4, 16, 3, 16, 16, 8),
+    XFIXED(kF32, float2, 4096, 256, 1, 30, 4, 8, 3, 16, 16, 16),
+    XFIXED(kF32, float2, 8192, 512, 1, 30, 4, 4, 4, 16, 8, 8, 8),
+    XFIXED(kF32, float2, 16384, 1024, 1, 4, 4, 2, 4, 16, 16, 8, 8),
     FIXED(kF32, float2, 6720, 512, 1, 30, 5, 8, 8, 7, 5, 3),
     // odd-radix lengths of the reference's table, one butterfly of the
     // largest radix per thread and pass, unpadded (odd strides spread over
@@ -279,7 +286,7 @@ static const FixedEntry kFixed[] = {
     // likewise; the FP32 odd-radix lengths of that size (15625, 16807,
     // 19683: 20-27 points per thread at 1024 threads) spill at the register
     // budget of two workgroups per CU and ran 28-64 % slower split.
-    XFIXED(kF64, double2, 6720, 480, 4, 2, 5, 8, 8, 7, 5, 3),
+    XFIXED(kF64, double2, 6720, 480, 1, 4, 4, 2, 5, 8, 8, 7, 5, 3),
     // FP64 forms of the same lengths and 256 / 512 / 2048 / 8192 (the
     // runtime-pass FP64 kernel keeps call frames in scratch): 1.4-2.9x
     // faster, 52-74 % of 8 TB/s
@@ -294,12 +301,16 @@ static const FixedEntry kFixed[] = {
     FIXED(kF64, double2, 2187, 243, 1, 30, 4, 9, 9, 9, 3),
     FIXED(kF64, double2, 2401, 343, 1, 30, 4, 7, 7, 7, 7),
     FIXED(kF64, double2, 3125, 625, 1, 30, 5, 5, 5, 5, 5, 5),
-    XFIXED(kF64, double2, 6561, 729, 30, 2, 4, 9, 9, 9, 9),
-    XFIXED(kF64, double2, 8192, 512, 4, 2, 4, 16, 8, 8, 8),
+    XFIXED(kF64, double2, 6561, 729, 1, 30, 30, 2, 4, 9, 9, 9, 9),
+    XFIXED(kF64, double2, 8192, 512, 1, 4, 4, 2, 4, 16, 8, 8, 8),
 };
 #undef FIXED
 #undef XFIXED
 #undef SFIXED
+
+int fixed_kernel_split_w(int idx) {
+  return idx >= 0 && idx < (int)(sizeof(kFixed) / sizeof(kFixed[0])) ? kFixed[idx].split_w : 0;
+}
 
 int find_fixed_kernel(int prec, const PassDesc& pd, LdsGeom* g) {
   for (int i = 0; i < (int)(sizeof(kFixed) / sizeof(kFixed[0])); ++i) {
