@@ -74,10 +74,11 @@ __device__ __forceinline__ void split_compute(V (&v)[cdiv(N / R, T)][R], int tid
 }
 
 // Exchange the outputs `prev` of the pass (RP, Ls / RP) into the inputs of
-// the pass (R, Ls), run it, and go on; the last pass stores to HBM.
-template <typename V, typename E, int DIR, int N, int T, int PS, int Ls, int RP, int R, int... Rest>
-__device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* lds, E* gout, int tid,
-                                          const V* __restrict__ tw, S<V> scale) {
+// the pass (R, Ls), run it, and go on; the last pass hands its outputs to
+// the sink dst(i, v).
+template <typename V, typename Dst, int DIR, int N, int T, int PS, int Ls, int RP, int R, int... Rest>
+__device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* lds, const Dst& dst, int tid,
+                                          const V* __restrict__ tw) {
   constexpr int LP = Ls / RP;
   V cur[cdiv(N / R, T)][R];
   __syncthreads();  // the previous exchange's reads are complete
@@ -90,11 +91,34 @@ __device__ __forceinline__ void run_split(V (&prev)[cdiv(N / RP, T)][RP], S<V>* 
   split_inputs<N, T, R, PS>(tid, [&](int b, int r, int, int l) { cur[b][r].y = lds[l]; });
   split_compute<V, DIR, N, T, Ls, R>(cur, tid, tw);
   if constexpr (sizeof...(Rest) == 0)
-    split_outputs<N, T, Ls, R, PS>(tid, [&](int b, int r, int i, int) {
-      Elem<V, E>::stnt(gout + i, cscale(cur[b][r], scale));
-    });
+    split_outputs<N, T, Ls, R, PS>(tid, [&](int b, int r, int i, int) { dst(i, cur[b][r]); });
   else
-    run_split<V, E, DIR, N, T, PS, Ls * R, R, Rest...>(cur, lds, gout, tid, tw, scale);
+    run_split<V, Dst, DIR, N, T, PS, Ls * R, R, Rest...>(cur, lds, dst, tid, tw);
+}
+
+// A whole transform: the first pass reads its points from the source (a
+// split source, kSplit, issues every load of the butterfly before the first
+// use), the last pass writes through the sink.  Every thread of the
+// workgroup calls it (barriers).
+template <typename V, int DIR, int N, int T, int PS, typename Src, typename Dst, int R0, int... Rs>
+__device__ __forceinline__ void split_fft_io(const Src& src, const Dst& dst, S<V>* lds, int tid,
+                                             const V* __restrict__ tw) {
+  constexpr int nb = N / R0;
+  V v[cdiv(nb, T)][R0];
+  if constexpr (SplitSrc<Src>::value) {
+    typename Src::Raw raw[cdiv(nb, T)][R0];
+    split_inputs<N, T, R0, PS>(tid, [&](int b, int r, int i, int) { raw[b][r] = src.load(i - r * nb, r * nb); });
+    split_inputs<N, T, R0, PS>(tid, [&](int b, int r, int i, int) {
+      v[b][r] = src.finish(raw[b][r], i - r * nb, r * nb);
+    });
+  } else {
+    split_inputs<N, T, R0, PS>(tid, [&](int b, int r, int i, int) { v[b][r] = src(i); });
+  }
+  split_compute<V, DIR, N, T, 1, R0>(v, tid, tw);
+  if constexpr (sizeof...(Rs) == 0)
+    split_outputs<N, T, 1, R0, PS>(tid, [&](int b, int r, int i, int) { dst(i, v[b][r]); });
+  else
+    run_split<V, Dst, DIR, N, T, PS, R0, R0, Rs...>(v, lds, dst, tid, tw);
 }
 
 // One transform per workgroup (the grid is the batch), W workgroups per CU
@@ -105,13 +129,10 @@ template <typename V, typename E, int DIR, int N, int T, int PS, int W, int R0, 
 __global__ __launch_bounds__(T, cdiv(W * cdiv(T, 64), 4)) void k_lds_fixed_split(const E* in, E* out, long long dist,
                                                           const V* __restrict__ tw, S<V> scale) {
   __shared__ S<V> lds[lds_len(N, PS)];
-  const int tid = threadIdx.x;
-  const E* gin = in + (long long)blockIdx.x * dist;
-  E* gout = out + (long long)blockIdx.x * dist;
-  V v[cdiv(N / R0, T)][R0];
-  split_inputs<N, T, R0, PS>(tid, [&](int b, int r, int i, int) { v[b][r] = Elem<V, E>::ldnt(gin + i); });
-  split_compute<V, DIR, N, T, 1, R0>(v, tid, tw);
-  run_split<V, E, DIR, N, T, PS, R0, R0, Rs...>(v, lds, gout, tid, tw, scale);
+  const long long off = (long long)blockIdx.x * dist;
+  split_fft_io<V, DIR, N, T, PS, GlobalSrc<V, E>, GlobalSink<V, E>, R0, Rs...>(
+      GlobalSrc<V, E>{(gbl_cptr<E>)(in + off)}, GlobalSink<V, E>{(gbl_ptr<E>)(out + off), scale}, lds,
+      (int)threadIdx.x, tw);
 }
 
 }  // namespace fyfft
