@@ -217,8 +217,10 @@ static const FixedEntry kFixed[] = {
     // passes, an exchange buffer of N floats, W workgroups per CU): 2048 x
     // 131072 0.712 -> 0.698 ms (75 -> 77 % of 8 TB/s), 4096 x 65536 0.723 ->
     // 0.707 (74 -> 76 %), 8192 x 32768 0.768 -> 0.719 (70 -> 75 %), 16384 x
-    // 16384 0.941 -> 0.740 (57 -> 72 %); the same results to the bit
-    XFIXED(kF32, float2, 2048, 128, 2, 30, 4, 16, 3, 16, 16, 8),
+    // 16384 0.941 -> 0.740 (57 -> 72 %); the same results to the bit.  2048
+    // at 12 workgroups per CU (no scratch; at 16 it spills 8 bytes and ran
+    // 3 % slower); 4096 at 6 per CU measured the same as at 8
+    XFIXED(kF32, float2, 2048, 128, 2, 30, 4, 12, 3, 16, 16, 8),
     XFIXED(kF32, float2, 4096, 256, 1, 30, 4, 8, 3, 16, 16, 16),
     XFIXED(kF32, float2, 8192, 512, 1, 30, 4, 4, 4, 16, 8, 8, 8),
     XFIXED(kF32, float2, 16384, 1024, 1, 4, 4, 2, 4, 16, 16, 8, 8),
