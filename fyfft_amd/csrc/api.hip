@@ -2413,7 +2413,15 @@ static fyfft_status create_real_impl(fyfft_plan** out, size_t n, size_t batch, f
   }
   static const char* kForms[] = {"fused half-length LDS", "fused full-length LDS",
                                  "half-length transform + pre/post pass", "full-length transform + expand pass"};
-  p->desc = "real N=" + std::to_string(n) + " form=" + kForms[p->real_form] + " [" + sp->desc + "]";
+  // the fused forms run their own kernels on the sub-plan's geometry and
+  // tables: describe those rather than the sub-plan's C2C kernel
+  const LdsGeom& q = sp->lds.geom;
+  const std::string sub =
+      fused ? std::string(sp->lds.fixed >= 0 ? "compile-time" : "runtime") + " passes N=" + std::to_string(len) +
+                  " radices=" + radix_string(sp->lds.pd) + " T=" + std::to_string(q.threads) +
+                  " TPB=" + std::to_string(q.per_block) + " pad=" + std::to_string(q.pad_shift)
+            : sp->desc;
+  p->desc = "real N=" + std::to_string(n) + " form=" + kForms[p->real_form] + " [" + sub + "]";
   *out = p;
   return FYFFT_SUCCESS;
 }
