@@ -102,6 +102,7 @@ using FixedLauncher = hipError_t (*)(int dir, const void* in, void* out, size_t 
 template <typename V, int N, int T, int PS, int W, int... Rs>
 static hipError_t launch_fixed_split(int dir, const void* in, void* out, size_t batch, size_t dist,
                                      const void* tw, double scale, hipStream_t s) {
+  if (batch > 0xffffffffull) return hipErrorInvalidValue;  // one workgroup per transform
   const dim3 grid((unsigned)batch), block(T);
   if (dir == kForward)
     hipLaunchKernelGGL((k_lds_fixed_split<V, V, -1, N, T, PS, W, Rs...>), grid, block, 0, s, (const V*)in, (V*)out,

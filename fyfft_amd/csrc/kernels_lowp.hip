@@ -68,6 +68,7 @@ static hipError_t launch_lowp_n(bool bf16, int dir, const void* in, void* out, s
 template <typename H, int N, int T, int PS, int W, int... Rs>
 static hipError_t launch_lowp_split_h(int dir, const void* in, void* out, size_t batch, const void* tw,
                                       double scale, hipStream_t s) {
+  if (batch > 0xffffffffull) return hipErrorInvalidValue;  // one workgroup per transform
   const dim3 grid((unsigned)batch), block(T);
   if (dir < 0)
     hipLaunchKernelGGL((k_lds_fixed_split<float2, cplx16<H>, -1, N, T, PS, W, Rs...>), grid, block, 0, s,
