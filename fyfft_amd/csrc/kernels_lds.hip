@@ -214,15 +214,15 @@ static const FixedEntry kFixed[] = {
     FIXED(kF32, float2, 256, 16, 16, 30, 2, 16, 16),
     FIXED(kF32, float2, 512, 32, 8, 30, 3, 8, 8, 8),
     FIXED(kF32, float2, 1024, 64, 4, 30, 3, 16, 8, 8),
-    // 2048 ... 16384 through the split-exchange kernel (registers between
-    // passes, an exchange buffer of N floats, W workgroups per CU): 2048 x
-    // 131072 0.712 -> 0.698 ms (75 -> 77 % of 8 TB/s), 4096 x 65536 0.723 ->
-    // 0.707 (74 -> 76 %), 8192 x 32768 0.768 -> 0.719 (70 -> 75 %), 16384 x
-    // 16384 0.941 -> 0.740 (57 -> 72 %); the same results to the bit.  2048
-    // at 12 workgroups per CU (no scratch; at 16 it spills 8 bytes and ran
-    // 3 % slower); 4096 at 6 per CU measured the same as at 8
-    XFIXED(kF32, float2, 2048, 128, 2, 30, 4, 12, 3, 16, 16, 8),
-    XFIXED(kF32, float2, 4096, 256, 1, 30, 4, 8, 3, 16, 16, 16),
+    FIXED(kF32, float2, 2048, 128, 2, 30, 3, 16, 16, 8),
+    FIXED(kF32, float2, 4096, 256, 1, 30, 3, 16, 16, 16),
+    // 8192 and 16384 through the split-exchange kernel (registers between
+    // passes, an exchange buffer of N floats, W workgroups per CU): 8192 x
+    // 32768 0.755 -> 0.736 ms (71 -> 73 % of 8 TB/s), 16384 x 16384 0.937 ->
+    // 0.743 (57 -> 72 %); the same results to the bit.  2048 and 4096 keep
+    // the whole transform in LDS: split (12 / 8 workgroups per CU) they ran
+    // faster on one host (0.712 -> 0.698, 0.723 -> 0.707 ms) and slower on
+    // the others (0.719 -> 0.755, 0.730 -> 0.741 ms)
     XFIXED(kF32, float2, 8192, 512, 1, 30, 4, 4, 4, 16, 8, 8, 8),
     XFIXED(kF32, float2, 16384, 1024, 1, 4, 4, 2, 4, 16, 16, 8, 8),
     FIXED(kF32, float2, 6720, 512, 1, 30, 5, 8, 8, 7, 5, 3),
