@@ -222,7 +222,8 @@ static const FixedEntry kFixed[] = {
     // 0.743 (57 -> 72 %); the same results to the bit.  2048 and 4096 keep
     // the whole transform in LDS: split (12 / 8 workgroups per CU) they ran
     // faster on one host (0.712 -> 0.698, 0.723 -> 0.707 ms) and slower on
-    // the others (0.719 -> 0.755, 0.730 -> 0.741 ms)
+    // the others (0.719 -> 0.755, 0.730 -> 0.741 ms); 8192 at 3 workgroups
+    // per CU measured the same as at 4
     XFIXED(kF32, float2, 8192, 512, 1, 30, 4, 4, 4, 16, 8, 8, 8),
     XFIXED(kF32, float2, 16384, 1024, 1, 4, 4, 2, 4, 16, 16, 8, 8),
     FIXED(kF32, float2, 6720, 512, 1, 30, 5, 8, 8, 7, 5, 3),
